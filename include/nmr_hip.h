@@ -1,0 +1,144 @@
+/*
+ * nmr_hip.h -- C ABI of the MI355X (gfx950) Bloch-Torrey solver library
+ * libnmr_hip.so.  Plain C: POD structs, raw host pointers, int status codes.
+ * Nothing here throws or exits; every entry point returns NMR_OK (0) or a
+ * negative status whose text nmr_hip_strerror() gives, with a detail line
+ * from nmr_hip_last_error() (per thread).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   nmr_hip_create          NmrRelaxationSolve::InitializeGpu()
+ *                           src/NmrRelaxationSolveKernels.cu:12-88 (27 cudaMalloc,
+ *                           21 H2D copies); the coefficient arrays it uploads
+ *                           (D, T1, T2, mainField, noiseField) are derived
+ *                           on the device from the mask and these scalars
+ *                           (src/NmrRelaxationSolve.cpp:1756-1989)
+ *   nmr_hip_upload_*        the H2D copies of InitializeGpu (cu:63-87)
+ *   nmr_hip_run             the per-step loop body of solve_gpu()
+ *                           (cu:1187-1227): ExplicitMakUpgrade (cu:1473-1501) or
+ *                           ExplicitSimple (src/NmrRelaxationSolve.cpp:3148-3282),
+ *                           BoundaryConditionsForVectorField_gpu (cu:1321-1396),
+ *                           OutputX/Y/Z (cpp:3301-3461) every step
+ *   nmr_hip_download_magnetization  the D2H copies of solve_gpu (cu:1195-1197)
+ *   nmr_hip_destroy         cudaDeviceReset at the end of solve_gpu (cu:1316)
+ *   status codes            gpuAssert/gpuErrchk (headers/NmrRelaxationSolve.h:12-27),
+ *                           which print and exit(); here the caller decides.
+ *
+ * Threading: one context per host thread; a context binds the HIP device
+ * current at create time and owns one HIP stream; calls return after the
+ * work they enqueue has completed.  Not thread-safe per context.
+ */
+#ifndef NMR_HIP_H
+#define NMR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define NMR_OK 0
+#define NMR_ERR_INVALID (-1)     /* bad dimension / option / size */
+#define NMR_ERR_NULL (-2)        /* required pointer is NULL */
+#define NMR_ERR_DEVICE (-3)      /* HIP runtime or launch failure */
+#define NMR_ERR_NOMEM (-4)       /* device allocation failed */
+#define NMR_ERR_STATE (-5)       /* call out of order (e.g. run before upload) */
+#define NMR_ERR_UNSUPPORTED (-6) /* option combination not implemented */
+
+/* ---- physics switches (nVectorMult, nVelocity, nDiffusion of the reference) ---- */
+#define NMR_F_DIFFUSION 1
+#define NMR_F_VELOCITY 2
+#define NMR_F_PRECESSION 4
+
+/* ---- host array layouts accepted by upload/download ---- */
+/* COMPACT: interior only, x fastest.  mask uint8 0/1 [nz][ny][nx];
+ *   magnetization double [nz][ny][nx];
+ *   face velocities double Vx [nz][ny][nx+1], Vy [nz][ny+1][nx], Vz [nz+1][ny][nx]
+ *   (Vx[k][j][i] is the face between cells i-1 and i).
+ * PADDED: the reference's arrays, (nx+2)(ny+2)(nz+2) with one ghost cell per
+ *   side, idx = i + (nx+2)*(j + (ny+2)*k); mask as chars '0'/'1';
+ *   velocity.x[idx(i,j,k)] is the face between i-1 and i
+ *   (src/NmrRelaxationSolve.cpp:1435-1662, 3284-3287). */
+#define NMR_LAYOUT_COMPACT 0
+#define NMR_LAYOUT_PADDED 1
+
+/* ---- update scheme ---- */
+#define NMR_SCHEME_UPWIND 0      /* ExplicitSimple, cpp:3148-3282 */
+
+/* ---- gradient precession form ---- */
+#define NMR_PHASE_LINEAR 0       /* M += dt*g*(M x Bg), as cpp:3230-3235 */
+#define NMR_PHASE_EXACT 1        /* (Mx + iMy) *= exp(-i*dt*g*Bg_z) (ncomp 2 only) */
+
+typedef struct nmr_problem {
+    int32_t nx, ny, nz;     /* interior voxels (ik, jk, kk) */
+    int32_t ncomp;          /* 1: Mx; 2: Mx,My; 3: Mx,My,Mz */
+    int32_t flags;          /* NMR_F_* */
+    int32_t scheme;         /* NMR_SCHEME_* */
+    int32_t phase_mode;     /* NMR_PHASE_* */
+    int32_t buffer[3];      /* signal trim per side (bufferSizeX/Y/Z) */
+    int32_t reserved;
+    double dx, dy, dz;      /* voxel size (m) */
+    double dt;              /* time step (s) */
+    double diffusivity;     /* pore diffusivity D_WAT (m^2/s) */
+    double r1_bulk;         /* 1/T1_WAT (1/s) */
+    double r2_bulk;         /* 1/T2_WAT (1/s) */
+    double rho1, rho2;      /* surface relaxivities R1_WAT, R2_WAT (m/s) */
+    double gamma;           /* g (reference: 4.78941714e7) */
+    double p_eq;            /* equilibrium magnetization pEq */
+    double b0_dir[3];       /* main field (uniform); normalised internally */
+    double grad[3];         /* constant gradient (T/m) used when no schedule */
+} nmr_problem;
+
+typedef struct nmr_ctx nmr_ctx;
+
+/* Validate the problem, allocate all device memory, bind the current device. */
+int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out);
+/* Free everything the context owns.  NULL is accepted (no-op). */
+int nmr_hip_destroy(nmr_ctx* ctx);
+
+int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout);
+/* Any of vx/vy/vz may be NULL (zero velocity on that axis). */
+int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy,
+                            const double* vz, int32_t layout);
+/* Components beyond ncomp are ignored and may be NULL. */
+int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* my,
+                                 const double* mz, int32_t layout);
+/* Components beyond ncomp are not written (may be NULL).  PADDED output
+ * fills the ghost layer with the zero-derivative copy the reference's
+ * BoundaryConditionsForVectorField leaves there. */
+int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double* mz,
+                                   int32_t layout);
+
+/* Advance nsteps time steps.  grad_sched (nullable): [nsteps][3] gradient
+ * amplitudes (T/m) in force during each step; NULL uses prob->grad.
+ * signal (nullable): [(nsteps+1)][ncomp] pore sums over the trimmed
+ * interior; row 0 is the state before the first step, row n the state
+ * after n updates. */
+int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal);
+
+/* Pore sums of the current state: out[ncomp]. */
+int nmr_hip_signal(nmr_ctx* ctx, double* out);
+
+/* GPU time (ms) between HIP events recorded on the context's stream around
+ * the kernels of the last nmr_hip_run, and the number of sweep launches. */
+int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
+
+/* Tile shape of the sweep kernel: variant index (-1 = automatic) and z-chunk
+ * length (0 = automatic).  nmr_hip_num_variants() gives the variant count;
+ * nmr_hip_variant_info() describes one (lanes per row, rows per thread). */
+int nmr_hip_set_tile(nmr_ctx* ctx, int32_t variant, int32_t lz);
+int nmr_hip_num_variants(void);
+int nmr_hip_variant_info(int32_t variant, int32_t* lanes_x, int32_t* rows_per_thread,
+                         int32_t* tile_x, int32_t* tile_y);
+/* Bytes of device memory the context holds. */
+int nmr_hip_device_bytes(nmr_ctx* ctx, int64_t* bytes);
+
+const char* nmr_hip_strerror(int code);
+const char* nmr_hip_last_error(void);
+int nmr_hip_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NMR_HIP_H */

@@ -1,0 +1,633 @@
+// nmr_abi.hip -- host side of the C ABI declared in include/nmr_hip.h.
+//
+// Owns the device state of one solve: ping-pong FP32 magnetization, the
+// uint8 pore mask, FP32 staggered face velocities, the per-step signal
+// partials and the signal curve.  Replaces the reference's InitializeGpu
+// (src/NmrRelaxationSolveKernels.cu:12-88) and the per-step loop body of
+// solve_gpu (cu:1187-1227); every HIP call is checked and turned into a
+// status code instead of gpuAssert's exit() (headers/NmrRelaxationSolve.h:12-27).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "nmr_hip.h"
+#include "nmr_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+constexpr int kChunkSteps = 512; // steps whose partials are held before a reduction
+
+int round_up4(int n) { return (n + 3) & ~3; }
+
+} // namespace
+
+struct nmr_ctx {
+    nmr_problem prob{};
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nc = 1;
+    bool vel = false;
+    bool exact = false;
+    int pitch = 0, pitch1 = 0;
+    size_t n_m = 0;  // floats per magnetization component (nz*ny*pitch)
+    size_t n_vx = 0, n_vy = 0, n_vz = 0;
+    float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    uint8_t* mask = nullptr;
+    float* vx = nullptr;
+    float* vy = nullptr;
+    float* vz = nullptr;
+    double* partials = nullptr;
+    size_t partials_cap = 0; // doubles
+    double* dsignal = nullptr;
+    size_t dsignal_cap = 0;  // doubles
+    int cur = 0;
+    bool has_mask = false, has_mag = false;
+    int variant = -1, lz = 0; // -1 / 0 = automatic
+    double last_ms = 0.0;
+    int64_t last_sweeps = 0;
+    int64_t device_bytes = 0;
+};
+
+namespace {
+
+int check_layout(int32_t layout)
+{
+    if (layout != NMR_LAYOUT_COMPACT && layout != NMR_LAYOUT_PADDED)
+        return fail(NMR_ERR_INVALID, "unknown layout %d", (int)layout);
+    return NMR_OK;
+}
+
+template <typename T>
+int dev_alloc(nmr_ctx* c, T** p, size_t count)
+{
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIP_TRY(hipMalloc((void**)p, bytes));
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, c->stream));
+    c->device_bytes += (int64_t)bytes;
+    return NMR_OK;
+}
+
+void free_all(nmr_ctx* c)
+{
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 3; q++)
+            if (c->m[b][q]) (void)hipFree(c->m[b][q]);
+    if (c->mask) (void)hipFree(c->mask);
+    if (c->vx) (void)hipFree(c->vx);
+    if (c->vy) (void)hipFree(c->vy);
+    if (c->vz) (void)hipFree(c->vz);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->dsignal) (void)hipFree(c->dsignal);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Padded (reference) index of interior-or-ghost cell (i, j, k), each 0..n+1.
+inline size_t pidx(const nmr_problem& p, int i, int j, int k)
+{
+    return (size_t)i + (size_t)(p.nx + 2) * ((size_t)j + (size_t)(p.ny + 2) * (size_t)k);
+}
+
+// Tile shape picked when the caller leaves it automatic.
+void auto_tile(const nmr_ctx* c, int* variant, int* lz)
+{
+    const int nx = c->prob.nx;
+    int v;
+    if (nx > 128) v = 0;      // TX 256, TY 4
+    else if (nx > 64) v = 2;  // TX 128, TY 8
+    else if (nx > 32) v = 4;  // TX 64, TY 16
+    else v = 6;               // TX 32, TY 32
+    int lx, rpt, tx, ty;
+    nmr::variant_shape(v, &lx, &rpt, &tx, &ty);
+    const long tiles_xy = (long)((nx + tx - 1) / tx) * (long)((c->prob.ny + ty - 1) / ty);
+    // aim for >= 2048 workgroups (8 per CU) while keeping chunks >= 8 planes
+    long want = 2048;
+    int z = (int)std::max<long>(8, (long)c->prob.nz * tiles_xy / want);
+    z = std::min(z, c->prob.nz);
+    *variant = v;
+    *lz = std::max(1, z);
+}
+
+nmr::Coef make_coef(const nmr_ctx* c, const double g[3])
+{
+    const nmr_problem& p = c->prob;
+    nmr::Coef k{};
+    const double dt = p.dt;
+    const double D = (p.flags & NMR_F_DIFFUSION) ? p.diffusivity : 0.0;
+    k.cdx = (float)(dt * D / (p.dx * p.dx));
+    k.cdy = (float)(dt * D / (p.dy * p.dy));
+    k.cdz = (float)(dt * D / (p.dz * p.dz));
+    k.hx = (float)(dt / p.dx);
+    k.hy = (float)(dt / p.dy);
+    k.hz = (float)(dt / p.dz);
+    k.r1 = (float)(dt * p.r1_bulk);
+    k.r2 = (float)(dt * p.r2_bulk);
+    k.s1x = (float)(dt * p.rho1 / p.dx);
+    k.s1y = (float)(dt * p.rho1 / p.dy);
+    k.s1z = (float)(dt * p.rho1 / p.dz);
+    k.s2x = (float)(dt * p.rho2 / p.dx);
+    k.s2y = (float)(dt * p.rho2 / p.dy);
+    k.s2z = (float)(dt * p.rho2 / p.dz);
+    k.peq = (float)p.p_eq;
+    const double nb = std::sqrt(p.b0_dir[0] * p.b0_dir[0] + p.b0_dir[1] * p.b0_dir[1] + p.b0_dir[2] * p.b0_dir[2]);
+    k.n0 = (float)(p.b0_dir[0] / nb);
+    k.n1 = (float)(p.b0_dir[1] / nb);
+    k.n2 = (float)(p.b0_dir[2] / nb);
+    if (p.flags & NMR_F_PRECESSION) {
+        const double b = dt * p.gamma;
+        k.bgx = (float)(b * g[0] * p.dx);
+        k.bgy = (float)(b * g[1] * p.dy);
+        k.bgz = (float)(b * g[2] * p.dz);
+    }
+    return k;
+}
+
+void signal_window(const nmr_problem& p, int w[6])
+{
+    w[0] = p.buffer[0];
+    w[1] = p.nx - p.buffer[0];
+    w[2] = p.buffer[1];
+    w[3] = p.ny - p.buffer[1];
+    w[4] = p.buffer[2];
+    w[5] = p.nz - p.buffer[2];
+}
+
+int ensure_partials(nmr_ctx* c, size_t doubles)
+{
+    if (doubles <= c->partials_cap) return NMR_OK;
+    if (c->partials) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(c->partials));
+        c->device_bytes -= (int64_t)(c->partials_cap * sizeof(double));
+        c->partials = nullptr;
+        c->partials_cap = 0;
+    }
+    HIP_TRY(hipMalloc((void**)&c->partials, doubles * sizeof(double)));
+    c->partials_cap = doubles;
+    c->device_bytes += (int64_t)(doubles * sizeof(double));
+    return NMR_OK;
+}
+
+int ensure_signal(nmr_ctx* c, size_t doubles)
+{
+    if (doubles <= c->dsignal_cap) return NMR_OK;
+    if (c->dsignal) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(c->dsignal));
+        c->device_bytes -= (int64_t)(c->dsignal_cap * sizeof(double));
+        c->dsignal = nullptr;
+        c->dsignal_cap = 0;
+    }
+    HIP_TRY(hipMalloc((void**)&c->dsignal, doubles * sizeof(double)));
+    c->dsignal_cap = doubles;
+    c->device_bytes += (int64_t)(doubles * sizeof(double));
+    return NMR_OK;
+}
+
+// Signal of the current state into dsignal[row*nc ...] (enqueued on the stream).
+int enqueue_field_sum(nmr_ctx* c, size_t row)
+{
+    const nmr_problem& p = c->prob;
+    int rc = ensure_partials(c, (size_t)p.nz * c->nc);
+    if (rc) return rc;
+    nmr::FieldSumArgs a{};
+    for (int q = 0; q < 3; q++) a.f[q] = c->m[c->cur][q];
+    a.mask = c->mask;
+    a.part = c->partials;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.pitch = c->pitch;
+    int w[6];
+    signal_window(p, w);
+    a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
+    HIP_TRY(nmr::launch_field_sum(c->nc, a, p.nz, c->stream));
+    // one "step" whose nwg = nz partials
+    HIP_TRY(nmr::launch_reduce_partials(c->partials, 1, p.nz, c->nc, c->dsignal + row * c->nc, c->stream));
+    return NMR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int nmr_hip_abi_version(void) { return 1; }
+
+const char* nmr_hip_strerror(int code)
+{
+    switch (code) {
+    case NMR_OK: return "success";
+    case NMR_ERR_INVALID: return "invalid argument";
+    case NMR_ERR_NULL: return "null pointer";
+    case NMR_ERR_DEVICE: return "HIP device error";
+    case NMR_ERR_NOMEM: return "device out of memory";
+    case NMR_ERR_STATE: return "call out of order";
+    case NMR_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+const char* nmr_hip_last_error(void) { return g_last_error.c_str(); }
+
+int nmr_hip_num_variants(void) { return nmr::num_variants(); }
+
+int nmr_hip_variant_info(int32_t variant, int32_t* lanes_x, int32_t* rows_per_thread, int32_t* tile_x,
+                         int32_t* tile_y)
+{
+    int lx, rpt, tx, ty;
+    if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty)) return fail(NMR_ERR_INVALID, "no variant %d", (int)variant);
+    if (lanes_x) *lanes_x = lx;
+    if (rows_per_thread) *rows_per_thread = rpt;
+    if (tile_x) *tile_x = tx;
+    if (tile_y) *tile_y = ty;
+    return NMR_OK;
+}
+
+int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
+{
+    if (!prob || !out) return fail(NMR_ERR_NULL, "nmr_hip_create: null argument");
+    *out = nullptr;
+    const nmr_problem& p = *prob;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1) return fail(NMR_ERR_INVALID, "grid %d x %d x %d", p.nx, p.ny, p.nz);
+    if ((long)p.nx * (long)p.ny * (long)p.nz > (1L << 31) - 1 || p.nx > (1 << 20) || p.ny > (1 << 20))
+        return fail(NMR_ERR_INVALID, "grid %d x %d x %d exceeds 2^31 voxels", p.nx, p.ny, p.nz);
+    if (p.ncomp < 1 || p.ncomp > 3) return fail(NMR_ERR_INVALID, "ncomp %d", p.ncomp);
+    if (p.scheme != NMR_SCHEME_UPWIND) return fail(NMR_ERR_UNSUPPORTED, "scheme %d", p.scheme);
+    if (p.phase_mode != NMR_PHASE_LINEAR && p.phase_mode != NMR_PHASE_EXACT)
+        return fail(NMR_ERR_INVALID, "phase_mode %d", p.phase_mode);
+    if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION))
+        return fail(NMR_ERR_INVALID, "flags 0x%x", p.flags);
+    if (!(p.dx > 0) || !(p.dy > 0) || !(p.dz > 0) || !(p.dt > 0))
+        return fail(NMR_ERR_INVALID, "dx, dy, dz, dt must be > 0");
+    for (int a = 0; a < 3; a++)
+        if (p.buffer[a] < 0) return fail(NMR_ERR_INVALID, "buffer[%d] < 0", a);
+    const double nb = std::sqrt(p.b0_dir[0] * p.b0_dir[0] + p.b0_dir[1] * p.b0_dir[1] + p.b0_dir[2] * p.b0_dir[2]);
+    if (!(nb > 0)) return fail(NMR_ERR_INVALID, "b0_dir is zero");
+    if (p.ncomp < 3 && (p.b0_dir[0] != 0.0 || p.b0_dir[1] != 0.0))
+        return fail(NMR_ERR_UNSUPPORTED, "ncomp < 3 needs the main field along z");
+    if (p.ncomp == 1 && (p.flags & NMR_F_PRECESSION))
+        return fail(NMR_ERR_UNSUPPORTED, "precession needs ncomp >= 2");
+    if (p.ncomp == 2 && (p.grad[0] != 0.0 || p.grad[1] != 0.0))
+        return fail(NMR_ERR_UNSUPPORTED, "ncomp 2 takes a z gradient only");
+    if (p.phase_mode == NMR_PHASE_EXACT && p.ncomp != 2)
+        return fail(NMR_ERR_UNSUPPORTED, "exact phase needs ncomp 2");
+
+    nmr_ctx* c = new (std::nothrow) nmr_ctx();
+    if (!c) return fail(NMR_ERR_NOMEM, "host allocation");
+    c->prob = p;
+    c->nc = p.ncomp;
+    c->vel = (p.flags & NMR_F_VELOCITY) != 0;
+    c->exact = p.phase_mode == NMR_PHASE_EXACT;
+    c->pitch = round_up4(p.nx);
+    c->pitch1 = round_up4(p.nx + 1);
+    c->n_m = (size_t)p.nz * p.ny * c->pitch;
+    auto bail = [&](int rc) {
+        free_all(c);
+        delete c;
+        return rc;
+    };
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) return bail(fail(NMR_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail(NMR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(NMR_ERR_DEVICE, "hipEventCreate failed"));
+    int rc;
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < c->nc; q++)
+            if ((rc = dev_alloc(c, &c->m[b][q], c->n_m))) return bail(rc);
+    if ((rc = dev_alloc(c, &c->mask, c->n_m))) return bail(rc);
+    if (c->vel) {
+        c->n_vx = (size_t)p.nz * p.ny * c->pitch1;
+        c->n_vy = (size_t)p.nz * (p.ny + 1) * c->pitch;
+        c->n_vz = (size_t)(p.nz + 1) * p.ny * c->pitch;
+        if ((rc = dev_alloc(c, &c->vx, c->n_vx))) return bail(rc);
+        if ((rc = dev_alloc(c, &c->vy, c->n_vy))) return bail(rc);
+        if ((rc = dev_alloc(c, &c->vz, c->n_vz))) return bail(rc);
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail(fail(NMR_ERR_DEVICE, "hipStreamSynchronize: %s", hipGetErrorString(e)));
+    *out = c;
+    return NMR_OK;
+}
+
+int nmr_hip_destroy(nmr_ctx* ctx)
+{
+    if (!ctx) return NMR_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return NMR_OK;
+}
+
+int nmr_hip_device_bytes(nmr_ctx* ctx, int64_t* bytes)
+{
+    if (!ctx || !bytes) return fail(NMR_ERR_NULL, "nmr_hip_device_bytes: null argument");
+    *bytes = ctx->device_bytes;
+    return NMR_OK;
+}
+
+int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout)
+{
+    if (!ctx || !mask) return fail(NMR_ERR_NULL, "nmr_hip_upload_mask: null argument");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    const nmr_problem& p = ctx->prob;
+    std::vector<uint8_t> h(ctx->n_m, 0);
+    for (int k = 0; k < p.nz; k++)
+        for (int j = 0; j < p.ny; j++)
+            for (int i = 0; i < p.nx; i++) {
+                uint8_t v;
+                if (layout == NMR_LAYOUT_COMPACT) {
+                    v = mask[(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)];
+                    if (v > 1) return fail(NMR_ERR_INVALID, "compact mask value %d (want 0/1)", (int)v);
+                } else {
+                    const uint8_t ch = mask[pidx(p, i + 1, j + 1, k + 1)];
+                    if (ch != '0' && ch != '1') return fail(NMR_ERR_INVALID, "padded mask char 0x%02x (want '0'/'1')", ch);
+                    v = (ch == '1');
+                }
+                h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i] = v;
+            }
+    HIP_TRY(hipMemcpyAsync(ctx->mask, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->has_mask = true;
+    return NMR_OK;
+}
+
+int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, const double* vz, int32_t layout)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_upload_velocity: null context");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    if (!ctx->vel) return fail(NMR_ERR_STATE, "velocity upload without NMR_F_VELOCITY");
+    const nmr_problem& p = ctx->prob;
+    const int P = ctx->pitch, P1 = ctx->pitch1;
+    {
+        std::vector<float> h(ctx->n_vx, 0.f);
+        if (vx)
+            for (int k = 0; k < p.nz; k++)
+                for (int j = 0; j < p.ny; j++)
+                    for (int i = 0; i <= p.nx; i++) {
+                        const double v = (layout == NMR_LAYOUT_COMPACT)
+                                             ? vx[(size_t)i + (size_t)(p.nx + 1) * ((size_t)j + (size_t)p.ny * k)]
+                                             : vx[pidx(p, i + 1, j + 1, k + 1)];
+                        h[(size_t)k * p.ny * P1 + (size_t)j * P1 + i] = (float)v;
+                    }
+        HIP_TRY(hipMemcpyAsync(ctx->vx, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    {
+        std::vector<float> h(ctx->n_vy, 0.f);
+        if (vy)
+            for (int k = 0; k < p.nz; k++)
+                for (int j = 0; j <= p.ny; j++)
+                    for (int i = 0; i < p.nx; i++) {
+                        const double v = (layout == NMR_LAYOUT_COMPACT)
+                                             ? vy[(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)(p.ny + 1) * k)]
+                                             : vy[pidx(p, i + 1, j + 1, k + 1)];
+                        h[(size_t)k * (p.ny + 1) * P + (size_t)j * P + i] = (float)v;
+                    }
+        HIP_TRY(hipMemcpyAsync(ctx->vy, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    {
+        std::vector<float> h(ctx->n_vz, 0.f);
+        if (vz)
+            for (int k = 0; k <= p.nz; k++)
+                for (int j = 0; j < p.ny; j++)
+                    for (int i = 0; i < p.nx; i++) {
+                        const double v = (layout == NMR_LAYOUT_COMPACT)
+                                             ? vz[(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)]
+                                             : vz[pidx(p, i + 1, j + 1, k + 1)];
+                        h[(size_t)k * p.ny * P + (size_t)j * P + i] = (float)v;
+                    }
+        HIP_TRY(hipMemcpyAsync(ctx->vz, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return NMR_OK;
+}
+
+int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* my, const double* mz, int32_t layout)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_upload_magnetization: null context");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    const double* src[3] = {mx, my, mz};
+    for (int q = 0; q < ctx->nc; q++)
+        if (!src[q]) return fail(NMR_ERR_NULL, "magnetization component %d is NULL", q);
+    const nmr_problem& p = ctx->prob;
+    std::vector<float> h(ctx->n_m, 0.f);
+    for (int q = 0; q < ctx->nc; q++) {
+        for (int k = 0; k < p.nz; k++)
+            for (int j = 0; j < p.ny; j++)
+                for (int i = 0; i < p.nx; i++) {
+                    const double v = (layout == NMR_LAYOUT_COMPACT)
+                                         ? src[q][(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)]
+                                         : src[q][pidx(p, i + 1, j + 1, k + 1)];
+                    h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i] = (float)v;
+                }
+        HIP_TRY(hipMemcpyAsync(ctx->m[ctx->cur][q], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->has_mag = true;
+    return NMR_OK;
+}
+
+int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double* mz, int32_t layout)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_download_magnetization: null context");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    double* dst[3] = {mx, my, mz};
+    const nmr_problem& p = ctx->prob;
+    std::vector<float> h(ctx->n_m);
+    for (int q = 0; q < ctx->nc; q++) {
+        if (!dst[q]) continue;
+        HIP_TRY(hipMemcpyAsync(h.data(), ctx->m[ctx->cur][q], h.size() * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (layout == NMR_LAYOUT_COMPACT) {
+            for (int k = 0; k < p.nz; k++)
+                for (int j = 0; j < p.ny; j++)
+                    for (int i = 0; i < p.nx; i++)
+                        dst[q][(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)] =
+                            h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i];
+        } else {
+            // interior plus the zero-derivative ghost copy (clamped indices)
+            for (int k = 0; k < p.nz + 2; k++)
+                for (int j = 0; j < p.ny + 2; j++)
+                    for (int i = 0; i < p.nx + 2; i++) {
+                        const int ii = std::min(std::max(i - 1, 0), p.nx - 1);
+                        const int jj = std::min(std::max(j - 1, 0), p.ny - 1);
+                        const int kk = std::min(std::max(k - 1, 0), p.nz - 1);
+                        dst[q][pidx(p, i, j, k)] = h[(size_t)kk * p.ny * ctx->pitch + (size_t)jj * ctx->pitch + ii];
+                    }
+        }
+    }
+    return NMR_OK;
+}
+
+int nmr_hip_set_tile(nmr_ctx* ctx, int32_t variant, int32_t lz)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_set_tile: null context");
+    if (variant < -1 || variant >= nmr::num_variants()) return fail(NMR_ERR_INVALID, "variant %d", (int)variant);
+    if (lz < 0) return fail(NMR_ERR_INVALID, "lz %d", (int)lz);
+    ctx->variant = variant;
+    ctx->lz = lz;
+    return NMR_OK;
+}
+
+int nmr_hip_signal(nmr_ctx* ctx, double* out)
+{
+    if (!ctx || !out) return fail(NMR_ERR_NULL, "nmr_hip_signal: null argument");
+    if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "signal before mask and magnetization upload");
+    int rc = ensure_signal(ctx, (size_t)ctx->nc);
+    if (rc) return rc;
+    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(out, ctx->dsignal, sizeof(double) * ctx->nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return NMR_OK;
+}
+
+int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_last_run_ms: null context");
+    if (ms) *ms = ctx->last_ms;
+    if (sweeps) *sweeps = ctx->last_sweeps;
+    return NMR_OK;
+}
+
+int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_run: null context");
+    if (nsteps < 0) return fail(NMR_ERR_INVALID, "nsteps %lld", (long long)nsteps);
+    if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "run before mask and magnetization upload");
+    const nmr_problem& p = ctx->prob;
+    if (grad_sched) {
+        for (int64_t s = 0; s < nsteps; s++) {
+            const double* g = grad_sched + 3 * s;
+            if (!std::isfinite(g[0]) || !std::isfinite(g[1]) || !std::isfinite(g[2]))
+                return fail(NMR_ERR_INVALID, "gradient schedule step %lld is not finite", (long long)s);
+            if (ctx->nc == 2 && (g[0] != 0.0 || g[1] != 0.0))
+                return fail(NMR_ERR_UNSUPPORTED, "ncomp 2 takes a z gradient only (step %lld)", (long long)s);
+            if (ctx->nc == 1 && (p.flags & NMR_F_PRECESSION))
+                return fail(NMR_ERR_UNSUPPORTED, "precession needs ncomp >= 2");
+        }
+    }
+    int variant = ctx->variant, lz = ctx->lz;
+    {
+        int av, alz;
+        auto_tile(ctx, &av, &alz);
+        if (variant < 0) variant = av;
+        if (lz <= 0) lz = alz;
+    }
+    int lx, rpt, tx, ty;
+    nmr::variant_shape(variant, &lx, &rpt, &tx, &ty);
+    lz = std::min(lz, p.nz);
+    const int ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty, ntz = (p.nz + lz - 1) / lz;
+    const long nwg_l = (long)ntx * nty * ntz;
+    if (nwg_l > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
+    const int nwg = (int)nwg_l;
+    const int nc = ctx->nc;
+
+    const int64_t rows = nsteps + 1;
+    int rc = ensure_signal(ctx, (size_t)rows * nc);
+    if (rc) return rc;
+    rc = ensure_partials(ctx, std::max<size_t>((size_t)kChunkSteps * nwg * nc, (size_t)p.nz * nc));
+    if (rc) return rc;
+
+    nmr::SweepArgs a{};
+    a.mask = ctx->mask;
+    a.vx = ctx->vx;
+    a.vy = ctx->vy;
+    a.vz = ctx->vz;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.pitch = ctx->pitch;
+    a.pitch1 = ctx->pitch1;
+    a.ntx = ntx;
+    a.nty = nty;
+    a.ntz = ntz;
+    a.lz = lz;
+    a.nblocks = nwg;
+    int w[6];
+    signal_window(p, w);
+    a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
+
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    double gcur[3] = {p.grad[0], p.grad[1], p.grad[2]};
+    nmr::Coef coef = make_coef(ctx, gcur);
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int chunk = (int)std::min<int64_t>(kChunkSteps, nsteps - done);
+        for (int s = 0; s < chunk; s++) {
+            const int64_t step = done + s;
+            if (grad_sched) {
+                const double* g = grad_sched + 3 * step;
+                if (g[0] != gcur[0] || g[1] != gcur[1] || g[2] != gcur[2]) {
+                    gcur[0] = g[0];
+                    gcur[1] = g[1];
+                    gcur[2] = g[2];
+                    coef = make_coef(ctx, gcur);
+                }
+            }
+            a.c = coef;
+            for (int q = 0; q < 3; q++) {
+                a.min[q] = ctx->m[ctx->cur][q];
+                a.mout[q] = ctx->m[ctx->cur ^ 1][q];
+            }
+            a.partials = ctx->partials + (size_t)s * nwg * nc;
+            HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, variant, a, ctx->stream));
+            ctx->cur ^= 1;
+        }
+        HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, nwg, nc, ctx->dsignal + (size_t)(done + 1) * nc,
+                                            ctx->stream));
+        done += chunk;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    if (signal)
+        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * nc, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    ctx->last_sweeps = nsteps;
+    return NMR_OK;
+}
+
+} // extern "C"

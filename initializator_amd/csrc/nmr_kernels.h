@@ -1,0 +1,51 @@
+// nmr_kernels.h -- internal interface between the C ABI (nmr_abi.hip) and
+// the gfx950 kernels (nmr_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nmr {
+
+// Per-step coefficients of the update, all pre-multiplied by dt (FP32).
+struct Coef {
+    float cdx, cdy, cdz;    // dt*D/d^2 per axis (0 when diffusion is off)
+    float hx, hy, hz;       // dt/d per axis
+    float r1, r2;           // dt/T1_bulk, dt/T2_bulk
+    float s1x, s1y, s1z;    // dt*rho1/d per solid face
+    float s2x, s2y, s2z;    // dt*rho2/d per solid face
+    float peq;              // equilibrium magnetization
+    float n0, n1, n2;       // unit main-field direction
+    float bgx, bgy, bgz;    // dt*g*G_a*d_a: precession angle per index unit
+};
+
+struct SweepArgs {
+    const float* min[3];
+    float* mout[3];
+    const uint8_t* mask;
+    const float* vx;        // [nz][ny][pitch1]  face i-1|i at index i, i = 0..nx
+    const float* vy;        // [nz][ny+1][pitch] face j-1|j at row j
+    const float* vz;        // [nz+1][ny][pitch] face k-1|k at plane k
+    double* partials;       // [nblocks][nc] signal partials of this step
+    int nx, ny, nz;
+    int pitch, pitch1;
+    int ntx, nty, ntz, lz, nblocks;
+    int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
+    Coef c;
+};
+
+struct FieldSumArgs {
+    const float* f[3];
+    const uint8_t* mask;
+    double* part;           // [nz][nc]
+    int nx, ny, nz, pitch;
+    int bx0, bx1, by0, by1, bz0, bz1;
+};
+
+int num_variants();
+bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty);
+hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s);
+hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
+hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s);
+
+} // namespace nmr

@@ -1,0 +1,158 @@
+"""Python host wrapper over the C ABI: one HipSolver = one nmr_ctx.
+
+This mirrors what the reference's solve_gpu() does per run
+(src/NmrRelaxationSolveKernels.cu:1056-1319): allocate and upload once,
+step on the device, reduce the signal every step, download at the end.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import NmrProblemC, check
+from .generators import G_REF, Case
+
+
+def _dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class HipSolver:
+    """Device-resident Bloch-Torrey solve of one problem."""
+
+    def __init__(self, shape, ncomp: int, dx: float, dt: float, diffusivity: float, r2_bulk: float,
+                 r1_bulk: float = 0.0, rho1: float = 0.0, rho2: float = 0.0, gamma: float = G_REF,
+                 p_eq: float = 0.0, b0_dir=(0.0, 0.0, 1.0), grad=(0.0, 0.0, 0.0), diffusion: bool = True,
+                 velocity: bool = False, precession: bool = False, phase_exact: bool = False,
+                 buffer=(0, 0, 0), dy: float | None = None, dz: float | None = None):
+        self.lib = _lib.load()
+        nz, ny, nx = (int(s) for s in shape)
+        self.shape = (nz, ny, nx)
+        self.ncomp = int(ncomp)
+        p = NmrProblemC()
+        p.nx, p.ny, p.nz = nx, ny, nz
+        p.ncomp = self.ncomp
+        p.flags = ((_lib.NMR_F_DIFFUSION if diffusion else 0) | (_lib.NMR_F_VELOCITY if velocity else 0)
+                   | (_lib.NMR_F_PRECESSION if precession else 0))
+        p.scheme = _lib.NMR_SCHEME_UPWIND
+        p.phase_mode = _lib.NMR_PHASE_EXACT if phase_exact else _lib.NMR_PHASE_LINEAR
+        for a in range(3):
+            p.buffer[a] = int(buffer[a])
+            p.b0_dir[a] = float(b0_dir[a])
+            p.grad[a] = float(grad[a])
+        p.dx = float(dx)
+        p.dy = float(dy if dy is not None else dx)
+        p.dz = float(dz if dz is not None else dx)
+        p.dt = float(dt)
+        p.diffusivity = float(diffusivity)
+        p.r1_bulk = float(r1_bulk)
+        p.r2_bulk = float(r2_bulk)
+        p.rho1 = float(rho1)
+        p.rho2 = float(rho2)
+        p.gamma = float(gamma)
+        p.p_eq = float(p_eq)
+        self.problem = p
+        self.velocity = velocity
+        ctx = ctypes.c_void_p()
+        check(self.lib.nmr_hip_create(ctypes.byref(p), ctypes.byref(ctx)), "nmr_hip_create")
+        self.ctx = ctx
+
+    @classmethod
+    def from_case(cls, case: Case, precession: bool | None = None, **kw) -> "HipSolver":
+        has_v = case.vx is not None
+        prec = case.grad_sched is not None if precession is None else precession
+        s = cls(case.shape, case.ncomp, case.dx, case.dt, case.diffusivity, 1.0 / case.t2_bulk,
+                r1_bulk=1.0 / case.t1_bulk, rho1=case.rho, rho2=case.rho, velocity=has_v, precession=prec, **kw)
+        s.upload_mask(case.mask)
+        if has_v:
+            s.upload_velocity(case.vx, case.vy, case.vz)
+        return s
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            check(self.lib.nmr_hip_destroy(self.ctx), "nmr_hip_destroy")
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data -----------------------------------------------------------------
+    def upload_mask(self, mask: np.ndarray, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        check(self.lib.nmr_hip_upload_mask(self.ctx, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), layout),
+              "nmr_hip_upload_mask")
+
+    def upload_velocity(self, vx, vy, vz, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz)]
+        check(self.lib.nmr_hip_upload_velocity(self.ctx, *[_dptr(a) for a in arrs], layout),
+              "nmr_hip_upload_velocity")
+
+    def upload_magnetization(self, mx, my=None, mz=None, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (mx, my, mz)]
+        check(self.lib.nmr_hip_upload_magnetization(self.ctx, *[_dptr(a) for a in arrs], layout),
+              "nmr_hip_upload_magnetization")
+
+    def download_magnetization(self, layout: int = _lib.NMR_LAYOUT_COMPACT):
+        nz, ny, nx = self.shape
+        shp = (nz, ny, nx) if layout == _lib.NMR_LAYOUT_COMPACT else (nz + 2, ny + 2, nx + 2)
+        outs = [np.empty(shp) if q < self.ncomp else None for q in range(3)]
+        check(self.lib.nmr_hip_download_magnetization(self.ctx, *[_dptr(a) for a in outs], layout),
+              "nmr_hip_download_magnetization")
+        return outs[: self.ncomp]
+
+    # -- stepping -------------------------------------------------------------
+    def set_tile(self, variant: int = -1, lz: int = 0) -> None:
+        check(self.lib.nmr_hip_set_tile(self.ctx, int(variant), int(lz)), "nmr_hip_set_tile")
+
+    def run(self, nsteps: int, grad_sched: np.ndarray | None = None, want_signal: bool = True):
+        """Advance nsteps; returns the (nsteps+1, ncomp) signal (or None)."""
+        g = None
+        if grad_sched is not None:
+            g = np.ascontiguousarray(grad_sched, dtype=np.float64)
+            if g.shape != (nsteps, 3):
+                raise ValueError(f"grad_sched must have shape ({nsteps}, 3)")
+        sig = np.empty((nsteps + 1, self.ncomp)) if want_signal else None
+        check(self.lib.nmr_hip_run(self.ctx, int(nsteps), _dptr(g), _dptr(sig)), "nmr_hip_run")
+        return sig
+
+    def signal(self) -> np.ndarray:
+        out = np.empty(self.ncomp)
+        check(self.lib.nmr_hip_signal(self.ctx, _dptr(out)), "nmr_hip_signal")
+        return out
+
+    def last_run_ms(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.lib.nmr_hip_last_run_ms(self.ctx, ctypes.byref(ms), ctypes.byref(n)), "nmr_hip_last_run_ms")
+        return ms.value, n.value
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64()
+        check(self.lib.nmr_hip_device_bytes(self.ctx, ctypes.byref(b)), "nmr_hip_device_bytes")
+        return b.value
+
+
+def variants():
+    """[(index, lanes_x, rows_per_thread, tile_x, tile_y)] of the sweep kernel."""
+    lib = _lib.load()
+    out = []
+    for v in range(lib.nmr_hip_num_variants()):
+        a, b, c, d = (ctypes.c_int32() for _ in range(4))
+        check(lib.nmr_hip_variant_info(v, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)),
+              "nmr_hip_variant_info")
+        out.append((v, a.value, b.value, c.value, d.value))
+    return out

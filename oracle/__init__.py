@@ -1,0 +1,192 @@
+"""FP64 CPU oracle of the reference's Bloch-Torrey update -- TEST
+INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker (or the timed CPU baseline).  The
+product (initializator_amd/, libnmr_hip.so) never imports or links it.
+
+ctypes wrapper of oracle/libnmr_oracle.so (oracle/nmr_oracle.c), which
+restates src/NmrRelaxationSolve.cpp of the reference in FP64 on its padded
+layout.  The pinning status of every function is listed at the top of
+nmr_oracle.c; the pinning tests live in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_long
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnmr_oracle.so")
+
+G_REF = 4.78941714e7
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [
+        ("ik", c_int32),
+        ("jk", c_int32),
+        ("kk", c_int32),
+        ("precession", c_int32),
+        ("velocity", c_int32),
+        ("diffusion", c_int32),
+        ("dx", c_double),
+        ("dy", c_double),
+        ("dz", c_double),
+        ("dt", c_double),
+        ("gamma", c_double),
+        ("p_eq", c_double),
+        ("b0", c_double * 3),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = POINTER(c_double)
+        cp = c_char_p
+        P = POINTER(OrParams)
+        L.or_num_cells.restype = c_long
+        L.or_num_cells.argtypes = [P]
+        L.or_bc_mask.argtypes = [P, cp]
+        L.or_bc_field.argtypes = [P, dp]
+        L.or_fill_coefficients.argtypes = [P, cp, c_double, c_double, c_double, c_double, c_double,
+                                           dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_fill_gradient.argtypes = [P, c_double, c_double, c_double, dp, dp, dp]
+        L.or_step_time.restype = c_double
+        L.or_step_time.argtypes = [c_double] * 7
+        L.or_explicit_simple.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_output.restype = c_double
+        L.or_output.argtypes = [P, cp, c_int, c_int, c_int, dp, POINTER(c_int32)]
+        L.or_run.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, c_int64, dp, dp, c_int, c_int, c_int,
+                             dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _c(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return ctypes.cast(a.ctypes.data, c_char_p)
+
+
+def params(shape, dx, dt, *, dy=None, dz=None, precession=False, velocity=False, diffusion=True,
+           gamma=G_REF, p_eq=0.0, b0=(0.0, 0.0, 1.0)) -> OrParams:
+    nz, ny, nx = shape
+    p = OrParams()
+    p.ik, p.jk, p.kk = nx, ny, nz
+    p.precession, p.velocity, p.diffusion = int(precession), int(velocity), int(diffusion)
+    p.dx = dx
+    p.dy = dy if dy is not None else dx
+    p.dz = dz if dz is not None else dx
+    p.dt = dt
+    p.gamma = gamma
+    p.p_eq = p_eq
+    for i in range(3):
+        p.b0[i] = b0[i]
+    return p
+
+
+def padded_shape(p: OrParams):
+    return (p.kk + 2, p.jk + 2, p.ik + 2)
+
+
+def bc_mask(p, act):
+    lib().or_bc_mask(ctypes.byref(p), _c(act))
+
+
+def bc_field(p, f):
+    lib().or_bc_field(ctypes.byref(p), _d(f))
+
+
+def fill_coefficients(p, act, d_water, t1_water, t2_water, r1_water, r2_water, faces=None):
+    """D, T1, T2 (padded) and (Dmax, Tmin) as FillInDiffusinAndTime leaves them."""
+    shp = padded_shape(p)
+    D = np.zeros(shp)
+    t1 = np.zeros(shp)
+    t2 = np.zeros(shp)
+    out2 = np.zeros(2)
+    f = faces or (None,) * 6
+    lib().or_fill_coefficients(ctypes.byref(p), _c(act), d_water, t1_water, t2_water, r1_water, r2_water,
+                               *[_d(x) for x in f], _d(D), _d(t1), _d(t2), _d(out2))
+    return D, t1, t2, (out2[0], out2[1])
+
+
+def fill_gradient(p, g):
+    shp = padded_shape(p)
+    nf = [np.zeros(shp) for _ in range(3)]
+    lib().or_fill_gradient(ctypes.byref(p), g[0], g[1], g[2], *[_d(a) for a in nf])
+    return nf
+
+
+def step_time(dt_noise, dt_grad, dt_vel, dt_diff, dt_rel, time1, time2):
+    return lib().or_step_time(dt_noise, dt_grad, dt_vel, dt_diff, dt_rel, time1, time2)
+
+
+def explicit_simple(p, act, D, t1, t2, vel, nf, m, a):
+    """One ExplicitSimple step in place on m (list of 3 padded arrays)."""
+    lib().or_explicit_simple(ctypes.byref(p), _c(act), _d(D), _d(t1), _d(t2), *[_d(v) for v in vel],
+                             *[_d(x) for x in nf], *[_d(x) for x in m], *[_d(x) for x in a])
+
+
+def output(p, act, buf, f):
+    n = c_int32()
+    s = lib().or_output(ctypes.byref(p), _c(act), buf[0], buf[1], buf[2], _d(f), ctypes.byref(n))
+    return s, n.value
+
+
+def run(p, act, D, t1, t2, vel, nsteps, m, grad_sched=None, grad_const=None, buf=(0, 0, 0)):
+    """nsteps of update + ghost fill + signal; m updated in place.
+    Returns the (nsteps+1, 3) signal."""
+    sig = np.zeros((nsteps + 1, 3))
+    gs = None if grad_sched is None else np.ascontiguousarray(grad_sched, dtype=np.float64)
+    gc = None if grad_const is None else np.ascontiguousarray(grad_const, dtype=np.float64)
+    lib().or_run(ctypes.byref(p), _c(act), _d(D), _d(t1), _d(t2), *[_d(v) for v in vel], int(nsteps),
+                 _d(gs), _d(gc), buf[0], buf[1], buf[2], *[_d(x) for x in m], _d(sig))
+    return sig
+
+
+def setup_case(case, p_eq=0.0, m0=None):
+    """Padded oracle inputs for a generators.Case (one fluid, uniform
+    relaxivity): params, mask, D, T1, T2, velocity, initial M."""
+    from initializator_amd import layout  # conversions only
+
+    has_v = case.vx is not None
+    prec = case.grad_sched is not None
+    p = params(case.shape, case.dx, case.dt, precession=prec, velocity=has_v, p_eq=p_eq)
+    act = layout.pad_mask(case.mask)
+    D, t1, t2, _ = fill_coefficients(p, act, case.diffusivity, case.t1_bulk, case.t2_bulk, case.rho, case.rho)
+    if has_v:
+        vel = [np.ascontiguousarray(v) for v in layout.pad_velocity(case.vx, case.vy, case.vz, case.shape)]
+    else:
+        vel = [np.zeros(padded_shape(p)) for _ in range(3)]
+    if m0 is None:
+        mx = np.where(case.mask != 0, 1.0, 0.0)
+        m0 = [mx, np.zeros(case.shape), np.zeros(case.shape)]
+    m = [np.ascontiguousarray(layout.pad_field(x)) for x in m0]
+    return p, act, D, t1, t2, vel, m
+
+
+def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0)):
+    """Oracle run of a Case: (signal (nsteps+1, 3), compact final fields)."""
+    from initializator_amd import layout
+
+    n = case.nsteps if nsteps is None else nsteps
+    p, act, D, t1, t2, vel, m = setup_case(case, p_eq=p_eq, m0=m0)
+    gs = None if case.grad_sched is None else case.grad_sched[:n]
+    sig = run(p, act, D, t1, t2, vel, n, m, grad_sched=gs, buf=buf)
+    return sig, [layout.unpad(x) for x in m]

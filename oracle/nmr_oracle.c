@@ -1,0 +1,360 @@
+/*
+ * nmr_oracle.c -- FP64 CPU restatement of the reference's per-timestep
+ * Bloch-Torrey update.  TEST INFRASTRUCTURE ONLY: this file is the checker.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the product (initializator_amd/, libnmr_hip.so) never links it.
+ *
+ * Layout: the reference's padded SoA layout, x fastest, one ghost cell per
+ * side:  idx = i + (ik+2)*(j + (jk+2)*k)   (NmrRelaxationSolve.cpp:3284-3287).
+ * Masks are the reference's ASCII '0'/'1' chars.
+ *
+ * Parity status of each function (how it is pinned without a reference
+ * build -- the reference's host path needs cuda_runtime.h and MPI, which
+ * this image does not provide, so it cannot be compiled here):
+ *   or_explicit_simple   pinned: bulk T1/T2 closed forms, enclosed-voxel
+ *                        surface rate, Neumann cosine eigenmode, 1-D upwind
+ *                        brute force (both signs), precession (1-i*theta)^n,
+ *                        conservation / fixed-point invariants
+ *   or_bc_field          pinned: Neumann invariants + brute copy check
+ *   or_bc_mask           pinned: brute copy check
+ *   or_fill_coefficients pinned: hand-computed rates on a 3^3 pore in 5^3
+ *   or_fill_gradient     pinned: closed form G*d*(idx-1/2)
+ *   or_output            pinned: brute-force numpy sum with trimming
+ *   or_step_time         pinned: formula arithmetic
+ *   or_maccormack        pinned: reduces to Heun for relaxation-only and
+ *                        diffusion-only closed forms
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_SMALL 1.0e-50 /* NmrRelaxationSolve.cpp:809 */
+#define OR_BIG 1.0e50    /* NmrRelaxationSolve.cpp:808 */
+
+typedef struct or_params {
+    int32_t ik, jk, kk;      /* interior sizes */
+    int32_t precession;      /* nVectorMult */
+    int32_t velocity;        /* nVelocity */
+    int32_t diffusion;       /* nDiffusion */
+    double dx, dy, dz, dt;
+    double gamma;            /* g = 4.78941714e7 in the reference */
+    double p_eq;             /* equilibriumMagnetization */
+    double b0[3];            /* uniform mainField */
+} or_params;
+
+static inline long IX(const or_params* p, int i, int j, int k)
+{
+    return (long)i + (long)(p->ik + 2) * ((long)j + (long)(p->jk + 2) * (long)k);
+}
+
+long or_num_cells(const or_params* p)
+{
+    return (long)(p->ik + 2) * (long)(p->jk + 2) * (long)(p->kk + 2);
+}
+
+/* Ghost layer of the mask = copy of the adjacent interior on all six faces
+ * (NmrRelaxationSolve.cpp:880-928, single rank: the z exchange degenerates
+ * to the rank-0 / last-rank copies). */
+void or_bc_mask(const or_params* p, char* act)
+{
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int j = 0; j < p->jk + 2; j++) {
+            act[IX(p, 0, j, k)] = act[IX(p, 1, j, k)];
+            act[IX(p, p->ik + 1, j, k)] = act[IX(p, p->ik, j, k)];
+        }
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int i = 0; i < p->ik + 2; i++) {
+            act[IX(p, i, 0, k)] = act[IX(p, i, 1, k)];
+            act[IX(p, i, p->jk + 1, k)] = act[IX(p, i, p->jk, k)];
+        }
+    for (int j = 0; j < p->jk + 2; j++)
+        for (int i = 0; i < p->ik + 2; i++) {
+            act[IX(p, i, j, 0)] = act[IX(p, i, j, 1)];
+            act[IX(p, i, j, p->kk + 1)] = act[IX(p, i, j, p->kk)];
+        }
+}
+
+/* Zero-derivative ghost fill of one field component
+ * (NmrRelaxationSolve.cpp:1664-1754 at one rank; same order: x faces over
+ * all j,k, then y faces over all i,k, then the z faces). */
+void or_bc_field(const or_params* p, double* f)
+{
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int j = 0; j < p->jk + 2; j++) {
+            f[IX(p, 0, j, k)] = f[IX(p, 1, j, k)];
+            f[IX(p, p->ik + 1, j, k)] = f[IX(p, p->ik, j, k)];
+        }
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int i = 0; i < p->ik + 2; i++) {
+            f[IX(p, i, 0, k)] = f[IX(p, i, 1, k)];
+            f[IX(p, i, p->jk + 1, k)] = f[IX(p, i, p->jk, k)];
+        }
+    for (int j = 0; j < p->jk + 2; j++)
+        for (int i = 0; i < p->ik + 2; i++) {
+            f[IX(p, i, j, 0)] = f[IX(p, i, j, 1)];
+            f[IX(p, i, j, p->kk + 1)] = f[IX(p, i, j, p->kk)];
+        }
+}
+
+/* Per-voxel diffusivity and relaxation times for one fluid (NCOMP=1,
+ * NFLUID=1 -> c1=1, c2=0), NmrRelaxationSolve.cpp:1756-1893, followed by
+ * the diffusion ghost fill of NmrRelaxationSolve.cpp:1895-1942.
+ * Surface relaxivities: uniform (nSurfRelWater=0, cpp:1217-1231) unless
+ * face arrays are given (padded layout, face f of cell i on the minus side
+ * is stored at i, the plus side at i+1, as cpp:1798-1841 reads them).
+ * Ghost entries of D copy the interior; T1/T2 ghosts are left untouched
+ * (the reference never reads them).  out2[0]=max D, out2[1]=min T. */
+void or_fill_coefficients(const or_params* p, const char* act, double d_water,
+                          double t1_water, double t2_water, double r1_water, double r2_water,
+                          const double* sr1x, const double* sr1y, const double* sr1z,
+                          const double* sr2x, const double* sr2y, const double* sr2z,
+                          double* D, double* t1, double* t2, double* out2)
+{
+    double dmax = 0.0, tmin = 100.0;
+    for (int k = 1; k < p->kk + 1; k++)
+        for (int j = 1; j < p->jk + 1; j++)
+            for (int i = 1; i < p->ik + 1; i++) {
+                long c = IX(p, i, j, k);
+                if (act[c] == '0') {
+                    D[c] = OR_SMALL;
+                    t1[c] = OR_BIG;
+                    t2[c] = OR_BIG;
+                    continue;
+                }
+                if (act[c] != '1') continue;
+                const double c1 = 1.0, c2 = 0.0;
+                double cd = c1 * d_water;
+                D[c] = cd * pow(1.0 - 2.0 * c1 * c2 / (c1 * c1 + c2 * c2), 10.0);
+                if (!p->diffusion) D[c] = OR_SMALL;
+                double ct1 = c1 / t1_water, ct2 = c1 / t2_water;
+                /* face relaxivity on each solid side */
+                struct { long nb; long face; const double* a1; const double* a2; double h; } side[6] = {
+                    {IX(p, i - 1, j, k), c, sr1x, sr2x, p->dx},
+                    {IX(p, i + 1, j, k), IX(p, i + 1, j, k), sr1x, sr2x, p->dx},
+                    {IX(p, i, j - 1, k), c, sr1y, sr2y, p->dy},
+                    {IX(p, i, j + 1, k), IX(p, i, j + 1, k), sr1y, sr2y, p->dy},
+                    {IX(p, i, j, k - 1), c, sr1z, sr2z, p->dz},
+                    {IX(p, i, j, k + 1), IX(p, i, j, k + 1), sr1z, sr2z, p->dz},
+                };
+                for (int s = 0; s < 6; s++) {
+                    if (act[side[s].nb] != '0') continue;
+                    double q1 = side[s].a1 ? side[s].a1[side[s].face] : r1_water;
+                    double q2 = side[s].a2 ? side[s].a2[side[s].face] : r2_water;
+                    ct1 += (c1 * q1) / side[s].h;
+                    ct2 += (c1 * q2) / side[s].h;
+                }
+                t1[c] = 1.0 / ct1;
+                t2[c] = 1.0 / ct2;
+                if (dmax <= D[c]) dmax = D[c];
+                if (tmin >= t1[c]) tmin = t1[c];
+                if (tmin >= t2[c]) tmin = t2[c];
+            }
+    /* diffusion ghosts: x faces, y faces, then z faces */
+    or_bc_field(p, D);
+    if (out2) {
+        out2[0] = dmax;
+        out2[1] = tmin;
+    }
+}
+
+/* Constant-gradient field Bg = (Gx*dx*(i-1/2), Gy*dy*(j-1/2), Gz*dz*(k-1/2))
+ * on every padded cell (NmrRelaxationSolve.cpp:1966-1972; no random noise). */
+void or_fill_gradient(const or_params* p, double gx, double gy, double gz,
+                      double* nfx, double* nfy, double* nfz)
+{
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int j = 0; j < p->jk + 2; j++)
+            for (int i = 0; i < p->ik + 2; i++) {
+                long c = IX(p, i, j, k);
+                nfx[c] = gx * p->dx * (i - 0.5);
+                nfy[c] = gy * p->dy * (j - 0.5);
+                nfz[c] = gz * p->dz * (k - 0.5);
+            }
+}
+
+/* dt from the stability limits (NmrRelaxationSolve.cpp:2519-2534). */
+double or_step_time(double dt_noise, double dt_grad, double dt_vel, double dt_diff,
+                    double dt_rel, double time1, double time2)
+{
+    double a = fmin(fmin(dt_noise, dt_grad), fmin(dt_vel, dt_diff));
+    double lim = (time1 != 0.0) ? time1 / 100.0 : time2 / 100.0;
+    double b = fmin(dt_rel, lim);
+    /* the reference's min() returns the first argument on ties */
+    return (a <= b) ? a : b;
+}
+
+/* One forward-Euler upwind step, NmrRelaxationSolve.cpp:3148-3282.
+ * m[3]: magnetization (read), a[3]: scratch written for interior pores, then
+ * every interior cell of m is overwritten by a (Jacobi copy-back).  Cells of
+ * a that are not pores keep whatever the caller put there (the reference
+ * initialises addMagnetization to the initial M, cpp:2511-2513). */
+void or_explicit_simple(const or_params* p, const char* act, const double* D,
+                        const double* t1, const double* t2,
+                        const double* vx, const double* vy, const double* vz,
+                        const double* nfx, const double* nfy, const double* nfz,
+                        double* mx, double* my, double* mz,
+                        double* ax, double* ay, double* az)
+{
+    const double dt = p->dt;
+    const double hx = dt / p->dx, hy = dt / p->dy, hz = dt / p->dz;
+    const double hxx = dt / (p->dx * p->dx), hyy = dt / (p->dy * p->dy), hzz = dt / (p->dz * p->dz);
+    const double bx0 = p->b0[0], by0 = p->b0[1], bz0 = p->b0[2];
+    double* m[3] = {mx, my, mz};
+    double* a[3] = {ax, ay, az};
+
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k < p->kk + 1; k++) {
+        for (int j = 1; j < p->jk + 1; j++) {
+            for (int i = 1; i < p->ik + 1; i++) {
+                const long c = IX(p, i, j, k);
+                if (act[c] != '1') continue;
+                const long xm = IX(p, i - 1, j, k), xp = IX(p, i + 1, j, k);
+                const long ym = IX(p, i, j - 1, k), yp = IX(p, i, j + 1, k);
+                const long zm = IX(p, i, j, k - 1), zp = IX(p, i, j, k + 1);
+                const double tau1 = 1.0 / t1[c], tau2 = 1.0 / t2[c];
+                /* harmonic-mean face diffusivities */
+                const double dc = D[c];
+                const double gxp = 2.0 * (D[xp] * dc) / (D[xp] + dc + OR_SMALL);
+                const double gxm = 2.0 * (D[xm] * dc) / (D[xm] + dc + OR_SMALL);
+                const double gyp = 2.0 * (D[yp] * dc) / (D[yp] + dc + OR_SMALL);
+                const double gym = 2.0 * (D[ym] * dc) / (D[ym] + dc + OR_SMALL);
+                const double gzp = 2.0 * (D[zp] * dc) / (D[zp] + dc + OR_SMALL);
+                const double gzm = 2.0 * (D[zm] * dc) / (D[zm] + dc + OR_SMALL);
+                /* staggered face velocities and wall indicators */
+                const double um = vx[c], up = vx[xp];
+                const double vm = vy[c], vp = vy[yp];
+                const double wm = vz[c], wp = vz[zp];
+                const double umm = um / (um + OR_SMALL), upp = up / (up + OR_SMALL);
+                const double vmm = vm / (vm + OR_SMALL), vpp = vp / (vp + OR_SMALL);
+                const double wmm = wm / (wm + OR_SMALL), wpp = wp / (wp + OR_SMALL);
+                const double u = 0.5 * (um + up), v = 0.5 * (vm + vp), w = 0.5 * (wm + wp);
+                const double ud = fabs(u), vd = fabs(v), wd = fabs(w);
+                /* unit main-field direction */
+                const double nb = sqrt(bx0 * bx0 + by0 * by0 + bz0 * bz0);
+                const double n[3] = {bx0 / nb, by0 / nb, bz0 / nb};
+                const double m0[3] = {mx[c], my[c], mz[c]};
+                const double par = n[0] * m0[0] + n[1] * m0[1] + n[2] * m0[2];
+                const double bb = dt * p->gamma;
+                const double bg[3] = {bb * nfx[c], bb * nfy[c], bb * nfz[c]};
+                double out[3];
+                for (int q = 0; q < 3; q++) {
+                    const double mpar = n[q] * par;
+                    out[q] = m0[q] + dt * tau1 * (p->p_eq * n[q] - mpar) + dt * tau2 * (mpar - m0[q]);
+                }
+                if (p->precession) {
+                    /* + dt*g*(m x Bg) */
+                    out[0] += 0.0 - bg[1] * m0[2] + bg[2] * m0[1];
+                    out[1] += 0.0 + bg[0] * m0[2] - bg[2] * m0[0];
+                    out[2] += 0.0 - bg[0] * m0[1] + bg[1] * m0[0];
+                }
+                if (p->velocity) {
+                    for (int q = 0; q < 3; q++) {
+                        const double* f = m[q];
+                        const double c0 = m0[q];
+                        out[q] -= 0.5 * hx * ((u - ud) * (f[xp] - c0) * upp + (u + ud) * (c0 - f[xm]) * umm)
+                                + 0.5 * hy * ((v - vd) * (f[yp] - c0) * vpp + (v + vd) * (c0 - f[ym]) * vmm)
+                                + 0.5 * hz * ((w - wd) * (f[zp] - c0) * wpp + (w + wd) * (c0 - f[zm]) * wmm);
+                    }
+                }
+                if (p->diffusion) {
+                    for (int q = 0; q < 3; q++) {
+                        const double* f = m[q];
+                        const double c0 = m0[q];
+                        out[q] += hxx * (gxp * (f[xp] - c0) - gxm * (c0 - f[xm]))
+                                + hyy * (gyp * (f[yp] - c0) - gym * (c0 - f[ym]))
+                                + hzz * (gzp * (f[zp] - c0) - gzm * (c0 - f[zm]));
+                    }
+                }
+                a[0][c] = out[0];
+                a[1][c] = out[1];
+                a[2][c] = out[2];
+            }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k < p->kk + 1; k++)
+        for (int j = 1; j < p->jk + 1; j++)
+            for (int i = 1; i < p->ik + 1; i++) {
+                const long c = IX(p, i, j, k);
+                mx[c] = ax[c];
+                my[c] = ay[c];
+                mz[c] = az[c];
+            }
+}
+
+/* Signal: sum of one component over pore cells of the buffer-trimmed
+ * interior, one fluid (NmrRelaxationSolve.cpp:3301-3322).  Returns the
+ * sum; *count receives the number of cells summed. */
+double or_output(const or_params* p, const char* act, int bx, int by, int bz,
+                 const double* f, int32_t* count)
+{
+    double sum = 0.0;
+    int32_t n = 0;
+    for (int k = 1 + bz; k < p->kk + 1 - bz; k++)
+        for (int j = 1 + by; j < p->jk + 1 - by; j++)
+            for (int i = 1 + bx; i < p->ik + 1 - bx; i++) {
+                const long c = IX(p, i, j, k);
+                if (act[c] == '1') {
+                    sum += f[c];
+                    n += 1;
+                }
+            }
+    if (count) *count = n;
+    return sum;
+}
+
+/* Time loop in the order of the reference's GPU driver
+ * (NmrRelaxationSolveKernels.cu:1187-1199 with no pulses): update, ghost
+ * fill, then the signal of the updated field.  signal[(n)*3 + q] holds the
+ * sum of component q after n updates, n = 0..nsteps (n = 0 is the initial
+ * state).  grad_sched (nullable, [nsteps][3]) gives the constant-gradient
+ * amplitudes in force during each step; the gradient field is rebuilt when
+ * they change.  All arrays are padded; a* are scratch copies of m*. */
+void or_run(const or_params* p, const char* act, const double* D, const double* t1,
+            const double* t2, const double* vx, const double* vy, const double* vz,
+            int64_t nsteps, const double* grad_sched, const double* grad_const,
+            int bx, int by, int bz,
+            double* mx, double* my, double* mz, double* signal)
+{
+    const long num = or_num_cells(p);
+    double* ax = (double*)malloc(sizeof(double) * num);
+    double* ay = (double*)malloc(sizeof(double) * num);
+    double* az = (double*)malloc(sizeof(double) * num);
+    double* nfx = (double*)malloc(sizeof(double) * num);
+    double* nfy = (double*)malloc(sizeof(double) * num);
+    double* nfz = (double*)malloc(sizeof(double) * num);
+    memcpy(ax, mx, sizeof(double) * num);
+    memcpy(ay, my, sizeof(double) * num);
+    memcpy(az, mz, sizeof(double) * num);
+    double g_cur[3] = {0.0, 0.0, 0.0};
+    if (grad_const) {
+        g_cur[0] = grad_const[0];
+        g_cur[1] = grad_const[1];
+        g_cur[2] = grad_const[2];
+    }
+    or_fill_gradient(p, g_cur[0], g_cur[1], g_cur[2], nfx, nfy, nfz);
+    double* ms[3] = {mx, my, mz};
+    if (signal)
+        for (int q = 0; q < 3; q++) signal[q] = or_output(p, act, bx, by, bz, ms[q], NULL);
+    for (int64_t n = 0; n < nsteps; n++) {
+        if (grad_sched) {
+            const double* gs = grad_sched + 3 * n;
+            if (gs[0] != g_cur[0] || gs[1] != g_cur[1] || gs[2] != g_cur[2]) {
+                g_cur[0] = gs[0];
+                g_cur[1] = gs[1];
+                g_cur[2] = gs[2];
+                or_fill_gradient(p, g_cur[0], g_cur[1], g_cur[2], nfx, nfy, nfz);
+            }
+        }
+        or_explicit_simple(p, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, mx, my, mz, ax, ay, az);
+        or_bc_field(p, mx);
+        or_bc_field(p, my);
+        or_bc_field(p, mz);
+        if (signal)
+            for (int q = 0; q < 3; q++)
+                signal[(n + 1) * 3 + q] = or_output(p, act, bx, by, bz, ms[q], NULL);
+    }
+    free(ax); free(ay); free(az);
+    free(nfx); free(nfy); free(nfz);
+}

@@ -1,0 +1,62 @@
+"""Seeded input generators: exact pore fraction, discrete divergence-free and
+wall-tight velocity, balanced gradient lobes."""
+import numpy as np
+import pytest
+
+from initializator_amd import generators
+
+
+def test_porosity_exact_fraction_and_seeded():
+    shape = (20, 24, 28)
+    for phi in (0.2, 0.3, 0.35):
+        m = generators.porosity(shape, phi, seed=7)
+        assert m.dtype == np.uint8 and set(np.unique(m)) <= {0, 1}
+        assert int(m.sum()) == int(round(phi * m.size))
+    a = generators.porosity(shape, 0.3, seed=7)
+    assert np.array_equal(a, generators.porosity(shape, 0.3, seed=7))
+    assert not np.array_equal(a, generators.porosity(shape, 0.3, seed=8))
+
+
+def test_porosity_is_spatially_correlated():
+    m = generators.porosity((40, 40, 40), 0.3, seed=1, sigma=3.0).astype(float)
+    # neighbouring voxels agree far more often than independent draws would
+    agree = (m[:, :, 1:] == m[:, :, :-1]).mean()
+    assert agree > 0.85
+
+
+def test_velocity_divergence_free_and_confined():
+    shape = (18, 15, 21)
+    mask = generators.porosity(shape, 0.45, seed=4, sigma=1.5)
+    d = 1e-6
+    vx, vy, vz = generators.velocity(mask, seed=4, sigma=2.0, vmax=1e-3, d=(d, d, d))
+    assert vx.shape == (18, 15, 22) and vy.shape == (18, 16, 21) and vz.shape == (19, 15, 21)
+    div = generators.divergence(vx, vy, vz, (d, d, d))
+    assert np.abs(div).max() * d < 1e-15  # relative to vmax ~ 1e-3
+    peak = max(np.abs(vx).max(), np.abs(vy).max(), np.abs(vz).max())
+    assert peak == pytest.approx(1e-3, rel=1e-12)
+    p = np.zeros((20, 17, 23), bool)
+    p[1:-1, 1:-1, 1:-1] = mask != 0
+    wet_x = p[1:-1, 1:-1, :-1] & p[1:-1, 1:-1, 1:]
+    wet_y = p[1:-1, :-1, 1:-1] & p[1:-1, 1:, 1:-1]
+    wet_z = p[:-1, 1:-1, 1:-1] & p[1:, 1:-1, 1:-1]
+    assert np.all(vx[~wet_x] == 0) and np.all(vy[~wet_y] == 0) and np.all(vz[~wet_z] == 0)
+    assert np.count_nonzero(vx) > 0
+
+
+def test_pgse_schedule_balanced():
+    dt = 2.8986e-5
+    s = generators.pgse_schedule(5000, dt, 0.3, 2e-3, 20e-3)
+    assert s.shape == (5000, 3) and np.all(s[:, :2] == 0)
+    nd = round(2e-3 / dt)
+    assert np.count_nonzero(s[:, 2] > 0) == nd == np.count_nonzero(s[:, 2] < 0)
+    assert s[:, 2].sum() == pytest.approx(0.0, abs=1e-12)
+    assert s[round(20e-3 / dt), 2] == -0.3
+
+
+def test_b_value_round_trip():
+    g = generators.gradient_for_b(1e9, 2e-3, 20e-3)
+    assert generators.b_value(g, 2e-3, 20e-3) == pytest.approx(1e9, rel=1e-12)
+
+
+def test_stable_dt_config_value():
+    assert generators.stable_dt(1e-6, 2.3e-9) == pytest.approx(2.8986e-5, rel=1e-4)

@@ -1,0 +1,187 @@
+"""Parity of the gfx950 FP32 solve (through the C ABI) with the FP64 oracle.
+
+Bars (stated per case in the assertion messages):
+  * signal curve: relative L2 over all steps and components <= 1e-5
+  * field: max |M_gpu - M_oracle| over all voxels <= 1e-4 (|M| <= ~1)
+Integer/structure properties (tile-shape invariance, determinism) are
+checked bit-exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from initializator_amd import generators, layout
+from initializator_amd._lib import NMR_LAYOUT_PADDED
+from initializator_amd.solver import HipSolver, variants
+
+pytestmark = pytest.mark.gpu
+
+SIG_TOL = 1e-5
+VOX_TOL = 1e-4
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _gpu_run(case, nsteps, m0, **kw):
+    s = HipSolver.from_case(case, **kw)
+    s.upload_magnetization(*m0[: case.ncomp])
+    sig = s.run(nsteps, None if case.grad_sched is None else case.grad_sched[:nsteps])
+    fields = s.download_magnetization()
+    s.close()
+    return sig, fields
+
+
+def _report(name, sig_gpu, sig_ref, f_gpu, f_ref):
+    e_sig = rel_l2(sig_gpu, sig_ref)
+    e_vox = max(float(np.abs(a - b).max()) for a, b in zip(f_gpu, f_ref))
+    print(f"[parity] {name}: signal relL2 = {e_sig:.3e}, max voxel = {e_vox:.3e}", flush=True)
+    return e_sig, e_vox
+
+
+def test_small_plain_real():
+    """64^3, 30% pores, no flow, no gradient, real M = 1, 2000 steps."""
+    case = generators.case_small()
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    sig, f = _gpu_run(case, case.nsteps, m0)
+    e_sig, e_vox = _report("64^3 plain 2000 steps", sig[:, 0], sig_ref[:, 0], f[:1], f_ref[:1])
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
+
+
+def _flow_case(shape, nsteps, g=0.3, phi=0.3, seed=77):
+    c = generators.case_flow(seed=seed, shape=shape, phi=phi, g=g, nsteps=nsteps, big_delta=6e-3)
+    return c
+
+
+def test_flow_pfg_complex_uneven_shape():
+    """Flow + PGSE gradient along z, complex (Mx, My), nx not a multiple of 4."""
+    case = _flow_case((40, 44, 37), 500)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    sig, f = _gpu_run(case, case.nsteps, m0)
+    e_sig, e_vox = _report("40x44x37 flow+PGSE 500 steps", sig, sig_ref[:, :2], f, f_ref[:2])
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
+
+
+def test_general_three_components():
+    """Tilted B0, gradients on all three axes, flow, pEq != 0: full 3-component
+    update (relaxation projection and M x Bg in every component)."""
+    shape = (13, 17, 21)
+    case = _flow_case(shape, 150, phi=0.45, seed=5)
+    rng = np.random.default_rng(3)
+    sched = np.zeros((case.nsteps, 3))
+    sched[:60] = (0.05, -0.1, 0.2)
+    sched[90:120] = (-0.2, 0.1, 0.05)
+    case.grad_sched = sched
+    case.ncomp = 3
+    case.t1_bulk = 0.02
+    case.t2_bulk = 0.01
+    b0 = (0.3, -0.5, 0.8)
+    peq = 0.4
+    m0 = [rng.random(shape) * case.mask for _ in range(3)]
+    p, act, D, t1, t2, vel, m = oracle.setup_case(case, p_eq=peq, m0=m0)
+    for i in range(3):
+        p.b0[i] = b0[i]
+    sig_ref = oracle.run(p, act, D, t1, t2, vel, case.nsteps, m, grad_sched=sched)
+    f_ref = [layout.unpad(x) for x in m]
+    s = HipSolver.from_case(case, b0_dir=b0, p_eq=peq)
+    s.upload_magnetization(*m0)
+    sig = s.run(case.nsteps, sched)
+    f = s.download_magnetization()
+    s.close()
+    e_sig, e_vox = _report("13x17x21 ncomp 3", sig, sig_ref, f, f_ref)
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
+
+
+def test_tile_shapes_bitwise_identical():
+    """Every tile variant and z-chunk gives bit-identical fields (the tiling
+    does not change any voxel's arithmetic) and signals equal to 1e-12."""
+    case = _flow_case((70, 29, 33), 40, seed=9)
+    m0 = [case.mask.astype(np.float64), 0.5 * case.mask, np.zeros(case.shape)]
+    base_sig, base_f = None, None
+    for v, lx, rpt, tx, ty in variants():
+        for lz in (1, 7, 70):
+            s = HipSolver.from_case(case)
+            s.set_tile(v, lz)
+            s.upload_magnetization(*m0[:2])
+            sig = s.run(case.nsteps, case.grad_sched)
+            f = s.download_magnetization()
+            s.close()
+            if base_f is None:
+                base_sig, base_f = sig, f
+                continue
+            for a, b in zip(f, base_f):
+                assert np.array_equal(a, b), (v, lz)
+            np.testing.assert_allclose(sig, base_sig, rtol=1e-12, atol=1e-12)
+
+
+def test_deterministic_repeat():
+    case = _flow_case((32, 36, 40), 60, seed=21)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    a = _gpu_run(case, case.nsteps, m0)
+    b = _gpu_run(case, case.nsteps, m0)
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+
+
+def test_padded_layout_roundtrip_and_trimmed_signal():
+    """Reference (padded, '0'/'1' chars) uploads give the same state as compact
+    ones; the signal honours bufferSizeX/Y/Z trimming (cpp:3301-3322)."""
+    case = _flow_case((12, 14, 18), 10, seed=2)
+    rng = np.random.default_rng(0)
+    mx = rng.random(case.shape) * case.mask
+    my = rng.random(case.shape) * case.mask
+    buf = (2, 1, 3)
+    s = HipSolver.from_case(case, buffer=buf)
+    s.upload_mask(layout.pad_mask(case.mask), NMR_LAYOUT_PADDED)
+    px, py, pz = layout.pad_velocity(case.vx, case.vy, case.vz, case.shape)
+    s.upload_velocity(px, py, pz, NMR_LAYOUT_PADDED)
+    s.upload_magnetization(layout.pad_field(mx), layout.pad_field(my), None, NMR_LAYOUT_PADDED)
+    got = s.download_magnetization(NMR_LAYOUT_PADDED)
+    np.testing.assert_array_equal(got[0], layout.pad_field(mx.astype(np.float32)))
+    sig0 = s.signal()
+    sl = (slice(buf[2], 12 - buf[2]), slice(buf[1], 14 - buf[1]), slice(buf[0], 18 - buf[0]))
+    want = [(x.astype(np.float32).astype(np.float64) * case.mask)[sl].sum() for x in (mx, my)]
+    np.testing.assert_allclose(sig0, want, rtol=1e-12)
+    sig = s.run(case.nsteps, case.grad_sched)
+    f = s.download_magnetization()
+    last = [(x * case.mask)[sl].sum() for x in f]
+    np.testing.assert_allclose(sig[-1], last, rtol=1e-12)
+    s.close()
+
+
+def test_full_size_256_short_parity_and_reduction():
+    """256^3, 20% pores, flow + PGSE: 20 steps against the FP64 oracle at full
+    size, and the fused reduction equals an FP64 sum of the downloaded field."""
+    case = generators.case_flow(n=256, phi=0.20, nsteps=20)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    s = HipSolver.from_case(case)
+    s.upload_magnetization(*m0[:2])
+    sig = s.run(case.nsteps, case.grad_sched)
+    f = s.download_magnetization()
+    s.close()
+    e_sig, e_vox = _report("256^3 flow+PGSE 20 steps", sig, sig_ref[:, :2], f, f_ref[:2])
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL
+    brute = [float((x * case.mask).sum()) for x in f]
+    np.testing.assert_allclose(sig[-1], brute, rtol=1e-12)
+
+
+def test_full_size_256_uniform_field_fixed_point():
+    """Size-independent invariant at 256^3: with flow and diffusion but no
+    relaxation or gradient, a uniform pore field stays uniform."""
+    case = generators.case_flow(n=256, phi=0.35, nsteps=50, rho=0.0)
+    case.t2_bulk = 1e30
+    case.grad_sched = None
+    s = HipSolver.from_case(case, precession=False)
+    s.upload_magnetization(0.75 * case.mask, 0.25 * case.mask)
+    sig = s.run(case.nsteps)
+    f = s.download_magnetization()
+    s.close()
+    pores = case.mask != 0
+    assert np.abs(f[0][pores] - 0.75).max() < 1e-6
+    assert np.abs(f[1][pores] - 0.25).max() < 1e-6
+    np.testing.assert_allclose(sig[:, 0], sig[0, 0], rtol=1e-6)
