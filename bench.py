@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -54,6 +56,29 @@ def cpu_baseline(case, target_s: float):
     }
 
 
+def copy_probe(mib: int) -> int:
+    """Plain copy of a known size: its GB/s is the practical HBM ceiling, and
+    its FETCH_SIZE / WRITE_SIZE counters give the bytes-per-KiB scale."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_mib": mib, "ms": ms, "GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +90,11 @@ def main() -> int:
     ap.add_argument("--lz", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="time a plain device-to-device copy of MIB MiB instead (HBM ceiling / counter scale)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the solve keeps the whole domain on one MI355X "
               "(multi-GPU z-slab sharding is a design note, not built)", file=sys.stderr)
@@ -78,7 +107,7 @@ def main() -> int:
     case = generators.case_flow(n=args.n, phi=args.phi, nsteps=total)
     s = HipSolver.from_case(case)
     s.set_tile(args.variant, args.lz)
-    s.upload_magnetization(case.mask.astype("float64"), None)
+    s.upload_magnetization(case.mask.astype("float64"), np.zeros(case.shape))
     s.run(args.warmup, case.grad_sched[: args.warmup], want_signal=False)
     t0 = time.perf_counter()
     s.run(args.steps, case.grad_sched[args.warmup :], want_signal=True)

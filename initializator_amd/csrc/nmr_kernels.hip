@@ -20,7 +20,7 @@
 //    the update, taken as 0), 1/T = 1/T_bulk + rho/d per solid face, and
 //    Bg = G*d*(index+1/2).
 //  * The signal (OutputX/Y/Z, cpp:3301-3461) is reduced inside the sweep:
-//    FP32 per lane and plane, FP64 across planes, waves and the workgroup,
+//    every updated FP32 voxel is added in FP64 per lane, then across waves and the workgroup,
 //    one FP64 partial per workgroup per step, summed in a fixed order by
 //    nmr_reduce_partials -- deterministic, no atomics.
 //
@@ -397,11 +397,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
             if (trim_z && trim_y) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) {
-                    float s = 0.f;
 #pragma unroll
                     for (int e = 0; e < 4; e++)
-                        if (trim_x[e] && mbyte(kc[r], e) != 0) s += out[q][e];
-                    acc[q] += (double)s;
+                        if (trim_x[e] && mbyte(kc[r], e) != 0) acc[q] += (double)out[q][e];
                 }
             }
         }
