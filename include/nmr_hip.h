@@ -125,11 +125,13 @@ int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
 /* Tile shape of the sweep kernel: variant index (-1 = automatic) and z-chunk
  * length (0 = automatic).  nmr_hip_num_variants() gives the variant count;
- * nmr_hip_variant_info() describes one (lanes per row, rows per thread). */
+ * nmr_hip_variant_info() describes one: lanes per row, rows per thread, tile
+ * width and height, and time steps per pass (2 = temporally blocked sweep,
+ * usable when nx <= tile_x).  Every variant gives bit-identical fields. */
 int nmr_hip_set_tile(nmr_ctx* ctx, int32_t variant, int32_t lz);
 int nmr_hip_num_variants(void);
 int nmr_hip_variant_info(int32_t variant, int32_t* lanes_x, int32_t* rows_per_thread,
-                         int32_t* tile_x, int32_t* tile_y);
+                         int32_t* tile_x, int32_t* tile_y, int32_t* steps_per_pass);
 /* Bytes of device memory the context holds. */
 int nmr_hip_device_bytes(nmr_ctx* ctx, int64_t* bytes);
 

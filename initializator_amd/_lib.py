@@ -129,7 +129,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_num_variants": (c_int, []),
         "nmr_hip_variant_info": (
             c_int,
-            [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)],
+            [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)],
         ),
         "nmr_hip_device_bytes": (c_int, [vp, POINTER(c_int64)]),
         "nmr_hip_strerror": (c_char_p, [c_int]),

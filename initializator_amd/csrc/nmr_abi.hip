@@ -72,6 +72,9 @@ struct nmr_ctx {
     size_t dsignal_cap = 0;  // doubles
     int cur = 0;
     bool has_mask = false, has_mag = false;
+    bool inert_dirty = true; // mask or velocity changed since the last check
+    int inert = 0;           // no pore-solid face carries velocity
+    int* dflag = nullptr;
     int variant = -1, lz = 0; // -1 / 0 = automatic
     double last_ms = 0.0;
     int64_t last_sweeps = 0;
@@ -108,6 +111,7 @@ void free_all(nmr_ctx* c)
     if (c->vz) (void)hipFree(c->vz);
     if (c->partials) (void)hipFree(c->partials);
     if (c->dsignal) (void)hipFree(c->dsignal);
+    if (c->dflag) (void)hipFree(c->dflag);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -120,23 +124,46 @@ inline size_t pidx(const nmr_problem& p, int i, int j, int k)
 }
 
 // Tile shape picked when the caller leaves it automatic.
-void auto_tile(const nmr_ctx* c, int* variant, int* lz)
+struct Launch {
+    int variant, lx, rpt, tx, ty, steps, lz;
+    int ntx, nty, ntz, nwg;
+};
+
+void fill_launch(const nmr_problem& p, Launch* L)
 {
-    const int nx = c->prob.nx;
-    int v;
-    if (nx > 128) v = 0;      // TX 256, TY 4
-    else if (nx > 64) v = 2;  // TX 128, TY 8
-    else if (nx > 32) v = 4;  // TX 64, TY 16
-    else v = 6;               // TX 32, TY 32
-    int lx, rpt, tx, ty;
-    nmr::variant_shape(v, &lx, &rpt, &tx, &ty);
-    const long tiles_xy = (long)((nx + tx - 1) / tx) * (long)((c->prob.ny + ty - 1) / ty);
-    // aim for >= 2048 workgroups (8 per CU) while keeping chunks >= 8 planes
-    long want = 2048;
-    int z = (int)std::max<long>(8, (long)c->prob.nz * tiles_xy / want);
-    z = std::min(z, c->prob.nz);
-    *variant = v;
-    *lz = std::max(1, z);
+    nmr::variant_shape(L->variant, &L->lx, &L->rpt, &L->tx, &L->ty, &L->steps);
+    L->lz = std::min(std::max(L->lz, 1), p.nz);
+    L->ntx = (p.nx + L->tx - 1) / L->tx;
+    L->nty = (p.ny + L->ty - 1) / L->ty;
+    L->ntz = (p.nz + L->lz - 1) / L->lz;
+    const long n = (long)L->ntx * L->nty * L->ntz;
+    L->nwg = n > (1L << 30) ? -1 : (int)n;
+}
+
+void auto_launch(const nmr_ctx* c, int variant, int lz, Launch* L)
+{
+    const nmr_problem& p = c->prob;
+    const int n1 = 7; // first two-step variant
+    (void)n1;
+    if (variant < 0) { // measured fastest one-step tiles (two-step variants are opt-in)
+        if (p.nx > 128) variant = 0;      // TX 256, TY 4
+        else if (p.nx > 64) variant = 2;  // TX 128, TY 8
+        else if (p.nx > 32) variant = 4;  // TX 64, TY 16
+        else variant = 6;                 // TX 32, TY 32
+    }
+    L->variant = variant;
+    L->lz = 1;
+    fill_launch(p, L);
+    if (lz <= 0) {
+        // one-step: aim for >= 2048 workgroups (8 per CU) with >= 8 planes;
+        // two-step (one 8-wave workgroup per CU): >= 512 workgroups
+        const long want = L->steps == 2 ? 512 : 2048;
+        const long tiles_xy = (long)L->ntx * L->nty;
+        const long ntz = std::max<long>(1, (want + tiles_xy - 1) / tiles_xy);
+        lz = (int)std::max<long>(L->steps == 2 ? 4 : 8, (p.nz + ntz - 1) / ntz);
+    }
+    L->lz = lz;
+    fill_launch(p, L);
 }
 
 nmr::Coef make_coef(const nmr_ctx* c, const double g[3])
@@ -238,6 +265,36 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
     return NMR_OK;
 }
 
+// Whether solid magnetization can reach a pore (a face touching a solid
+// carries velocity); recomputed on the device after mask/velocity uploads.
+int refresh_inert(nmr_ctx* c)
+{
+    if (!c->inert_dirty) return NMR_OK;
+    if (!c->vel) {
+        c->inert = 1;
+    } else {
+        HIP_TRY(hipMemsetAsync(c->dflag, 0, sizeof(int), c->stream));
+        nmr::InertArgs a{};
+        a.mask = c->mask;
+        a.vx = c->vx;
+        a.vy = c->vy;
+        a.vz = c->vz;
+        a.flag = c->dflag;
+        a.nx = c->prob.nx;
+        a.ny = c->prob.ny;
+        a.nz = c->prob.nz;
+        a.pitch = c->pitch;
+        a.pitch1 = c->pitch1;
+        HIP_TRY(nmr::launch_inert_check(a, c->stream));
+        int h = 0;
+        HIP_TRY(hipMemcpyAsync(&h, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->inert = h == 0;
+    }
+    c->inert_dirty = false;
+    return NMR_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -263,14 +320,16 @@ const char* nmr_hip_last_error(void) { return g_last_error.c_str(); }
 int nmr_hip_num_variants(void) { return nmr::num_variants(); }
 
 int nmr_hip_variant_info(int32_t variant, int32_t* lanes_x, int32_t* rows_per_thread, int32_t* tile_x,
-                         int32_t* tile_y)
+                         int32_t* tile_y, int32_t* steps_per_pass)
 {
-    int lx, rpt, tx, ty;
-    if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty)) return fail(NMR_ERR_INVALID, "no variant %d", (int)variant);
+    int lx, rpt, tx, ty, st;
+    if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st))
+        return fail(NMR_ERR_INVALID, "no variant %d", (int)variant);
     if (lanes_x) *lanes_x = lx;
     if (rows_per_thread) *rows_per_thread = rpt;
     if (tile_x) *tile_x = tx;
     if (tile_y) *tile_y = ty;
+    if (steps_per_pass) *steps_per_pass = st;
     return NMR_OK;
 }
 
@@ -312,6 +371,12 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     c->pitch = round_up4(p.nx);
     c->pitch1 = round_up4(p.nx + 1);
     c->n_m = (size_t)p.nz * p.ny * c->pitch;
+    // the sweep addresses every array with 32-bit byte offsets
+    const size_t biggest = std::max((size_t)(p.nz + 1) * p.ny * c->pitch, (size_t)p.nz * p.ny * c->pitch1) * 4;
+    if (biggest >= ((size_t)1 << 32) - 64) {
+        delete c;
+        return fail(NMR_ERR_UNSUPPORTED, "arrays of %zu bytes exceed the 4 GiB addressing of the sweep", biggest);
+    }
     auto bail = [&](int rc) {
         free_all(c);
         delete c;
@@ -328,6 +393,7 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
         for (int q = 0; q < c->nc; q++)
             if ((rc = dev_alloc(c, &c->m[b][q], c->n_m))) return bail(rc);
     if ((rc = dev_alloc(c, &c->mask, c->n_m))) return bail(rc);
+    if ((rc = dev_alloc(c, &c->dflag, 64))) return bail(rc);
     if (c->vel) {
         c->n_vx = (size_t)p.nz * p.ny * c->pitch1;
         c->n_vy = (size_t)p.nz * (p.ny + 1) * c->pitch;
@@ -382,6 +448,7 @@ int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout)
     HIP_TRY(hipMemcpyAsync(ctx->mask, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->has_mask = true;
+    ctx->inert_dirty = true;
     return NMR_OK;
 }
 
@@ -435,6 +502,7 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
         HIP_TRY(hipMemcpyAsync(ctx->vz, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
+    ctx->inert_dirty = true;
     return NMR_OK;
 }
 
@@ -457,8 +525,11 @@ int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* m
                                          : src[q][pidx(p, i + 1, j + 1, k + 1)];
                     h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i] = (float)v;
                 }
-        HIP_TRY(hipMemcpyAsync(ctx->m[ctx->cur][q], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
-                               ctx->stream));
+        // both ping-pong buffers: the sweeps do not rewrite all-solid chunks,
+        // whose values therefore have to agree in the two buffers
+        for (int b = 0; b < 2; b++)
+            HIP_TRY(hipMemcpyAsync(ctx->m[b][q], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                                   ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     ctx->has_mag = true;
@@ -546,28 +617,26 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                 return fail(NMR_ERR_UNSUPPORTED, "precession needs ncomp >= 2");
         }
     }
-    int variant = ctx->variant, lz = ctx->lz;
-    {
-        int av, alz;
-        auto_tile(ctx, &av, &alz);
-        if (variant < 0) variant = av;
-        if (lz <= 0) lz = alz;
-    }
-    int lx, rpt, tx, ty;
-    nmr::variant_shape(variant, &lx, &rpt, &tx, &ty);
-    lz = std::min(lz, p.nz);
-    const int ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty, ntz = (p.nz + lz - 1) / lz;
-    const long nwg_l = (long)ntx * nty * ntz;
-    if (nwg_l > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
-    const int nwg = (int)nwg_l;
+    int rc0 = refresh_inert(ctx);
+    if (rc0) return rc0;
+    Launch L1{}, L2{};
+    auto_launch(ctx, ctx->variant, ctx->lz, &L1);
+    if (L1.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
+    if (L1.steps == 2 && p.nx > L1.tx)
+        return fail(NMR_ERR_INVALID, "variant %d needs nx <= %d", L1.variant, L1.tx);
+    // one-step launch for an odd last step of a two-step run
+    auto_launch(ctx, L1.steps == 2 ? 1 : L1.variant, L1.steps == 2 ? 0 : L1.lz, &L2);
     const int nc = ctx->nc;
 
     const int64_t rows = nsteps + 1;
     int rc = ensure_signal(ctx, (size_t)rows * nc);
     if (rc) return rc;
-    rc = ensure_partials(ctx, std::max<size_t>((size_t)kChunkSteps * nwg * nc, (size_t)p.nz * nc));
+    rc = ensure_partials(ctx, std::max({(size_t)kChunkSteps * L1.nwg * nc, (size_t)kChunkSteps * L2.nwg * nc,
+                                        (size_t)p.nz * nc}));
     if (rc) return rc;
 
+    int w[6];
+    signal_window(p, w);
     nmr::SweepArgs a{};
     a.mask = ctx->mask;
     a.vx = ctx->vx;
@@ -578,43 +647,74 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     a.nz = p.nz;
     a.pitch = ctx->pitch;
     a.pitch1 = ctx->pitch1;
-    a.ntx = ntx;
-    a.nty = nty;
-    a.ntz = ntz;
-    a.lz = lz;
-    a.nblocks = nwg;
-    int w[6];
-    signal_window(p, w);
     a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
+    a.inert = ctx->inert;
+    nmr::Sweep2Args b{};
+    b.mask = ctx->mask;
+    b.vx = ctx->vx;
+    b.vy = ctx->vy;
+    b.vz = ctx->vz;
+    b.nx = p.nx;
+    b.ny = p.ny;
+    b.nz = p.nz;
+    b.pitch = ctx->pitch;
+    b.pitch1 = ctx->pitch1;
+    b.bx0 = w[0]; b.bx1 = w[1]; b.by0 = w[2]; b.by1 = w[3]; b.bz0 = w[4]; b.bz1 = w[5];
+
+    auto coef_of = [&](int64_t step) {
+        double g[3] = {p.grad[0], p.grad[1], p.grad[2]};
+        if (grad_sched) {
+            g[0] = grad_sched[3 * step];
+            g[1] = grad_sched[3 * step + 1];
+            g[2] = grad_sched[3 * step + 2];
+        }
+        return make_coef(ctx, g);
+    };
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if ((rc = enqueue_field_sum(ctx, 0))) return rc;
-    double gcur[3] = {p.grad[0], p.grad[1], p.grad[2]};
-    nmr::Coef coef = make_coef(ctx, gcur);
     int64_t done = 0;
     while (done < nsteps) {
-        const int chunk = (int)std::min<int64_t>(kChunkSteps, nsteps - done);
-        for (int s = 0; s < chunk; s++) {
-            const int64_t step = done + s;
-            if (grad_sched) {
-                const double* g = grad_sched + 3 * step;
-                if (g[0] != gcur[0] || g[1] != gcur[1] || g[2] != gcur[2]) {
-                    gcur[0] = g[0];
-                    gcur[1] = g[1];
-                    gcur[2] = g[2];
-                    coef = make_coef(ctx, gcur);
+        const int64_t left = nsteps - done;
+        const bool two = L1.steps == 2 && left >= 2;
+        const Launch& L = two ? L1 : L2;
+        int chunk = (int)std::min<int64_t>(kChunkSteps, left);
+        if (two) chunk &= ~1;
+        if (two) {
+            b.nty = L.nty;
+            b.ntz = L.ntz;
+            b.lz = L.lz;
+            b.nblocks = L.nwg;
+            for (int s = 0; s < chunk; s += 2) {
+                b.c1 = coef_of(done + s);
+                b.c2 = coef_of(done + s + 1);
+                for (int q = 0; q < 3; q++) {
+                    b.min[q] = ctx->m[ctx->cur][q];
+                    b.mout[q] = ctx->m[ctx->cur ^ 1][q];
                 }
+                b.partials1 = ctx->partials + (size_t)s * L.nwg * nc;
+                b.partials2 = ctx->partials + (size_t)(s + 1) * L.nwg * nc;
+                HIP_TRY(nmr::launch_sweep2(nc, ctx->vel, ctx->exact, L.lx, b, ctx->stream));
+                ctx->cur ^= 1;
             }
-            a.c = coef;
-            for (int q = 0; q < 3; q++) {
-                a.min[q] = ctx->m[ctx->cur][q];
-                a.mout[q] = ctx->m[ctx->cur ^ 1][q];
+        } else {
+            a.ntx = L.ntx;
+            a.nty = L.nty;
+            a.ntz = L.ntz;
+            a.lz = L.lz;
+            a.nblocks = L.nwg;
+            for (int s = 0; s < chunk; s++) {
+                a.c = coef_of(done + s);
+                for (int q = 0; q < 3; q++) {
+                    a.min[q] = ctx->m[ctx->cur][q];
+                    a.mout[q] = ctx->m[ctx->cur ^ 1][q];
+                }
+                a.partials = ctx->partials + (size_t)s * L.nwg * nc;
+                HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, L.variant, a, ctx->stream));
+                ctx->cur ^= 1;
             }
-            a.partials = ctx->partials + (size_t)s * nwg * nc;
-            HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, variant, a, ctx->stream));
-            ctx->cur ^= 1;
         }
-        HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, nwg, nc, ctx->dsignal + (size_t)(done + 1) * nc,
+        HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, L.nwg, nc, ctx->dsignal + (size_t)(done + 1) * nc,
                                             ctx->stream));
         done += chunk;
     }
@@ -626,7 +726,7 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
-    ctx->last_sweeps = nsteps;
+    ctx->last_sweeps = L1.steps == 2 ? nsteps / 2 + (nsteps & 1) : nsteps;
     return NMR_OK;
 }
 

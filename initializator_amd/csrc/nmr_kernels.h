@@ -31,7 +31,35 @@ struct SweepArgs {
     int pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
+    int inert;              // no pore-solid face carries velocity
     Coef c;
+};
+
+struct InertArgs {
+    const uint8_t* mask;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    int* flag;
+    int nx, ny, nz, pitch, pitch1;
+};
+
+// Two-step (temporally blocked) sweep: one x tile (nx <= 4*LX), TY = 512/LX - 2
+// rows per tile, z-chunks of lz planes.
+struct Sweep2Args {
+    const float* min[3];
+    float* mout[3];
+    const uint8_t* mask;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    double* partials1;      // [nblocks][nc] signal after the first step
+    double* partials2;      // [nblocks][nc] signal after the second step
+    int nx, ny, nz;
+    int pitch, pitch1;
+    int nty, ntz, lz, nblocks;
+    int bx0, bx1, by0, by1, bz0, bz1;
+    Coef c1, c2;            // coefficients of the two steps
 };
 
 struct FieldSumArgs {
@@ -43,9 +71,12 @@ struct FieldSumArgs {
 };
 
 int num_variants();
-bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty);
+// steps: time steps per pass (1, or 2 for the temporally blocked sweep)
+bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);
 hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s);
+hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
+hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
 hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s);
 
 } // namespace nmr

@@ -38,30 +38,36 @@
 
 #include "nmr_kernels.h"
 
+
 namespace nmr {
 
-__device__ __forceinline__ void load4(float (&d)[4], const float* p)
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Global accesses take a scalar base pointer plus a 32-bit byte offset, so
+// they map onto the scalar-base + 32-bit-VGPR-offset addressing form.
+__device__ __forceinline__ f4 ld4_nt(const float* base, uint32_t boff)
 {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    d[0] = v.x;
-    d[1] = v.y;
-    d[2] = v.z;
-    d[3] = v.w;
+    return __builtin_nontemporal_load(reinterpret_cast<const f4*>(reinterpret_cast<const char*>(base) + boff));
+}
+__device__ __forceinline__ f4 ld4(const float* base, uint32_t boff)
+{
+    return *reinterpret_cast<const f4*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ __forceinline__ float ld1(const float* base, uint32_t boff)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ __forceinline__ void st4_nt(float* base, uint32_t boff, f4 v)
+{
+    __builtin_nontemporal_store(v, reinterpret_cast<f4*>(reinterpret_cast<char*>(base) + boff));
+}
+__device__ __forceinline__ uint32_t ldw_nt(const uint8_t* base, uint32_t boff)
+{
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(base + boff));
 }
 
-__device__ __forceinline__ void store4(float* p, const float (&s)[4])
-{
-    float4 v;
-    v.x = s[0];
-    v.y = s[1];
-    v.z = s[2];
-    v.w = s[3];
-    *reinterpret_cast<float4*>(p) = v;
-}
-
-__device__ __forceinline__ float pick(bool in, float nb, float self) { return in ? nb : self; }
-
-__device__ __forceinline__ int mbyte(uint32_t w, int e) { return (int)((w >> (8 * e)) & 0xffu); }
+// byte e (0/1) of a packed mask word as a float
+__device__ __forceinline__ float ub(uint32_t w, int e) { return (float)((w >> (8 * e)) & 0xffu); }
 
 // Block index -> tile index such that XCD x (blocks x, x+8, x+16, ...) gets
 // the contiguous tile range [x*q + min(x,r), ...), q = nb/8, r = nb%8.
@@ -72,6 +78,166 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return xcd * q + (xcd < r ? xcd : r) + ib;
 }
 
+// Replace the elements of a chunk that lie beyond the domain end (index >
+// last) by element `last`: the zero-derivative ghost value.
+__device__ __forceinline__ f4 clamp_tail(f4 v, int last)
+{
+    const float t = last == 0 ? v.x : (last == 1 ? v.y : v.z);
+    if (last < 3) v.w = t;
+    if (last < 2) v.z = t;
+    if (last < 1) v.y = t;
+    return v;
+}
+__device__ __forceinline__ uint32_t clamp_tail_w(uint32_t w, int last)
+{
+    if (last >= 3) return w;
+    const uint32_t t = (w >> (8 * last)) & 0xffu;
+    const uint32_t keep = (1u << (8 * (last + 1))) - 1u;
+    return (w & keep) | ((t * 0x01010101u) & ~keep);
+}
+
+// Update of one 4-voxel x-chunk (ExplicitSimple, cpp:3174-3262), shared by the
+// one-step and the two-step sweeps so that both give bit-identical voxels.
+// Inputs: centre values mc, z/y neighbour chunks, x neighbours xl (x-1 of
+// element 0) and xr (x+1 of element 3), packed 0/1 mask words of the chunk and
+// of its six neighbour chunks (kxm/kxp already shifted by one voxel), and the
+// staggered face velocities.  Ghost cells hold zero-derivative copies, so no
+// bounds tests are needed.  Solid voxels keep their value.
+template <int NC, bool VEL, bool EXACT>
+__device__ __forceinline__ void update_chunk(const Coef& cf, const f4 (&mc)[NC], const f4 (&fzm)[NC],
+                                             const f4 (&fzp)[NC], const f4 (&fym)[NC], const f4 (&fyp)[NC],
+                                             const float (&xl)[NC], const float (&xr)[NC], uint32_t kc,
+                                             uint32_t kxm, uint32_t kxp, uint32_t kym, uint32_t kyp,
+                                             uint32_t kzm, uint32_t kzp, const f4& vx4, float vxe, const f4& vy4,
+                                             const f4& vy4n, const f4& vz0, const f4& vz1, int ix, float thy,
+                                             float thz, float sn, float cm1, f4 (&out)[NC])
+{
+    const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
+    // solid neighbours per axis, one byte per voxel
+    const uint32_t solx = 0x02020202u - (kxm + kxp);
+    const uint32_t soly = 0x02020202u - (kym + kyp);
+    const uint32_t solz = 0x02020202u - (kzm + kzp);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        // face weights: diffusion between pores minus upwind advection.
+        // Every fused multiply-add is written out (the file is built with
+        // -ffp-contract=off) so that the arithmetic of a voxel does not depend
+        // on the kernel or tile shape it was compiled into.
+        float wxm = cf.cdx * ub(kxm, e), wxp = cf.cdx * ub(kxp, e);
+        float wym = cf.cdy * ub(kym, e), wyp = cf.cdy * ub(kyp, e);
+        float wzm = cf.cdz * ub(kzm, e), wzp = cf.cdz * ub(kzp, e);
+        if constexpr (VEL) {
+            const float um = vx4[e], up = (e < 3) ? vx4[e + 1] : vxe;
+            const float vm = vy4[e], vp = vy4n[e];
+            const float wm = vz0[e], wp = vz1[e];
+            const float su = um + up, sv = vm + vp, sw = wm + wp;
+            wxm = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), wxm) : wxm;
+            wxp = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), wxp) : wxp;
+            wym = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), wym) : wym;
+            wyp = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), wyp) : wyp;
+            wzm = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), wzm) : wzm;
+            wzp = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), wzp) : wzp;
+        }
+        const float nsx = ub(solx, e), nsy = ub(soly, e), nsz = ub(solz, e);
+        const float rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+        float m0[NC], d[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const float m = mc[q][e];
+            m0[q] = m;
+            const float fxm = (e > 0) ? mc[q][e - 1] : xl[q];
+            const float fxp = (e < 3) ? mc[q][e + 1] : xr[q];
+            float t = wxm * (fxm - m);
+            t = fmaf(wxp, fxp - m, t);
+            t = fmaf(wym, fym[q][e] - m, t);
+            t = fmaf(wyp, fyp[q][e] - m, t);
+            t = fmaf(wzm, fzm[q][e] - m, t);
+            d[q] = fmaf(wzp, fzp[q][e] - m, t);
+        }
+        if constexpr (NC == 3) {
+            const float rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
+            const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
+            const float nn[3] = {cf.n0, cf.n1, cf.n2};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float mpar = nn[q] * par;
+                d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
+                d[q] = fmaf(rate2, mpar - m0[q], d[q]);
+            }
+            const float thx = cf.bgx * ((float)(ix + e) + 0.5f);
+            // + dt*g*(m x Bg)
+            d[0] = fmaf(thz, m0[1], fmaf(-thy, m0[2], d[0]));
+            d[1] = fmaf(-thz, m0[0], fmaf(thx, m0[2], d[1]));
+            d[2] = fmaf(thy, m0[0], fmaf(-thx, m0[1], d[2]));
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
+            if constexpr (NC == 2) {
+                if constexpr (EXACT) {
+                    d[0] = fmaf(m0[1], sn, fmaf(m0[0], cm1, d[0]));
+                    d[1] = fmaf(-m0[0], sn, fmaf(m0[1], cm1, d[1]));
+                } else {
+                    d[0] = fmaf(thz, m0[1], d[0]);
+                    d[1] = fmaf(-thz, m0[0], d[1]);
+                }
+            }
+        }
+        const bool pore = ((kc >> (8 * e)) & 0xffu) != 0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) out[q][e] = pore ? m0[q] + d[q] : m0[q];
+    }
+}
+
+// Per-plane scalars of the exact precession rotation.
+template <bool EXACT>
+__device__ __forceinline__ void plane_rotation(float thz, float& sn, float& cm1)
+{
+    sn = 0.f;
+    cm1 = 0.f;
+    if constexpr (EXACT) {
+        sn = sinf(thz);
+        const float h = sinf(0.5f * thz);
+        cm1 = -2.f * h * h; // cos(th) - 1 without cancellation
+    }
+}
+
+// Signal of one updated chunk: pores inside the trim window.
+template <int NC>
+__device__ __forceinline__ void accumulate(double (&acc)[NC], const f4 (&out)[NC], uint32_t kc, const bool (&tx)[4])
+{
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        if (tx[e] && ((kc >> (8 * e)) & 0xffu) != 0) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) acc[q] += (double)out[q][e];
+        }
+}
+
+// Workgroup sum of acc into part[blockIdx.x*NC + q] (fixed order).
+template <int NC, int NT>
+__device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[NC], double* part)
+{
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[q] = v;
+    }
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) sred[wave][q] = acc[q];
+    }
+    __syncthreads();
+    if (tid < NC) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) s += sred[w][tid];
+        part[(size_t)blockIdx.x * NC + tid] = s;
+    }
+}
+
 template <int NC, bool VEL, bool EXACT, int LX, int RPT>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
@@ -79,9 +245,12 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     constexpr int RY = NT / LX;  // thread rows
     constexpr int TX = 4 * LX;   // tile width (voxels)
     constexpr int TY = RY * RPT; // tile height (rows)
-    constexpr int LW = TX + 8;   // LDS row: col 3 = left halo, 4..4+TX-1 own, 4+TX = right halo
-    constexpr int LH = TY + 2;   // row 0 = y0-1 halo, 1..TY own, TY+1 = y0+TY halo
+    constexpr int LW = TX + 8;   // LDS row: col 3 = x0-1, 4..4+TX-1 tile, 4+TX = x0+TX
+    constexpr int LH = TY + 2;   // row 0 = y0-1, 1..TY tile, TY+1 = y0+TY
 
+    // Plane k of the tile plus its one-voxel halo; ghost cells of the domain
+    // boundary hold the zero-derivative copy, so the update reads neighbours
+    // without bounds tests.
     __shared__ __attribute__((aligned(16))) float sm[2][NC][LH][LW];
     __shared__ __attribute__((aligned(16))) uint8_t sk[2][LH][LW];
     __shared__ double sred[NT / 64][NC];
@@ -99,308 +268,264 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     const int k0 = tz * a.lz;
     const int k1 = min(k0 + a.lz, a.nz);
     const int ix = x0 + 4 * lx;
-    const bool xin = ix < a.nx;
-
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const size_t P = (size_t)a.pitch;
-    const size_t plane = (size_t)ny * P;
+    const bool xin = ix < nx;
+    const int xlast = nx - 1 - ix;             // last valid element of this chunk (>= 3: all valid)
+    const bool xend = xin && xlast <= 3;       // chunk holds voxel nx-1
+    const bool xbeg = (ix == 0);               // chunk holds voxel 0
+    const int c = 4 + 4 * lx;                  // LDS column of element 0
+
+    const uint32_t Pb = (uint32_t)a.pitch * 4u;      // bytes per row (float arrays)
+    const uint32_t plane_b = (uint32_t)ny * Pb;      // bytes per plane
+    const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
+    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
+    const uint32_t planex_b = (uint32_t)ny * P1b;
+    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
 
     int iy[RPT];
     bool rin[RPT];
+    uint32_t rowf[RPT], rowm[RPT], rowx[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         iy[r] = y0 + ry + r * RY;
         rin[r] = xin && (iy[r] < ny);
+        rowf[r] = (uint32_t)iy[r] * Pb + (uint32_t)ix * 4u;
+        rowm[r] = (uint32_t)iy[r] * (uint32_t)a.pitch + (uint32_t)ix;
+        rowx[r] = (uint32_t)iy[r] * P1b + (uint32_t)ix * 4u;
     }
-    // halo roles
     const bool top = (ry == 0);
     const bool bot = (ry == RY - 1);
     const int hy = top ? (y0 - 1) : (y0 + TY);
     const bool hrow = (top || bot) && xin && hy >= 0 && hy < ny;
-    const int hl = x0 - 1;  // left halo column
-    const int hr = x0 + TX; // right halo column
-    const bool hcl = (lx == 0) && hl >= 0;
-    const bool hcr = (lx == LX - 1) && hr < nx;
+    const uint32_t hrowf = (uint32_t)(hrow ? hy : 0) * Pb + (uint32_t)ix * 4u;
+    const uint32_t hrowm = (uint32_t)(hrow ? hy : 0) * (uint32_t)a.pitch + (uint32_t)ix;
+    const bool hcl = (lx == 0) && x0 > 0;
+    const bool hcr = (lx == LX - 1) && x0 + TX < nx;
+    const int hcx = (lx == 0) ? x0 - 1 : x0 + TX;
 
-    // register planes
-    float mp[RPT][NC][4], mc[RPT][NC][4], mn[RPT][NC][4];
-    uint32_t kp[RPT], kc[RPT], kn[RPT];
-    float hc[NC][4] = {}, hn[NC][4] = {};
-    uint32_t hkc = 0, hkn = 0;
-    float xcl[RPT][NC] = {}, xcr[RPT][NC] = {}, xnl[RPT][NC] = {}, xnr[RPT][NC] = {};
-    int kxcl[RPT] = {}, kxcr[RPT] = {}, kxnl[RPT] = {}, kxnr[RPT] = {};
-    float wc[RPT][4];
+    // All-solid 4-voxel chunks never change, so their stores and velocity
+    // loads are skipped; when no pore-solid face carries velocity (a.inert),
+    // their magnetization is never read by a pore either and is not loaded.
+    // The mask is therefore loaded two planes ahead of the magnetization.
+    const bool skip_m = a.inert != 0;
+    f4 mp[RPT][NC], mc[RPT][NC], mn[RPT][NC];
+    uint32_t kp[RPT], kc[RPT], kn[RPT], kq[RPT];
+    f4 hc[NC], hn[NC];
+    uint32_t hkc = 0, hkn = 0, hkq = 0;
+    float xc[RPT][NC], xn[RPT][NC];
+    uint32_t kxc[RPT], kxn[RPT];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        hc[q] = f4{0.f, 0.f, 0.f, 0.f};
+        hn[q] = hc[q];
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        kxc[r] = kxn[r] = 0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) xc[r][q] = xn[r][q] = 0.f;
+    }
+    auto clampz = [&](int kk) { return kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk); };
 
-    auto load_own = [&](int kk, float (&m)[RPT][NC][4], uint32_t (&kw)[RPT]) {
+    auto load_mask = [&](int kk, uint32_t (&kw)[RPT]) {
+        const uint32_t pm = (uint32_t)clampz(kk) * plane_m;
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
+            kw[r] = 0;
             if (rin[r]) {
-                const size_t o = (size_t)kk * plane + (size_t)iy[r] * P + ix;
+                kw[r] = ldw_nt(a.mask, pm + rowm[r]);
+                if (xend) kw[r] = clamp_tail_w(kw[r], xlast); // ghost past nx
+            }
+        }
+    };
+    auto load_m = [&](int kk, const uint32_t (&kw)[RPT], f4 (&m)[RPT][NC]) {
+        const uint32_t pf = (uint32_t)clampz(kk) * plane_b;
 #pragma unroll
-                for (int q = 0; q < NC; q++) load4(m[r][q], a.min[q] + o);
-                kw[r] = *reinterpret_cast<const uint32_t*>(a.mask + o);
+        for (int r = 0; r < RPT; r++) {
+            if (rin[r] && (!skip_m || kw[r] != 0)) {
+#pragma unroll
+                for (int q = 0; q < NC; q++) m[r][q] = ld4_nt(a.min[q], pf + rowf[r]);
+                if (xend) { // elements past nx take the ghost value (registers and LDS agree)
+#pragma unroll
+                    for (int q = 0; q < NC; q++) m[r][q] = clamp_tail(m[r][q], xlast);
+                }
             } else {
 #pragma unroll
-                for (int q = 0; q < NC; q++)
-#pragma unroll
-                    for (int e = 0; e < 4; e++) m[r][q][e] = 0.f;
-                kw[r] = 0;
+                for (int q = 0; q < NC; q++) m[r][q] = f4{0.f, 0.f, 0.f, 0.f};
             }
         }
     };
-    auto load_halo = [&](int kk, float (&h)[NC][4], uint32_t& hk, float (&cl)[RPT][NC],
-                         float (&cr)[RPT][NC], int (&kl)[RPT], int (&kr)[RPT]) {
-        if (hrow) {
-            const size_t o = (size_t)kk * plane + (size_t)hy * P + ix;
+    auto load_hmask = [&](int kk) -> uint32_t {
+        if (!hrow) return 0;
+        uint32_t w = *reinterpret_cast<const uint32_t*>(a.mask + (uint32_t)clampz(kk) * plane_m + hrowm);
+        return xend ? clamp_tail_w(w, xlast) : w;
+    };
+    auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
+        if (hrow && (!skip_m || hk != 0)) {
+            const uint32_t pf = (uint32_t)clampz(kk) * plane_b;
 #pragma unroll
-            for (int q = 0; q < NC; q++) load4(h[q], a.min[q] + o);
-            hk = *reinterpret_cast<const uint32_t*>(a.mask + o);
-        }
+            for (int q = 0; q < NC; q++) h[q] = ld4(a.min[q], pf + hrowf);
+            if (xend) {
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const bool yok = iy[r] < ny;
-            if (hcl && yok) {
-                const size_t o = (size_t)kk * plane + (size_t)iy[r] * P + hl;
-#pragma unroll
-                for (int q = 0; q < NC; q++) cl[r][q] = a.min[q][o];
-                kl[r] = a.mask[o];
+                for (int q = 0; q < NC; q++) h[q] = clamp_tail(h[q], xlast);
             }
-            if (hcr && yok) {
-                const size_t o = (size_t)kk * plane + (size_t)iy[r] * P + hr;
+        } else {
 #pragma unroll
-                for (int q = 0; q < NC; q++) cr[r][q] = a.min[q][o];
-                kr[r] = a.mask[o];
-            }
+            for (int q = 0; q < NC; q++) h[q] = f4{0.f, 0.f, 0.f, 0.f};
         }
     };
-    auto load_vz = [&](int kk, float (&w)[RPT][4]) {
+    auto load_xhalo = [&](int kk, float (&xv)[RPT][NC], uint32_t (&kx)[RPT]) {
+        if (hcl || hcr) {
+            const uint32_t pf = (uint32_t)clampz(kk) * plane_b, pm = (uint32_t)clampz(kk) * plane_m;
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            if (rin[r]) {
-                load4(w[r], a.vz + (size_t)kk * plane + (size_t)iy[r] * P + ix);
-            } else {
+            for (int r = 0; r < RPT; r++) {
+                if (iy[r] < ny) {
+                    const uint32_t of = pf + (uint32_t)iy[r] * Pb + (uint32_t)hcx * 4u;
 #pragma unroll
-                for (int e = 0; e < 4; e++) w[r][e] = 0.f;
+                    for (int q = 0; q < NC; q++) xv[r][q] = ld1(a.min[q], of);
+                    kx[r] = a.mask[pm + (uint32_t)iy[r] * (uint32_t)a.pitch + (uint32_t)hcx];
+                }
             }
         }
     };
 
-    // prologue: plane k0-1 (clamped), plane k0 and its halo, Vz at face k0
-    load_own(k0 > 0 ? k0 - 1 : k0, mp, kp);
-    load_own(k0, mc, kc);
-    load_halo(k0, hc, hkc, xcl, xcr, kxcl, kxcr);
-    if constexpr (VEL) load_vz(k0, wc);
+    // velocity faces of a plane: Vx (faces i..i+4), Vy (rows y, y+1), Vz (k, k+1)
+    auto load_vel = [&](int kk, const uint32_t (&kw)[RPT], f4 (&vx4)[RPT], float (&vxe)[RPT], f4 (&vy4)[RPT],
+                        f4 (&vy4n)[RPT], f4 (&wc)[RPT], f4 (&wn)[RPT]) {
+        const bool inz = kk < nz;
+        const uint32_t px = (uint32_t)kk * planex_b, py = (uint32_t)kk * planey_b, pz = (uint32_t)kk * plane_b;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            if (inz && rin[r] && kw[r] != 0) {
+                vx4[r] = ld4_nt(a.vx, px + rowx[r]);
+                vxe[r] = (xlast >= 3) ? ld1(a.vx, px + rowx[r] + 16u) : 0.f; // face ix+4 <= nx
+                vy4[r] = ld4_nt(a.vy, py + rowf[r]);
+                vy4n[r] = ld4(a.vy, py + rowf[r] + Pb);
+                wc[r] = ld4(a.vz, pz + rowf[r]);
+                wn[r] = ld4(a.vz, pz + plane_b + rowf[r]);
+            } else {
+                vx4[r] = vy4[r] = vy4n[r] = wc[r] = wn[r] = f4{0.f, 0.f, 0.f, 0.f};
+                vxe[r] = 0.f;
+            }
+        }
+    };
+    f4 vx4[RPT], vy4[RPT], vy4n[RPT], wc[RPT], wn[RPT];
+    float vxe[RPT];
+
+    // prologue: masks of planes k0-1, k0, k0+1; magnetization of k0-1, k0
+    load_mask(k0 - 1, kp);
+    load_mask(k0, kc);
+    load_mask(k0 + 1, kn);
+    load_m(k0 - 1, kp, mp);
+    load_m(k0, kc, mc);
+    hkc = load_hmask(k0);
+    hkn = load_hmask(k0 + 1);
+    load_hm(k0, hkc, hc);
+    load_xhalo(k0, xc, kxc);
 
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) acc[q] = 0.0;
 
-    const Coef c = a.c;
-    const bool trim_x[4] = {ix + 0 >= a.bx0 && ix + 0 < a.bx1, ix + 1 >= a.bx0 && ix + 1 < a.bx1,
-                            ix + 2 >= a.bx0 && ix + 2 < a.bx1, ix + 3 >= a.bx0 && ix + 3 < a.bx1};
+    const Coef cf = a.c;
+    bool trim_x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) trim_x[e] = (ix + e >= a.bx0) && (ix + e < a.bx1);
 
     for (int k = k0; k < k1; k++) {
         const int buf = (k - k0) & 1;
-        const int kn_plane = (k + 1 < nz) ? k + 1 : k;
-        // prefetch the next plane (own rows, halo) and the next Vz face
-        load_own(kn_plane, mn, kn);
-        load_halo(kn_plane, hn, hkn, xnl, xnr, kxnl, kxnr);
-        float vxr[RPT][5], vyr[RPT][4], vyr1[RPT][4], wnx[RPT][4];
-        if constexpr (VEL) {
-            load_vz(k + 1, wnx);
-#pragma unroll
-            for (int r = 0; r < RPT; r++) {
-                if (rin[r]) {
-                    const float* px = a.vx + (size_t)k * (size_t)ny * (size_t)a.pitch1 +
-                                      (size_t)iy[r] * (size_t)a.pitch1 + ix;
-                    float t4[4];
-                    load4(t4, px);
-#pragma unroll
-                    for (int e = 0; e < 4; e++) vxr[r][e] = t4[e];
-                    vxr[r][4] = (ix + 4 <= nx) ? px[4] : 0.f;
-                    const float* py = a.vy + (size_t)k * (size_t)(ny + 1) * P + (size_t)iy[r] * P + ix;
-                    load4(vyr[r], py);
-                    load4(vyr1[r], py + P);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        vxr[r][e] = 0.f;
-                        vyr[r][e] = 0.f;
-                        vyr1[r][e] = 0.f;
-                    }
-                    vxr[r][4] = 0.f;
-                }
-            }
-        }
+        // prefetch: masks of plane k+2, magnetization of plane k+1
+        load_mask(k + 2, kq);
+        hkq = load_hmask(k + 2);
+        load_m(k + 1, kn, mn);
+        load_hm(k + 1, hkn, hn);
+        load_xhalo(k + 1, xn, kxn);
+        // velocity faces of plane k (only for chunks holding a pore)
+        if constexpr (VEL) load_vel(k, kc, vx4, vxe, vy4, vy4n, wc, wn);
 
-        // stage plane k into LDS
+        // stage plane k (with ghosts) into LDS
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
+            if (!rin[r]) continue;
             const int lr = 1 + ry + r * RY;
+            f4 v[NC];
+            const uint32_t w = kc[r];
 #pragma unroll
-            for (int q = 0; q < NC; q++) store4(&sm[buf][q][lr][4 + 4 * lx], mc[r][q]);
-            *reinterpret_cast<uint32_t*>(&sk[buf][lr][4 + 4 * lx]) = kc[r];
-            if (hcl) {
+            for (int q = 0; q < NC; q++) v[q] = mc[r][q];
 #pragma unroll
-                for (int q = 0; q < NC; q++) sm[buf][q][lr][3] = xcl[r][q];
-                sk[buf][lr][3] = (uint8_t)kxcl[r];
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][lr][c]) = v[q];
+            *reinterpret_cast<uint32_t*>(&sk[buf][lr][c]) = w;
+            if (xend) { // ghost column nx (inside this chunk or the next column)
+                const int xl = xlast < 3 ? xlast : 3;
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[buf][q][lr][c + 4] = v[q][xl];
+                sk[buf][lr][c + 4] = (uint8_t)((w >> (8 * xl)) & 0xffu);
             }
-            if (hcr) {
+            if (xbeg) { // ghost column -1
 #pragma unroll
-                for (int q = 0; q < NC; q++) sm[buf][q][lr][4 + TX] = xcr[r][q];
-                sk[buf][lr][4 + TX] = (uint8_t)kxcr[r];
+                for (int q = 0; q < NC; q++) sm[buf][q][lr][c - 1] = v[q].x;
+                sk[buf][lr][c - 1] = (uint8_t)(w & 0xffu);
+            }
+            if (hcl || hcr) {
+                const int col = hcl ? c - 1 : c + 4;
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[buf][q][lr][col] = xc[r][q];
+                sk[buf][lr][col] = (uint8_t)kxc[r];
+            }
+            if (iy[r] == 0) { // ghost row -1
+#pragma unroll
+                for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][lr - 1][c]) = v[q];
+                *reinterpret_cast<uint32_t*>(&sk[buf][lr - 1][c]) = w;
+            }
+            if (iy[r] == ny - 1) { // ghost row ny
+#pragma unroll
+                for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][lr + 1][c]) = v[q];
+                *reinterpret_cast<uint32_t*>(&sk[buf][lr + 1][c]) = w;
             }
         }
         if (hrow) {
             const int lr = top ? 0 : TY + 1;
 #pragma unroll
-            for (int q = 0; q < NC; q++) store4(&sm[buf][q][lr][4 + 4 * lx], hc[q]);
-            *reinterpret_cast<uint32_t*>(&sk[buf][lr][4 + 4 * lx]) = hkc;
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][lr][c]) = hc[q];
+            *reinterpret_cast<uint32_t*>(&sk[buf][lr][c]) = hkc;
         }
         __syncthreads();
 
-        const bool zm_in = k > 0, zp_in = k + 1 < nz;
+        // per-plane scalars
         const bool trim_z = k >= a.bz0 && k < a.bz1;
-        float thx = 0.f, thy = 0.f, thz = 0.f;
-        if (NC == 3) {
-            thz = c.bgz * ((float)k + 0.5f);
-        } else if (NC == 2) {
-            thz = c.bgz * ((float)k + 0.5f);
-        }
+        const float thz = cf.bgz * ((float)k + 0.5f);
+        float sn, cm1;
+        plane_rotation<EXACT>(thz, sn, cm1);
 
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             if (!rin[r]) continue;
             const int lr = 1 + ry + r * RY;
             const int y = iy[r];
-            const bool ym_in = y > 0, yp_in = y + 1 < ny;
             const bool trim_y = y >= a.by0 && y < a.by1;
-            if (NC == 3) thy = c.bgy * ((float)y + 0.5f);
-            // neighbours from LDS
-            float ym[NC][4], yp[NC][4], xl[NC], xr[NC];
+            const float thy = cf.bgy * ((float)y + 0.5f);
+            f4 ym[NC], yp[NC];
+            float xl[NC], xr[NC];
 #pragma unroll
             for (int q = 0; q < NC; q++) {
-                const float4 a4 = *reinterpret_cast<const float4*>(&sm[buf][q][lr - 1][4 + 4 * lx]);
-                const float4 b4 = *reinterpret_cast<const float4*>(&sm[buf][q][lr + 1][4 + 4 * lx]);
-                ym[q][0] = a4.x; ym[q][1] = a4.y; ym[q][2] = a4.z; ym[q][3] = a4.w;
-                yp[q][0] = b4.x; yp[q][1] = b4.y; yp[q][2] = b4.z; yp[q][3] = b4.w;
-                xl[q] = sm[buf][q][lr][3 + 4 * lx];
-                xr[q] = sm[buf][q][lr][8 + 4 * lx];
+                ym[q] = *reinterpret_cast<const f4*>(&sm[buf][q][lr - 1][c]);
+                yp[q] = *reinterpret_cast<const f4*>(&sm[buf][q][lr + 1][c]);
+                xl[q] = sm[buf][q][lr][c - 1];
+                xr[q] = sm[buf][q][lr][c + 4];
             }
-            const uint32_t kym = *reinterpret_cast<const uint32_t*>(&sk[buf][lr - 1][4 + 4 * lx]);
-            const uint32_t kyp = *reinterpret_cast<const uint32_t*>(&sk[buf][lr + 1][4 + 4 * lx]);
-            const int kxl = sk[buf][lr][3 + 4 * lx];
-            const int kxr = sk[buf][lr][8 + 4 * lx];
-
-            float out[NC][4];
+            const uint32_t kym = *reinterpret_cast<const uint32_t*>(&sk[buf][lr - 1][c]);
+            const uint32_t kyp = *reinterpret_cast<const uint32_t*>(&sk[buf][lr + 1][c]);
+            const uint32_t kxm = (kc[r] << 8) | (uint32_t)sk[buf][lr][c - 1];
+            const uint32_t kxp = (kc[r] >> 8) | ((uint32_t)sk[buf][lr][c + 4] << 24);
+            f4 out[NC];
+            update_chunk<NC, VEL, EXACT>(cf, mc[r], mp[r], mn[r], ym, yp, xl, xr, kc[r], kxm, kxp, kym, kyp, kp[r],
+                                         kn[r], vx4[r], vxe[r], vy4[r], vy4n[r], wc[r], wn[r], ix, thy, thz, sn,
+                                         cm1, out);
+            if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
+            if (kc[r] != 0) { // all-solid chunks keep their value in both buffers
+                const uint32_t of = (uint32_t)k * plane_b + rowf[r];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int x = ix + e;
-                const bool pore = mbyte(kc[r], e) != 0;
-#pragma unroll
-                for (int q = 0; q < NC; q++) out[q][e] = mc[r][q][e];
-                if (!(x < nx) || !pore) continue;
-                const bool xm_in = x > 0, xp_in = x + 1 < nx;
-                // neighbour masks (out of domain -> the centre's, i.e. pore)
-                const int mxm = xm_in ? (e > 0 ? mbyte(kc[r], e - 1) : kxl) : 1;
-                const int mxp = xp_in ? (e < 3 ? mbyte(kc[r], e + 1) : kxr) : 1;
-                const int mym = ym_in ? mbyte(kym, e) : 1;
-                const int myp = yp_in ? mbyte(kyp, e) : 1;
-                const int mzm = zm_in ? mbyte(kp[r], e) : 1;
-                const int mzp = zp_in ? mbyte(kn[r], e) : 1;
-                const float sx = (float)((mxm == 0) + (mxp == 0));
-                const float sy = (float)((mym == 0) + (myp == 0));
-                const float sz = (float)((mzm == 0) + (mzp == 0));
-                const float rate2 = c.r2 + c.s2x * sx + c.s2y * sy + c.s2z * sz;
-                const float rate1 = c.r1 + c.s1x * sx + c.s1y * sy + c.s1z * sz;
-                const float pxm = (mxm != 0) ? 1.f : 0.f, pxp = (mxp != 0) ? 1.f : 0.f;
-                const float pym = (mym != 0) ? 1.f : 0.f, pyp = (myp != 0) ? 1.f : 0.f;
-                const float pzm = (mzm != 0) ? 1.f : 0.f, pzp = (mzp != 0) ? 1.f : 0.f;
-
-                float um = 0.f, up = 0.f, vm = 0.f, vp = 0.f, wm = 0.f, wp = 0.f;
-                if constexpr (VEL) {
-                    um = vxr[r][e];
-                    up = vxr[r][e + 1];
-                    vm = vyr[r][e];
-                    vp = vyr1[r][e];
-                    wm = wc[r][e];
-                    wp = wnx[r][e];
-                }
-                const float u = 0.5f * (um + up), v = 0.5f * (vm + vp), w = 0.5f * (wm + wp);
-                const float un = fminf(u, 0.f), upos = fmaxf(u, 0.f);
-                const float vn = fminf(v, 0.f), vpos = fmaxf(v, 0.f);
-                const float wn_ = fminf(w, 0.f), wpos = fmaxf(w, 0.f);
-                const float iup = (up != 0.f) ? 1.f : 0.f, ium = (um != 0.f) ? 1.f : 0.f;
-                const float ivp = (vp != 0.f) ? 1.f : 0.f, ivm = (vm != 0.f) ? 1.f : 0.f;
-                const float iwp = (wp != 0.f) ? 1.f : 0.f, iwm = (wm != 0.f) ? 1.f : 0.f;
-
-                float m0[NC], d[NC];
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-                    const float mv = mc[r][q][e];
-                    m0[q] = mv;
-                    const float fxm = xm_in ? (e > 0 ? mc[r][q][e - 1] : xl[q]) : mv;
-                    const float fxp = xp_in ? (e < 3 ? mc[r][q][e + 1] : xr[q]) : mv;
-                    const float fym = pick(ym_in, ym[q][e], mv);
-                    const float fyp = pick(yp_in, yp[q][e], mv);
-                    const float fzm = pick(zm_in, mp[r][q][e], mv);
-                    const float fzp = pick(zp_in, mn[r][q][e], mv);
-                    float dd = c.cdx * (pxp * (fxp - mv) + pxm * (fxm - mv)) +
-                               c.cdy * (pyp * (fyp - mv) + pym * (fym - mv)) +
-                               c.cdz * (pzp * (fzp - mv) + pzm * (fzm - mv));
-                    if constexpr (VEL) {
-                        const float adv = c.hx * (un * (fxp - mv) * iup + upos * (mv - fxm) * ium) +
-                                          c.hy * (vn * (fyp - mv) * ivp + vpos * (mv - fym) * ivm) +
-                                          c.hz * (wn_ * (fzp - mv) * iwp + wpos * (mv - fzm) * iwm);
-                        dd -= adv;
-                    }
-                    d[q] = dd;
-                }
-                // relaxation
-                if constexpr (NC == 3) {
-                    const float par = c.n0 * m0[0] + c.n1 * m0[1] + c.n2 * m0[2];
-                    const float nn[3] = {c.n0, c.n1, c.n2};
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        const float mpar = nn[q] * par;
-                        d[q] += rate1 * (c.peq * nn[q] - mpar) + rate2 * (mpar - m0[q]);
-                    }
-                    thx = c.bgx * ((float)x + 0.5f);
-                    // + dt*g*(m x Bg)
-                    d[0] += -thy * m0[2] + thz * m0[1];
-                    d[1] += thx * m0[2] - thz * m0[0];
-                    d[2] += -thx * m0[1] + thy * m0[0];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < NC; q++) d[q] -= rate2 * m0[q];
-                    (void)rate1;
-                    if constexpr (NC == 2) {
-                        if constexpr (EXACT) {
-                            const float sn = sinf(thz);
-                            const float h = sinf(0.5f * thz);
-                            const float cm1 = -2.f * h * h; // cos(th) - 1 without cancellation
-                            d[0] += m0[0] * cm1 + m0[1] * sn;
-                            d[1] += m0[1] * cm1 - m0[0] * sn;
-                        } else {
-                            d[0] += thz * m0[1];
-                            d[1] -= thz * m0[0];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < NC; q++) out[q][e] = m0[q] + d[q];
-            }
-            // store and accumulate the signal of the updated plane
-            const size_t o = (size_t)k * plane + (size_t)y * P + ix;
-#pragma unroll
-            for (int q = 0; q < NC; q++) store4(a.mout[q] + o, out[q]);
-            if (trim_z && trim_y) {
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-#pragma unroll
-                    for (int e = 0; e < 4; e++)
-                        if (trim_x[e] && mbyte(kc[r], e) != 0) acc[q] += (double)out[q][e];
-                }
+                for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
             }
         }
 
@@ -408,53 +533,318 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
 #pragma unroll
-            for (int q = 0; q < NC; q++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    mp[r][q][e] = mc[r][q][e];
-                    mc[r][q][e] = mn[r][q][e];
-                }
+            for (int q = 0; q < NC; q++) {
+                mp[r][q] = mc[r][q];
+                mc[r][q] = mn[r][q];
+                xc[r][q] = xn[r][q];
+            }
             kp[r] = kc[r];
             kc[r] = kn[r];
-#pragma unroll
-            for (int q = 0; q < NC; q++) {
-                xcl[r][q] = xnl[r][q];
-                xcr[r][q] = xnr[r][q];
-            }
-            kxcl[r] = kxnl[r];
-            kxcr[r] = kxnr[r];
-            if constexpr (VEL) {
-#pragma unroll
-                for (int e = 0; e < 4; e++) wc[r][e] = wnx[r][e];
-            }
+            kn[r] = kq[r];
+            kxc[r] = kxn[r];
         }
 #pragma unroll
-        for (int q = 0; q < NC; q++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) hc[q][e] = hn[q][e];
+        for (int q = 0; q < NC; q++) hc[q] = hn[q];
         hkc = hkn;
+        hkn = hkq;
     }
 
-    // workgroup reduction of the signal: wave shuffle (FP64), then LDS
+    block_reduce<NC, NT>(acc, sred, a.partials);
+}
+
+// Two time steps per pass (temporal blocking).  The workgroup owns TY full-
+// width rows (nx <= 4*LX, one x tile) and a z-chunk [k0, k1).  Each thread
+// owns one "level-1" row y = y0-1+ry of the halo-extended tile and one
+// 4-voxel x-chunk.  Per plane p the pass
+//   1. stages M^n(p) of rows y0-2 .. y0+TY+1 (with ghosts) in LDS A,
+//   2. computes M^{n+1}(p) on rows y0-1 .. y0+TY (level 1, kept in registers
+//      for the z-neighbours and published in LDS B after the level-2 step),
+//   3. computes M^{n+2}(p-1) on rows y0 .. y0+TY-1 (level 2) from LDS B and
+//      the level-1 registers of planes p-2, p-1, p, and writes it to HBM.
+// M^{n+1} never leaves the chip, so one pass moves the 29 B/voxel of a single
+// step for two steps.  Both levels call update_chunk, so the result is bit-
+// identical to two one-step sweeps.  Signals of both steps are reduced.
+template <int NC, bool VEL, bool EXACT, int LX>
+__global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
+{
+    constexpr int NT = 512;
+    constexpr int RY = NT / LX;  // level-1 rows
+    constexpr int TY = RY - 2;   // output rows
+    constexpr int LW = 4 * LX + 8;
+    constexpr int HA = RY + 2;   // LDS A rows: y0-2 .. y0+TY+1 (index y - y0 + 2)
+    constexpr int HB = RY;       // LDS B rows: y0-1 .. y0+TY   (index y - y0 + 1)
+
+    __shared__ __attribute__((aligned(16))) float sA[NC][HA][LW];
+    __shared__ __attribute__((aligned(16))) float sB[NC][HB][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t kA[HA][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t kB[HB][LW];
+    __shared__ double sred[NT / 64][NC];
+
+    const int tid = threadIdx.x;
+    const int lx = tid % LX;
+    const int ry = tid / LX;
+    const int t = xcd_tile(blockIdx.x, a.nblocks);
+    const int ty = t % a.nty;
+    const int tz = t / a.nty;
+    const int y0 = ty * TY;
+    const int k0 = tz * a.lz;
+    const int k1 = min(k0 + a.lz, a.nz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int ix = 4 * lx;
+    const bool xin = ix < nx;
+    const int xlast = nx - 1 - ix;
+    const bool xend = xin && xlast <= 3;
+    const bool xbeg = (ix == 0);
+    const int c = 4 + 4 * lx;
+
+    const int y = y0 - 1 + ry;                     // level-1 row
+    const bool act = xin && y >= 0 && y < ny;      // computes level 1
+    const bool own = act && ry >= 1 && ry <= RY - 2; // level-2 output row
+    const int aR = ry + 1, bR = ry;
+    // extra LDS A rows y0-2 (thread row 0) and y0+TY+1 (thread row RY-1)
+    const int ye = (ry == 0) ? y0 - 2 : y0 + TY + 1;
+    const bool eact = (ry == 0 || ry == RY - 1) && xin && ye >= 0 && ye < ny;
+    const int eR = (ry == 0) ? 0 : HA - 1;
+
+    const uint32_t Pb = (uint32_t)a.pitch * 4u;
+    const uint32_t plane_b = (uint32_t)ny * Pb;
+    const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
+    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
+    const uint32_t planex_b = (uint32_t)ny * P1b;
+    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
+    const int yc = act ? y : 0, yec = eact ? ye : 0;
+    const uint32_t rowf = (uint32_t)yc * Pb + (uint32_t)ix * 4u;
+    const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)ix;
+    const uint32_t rowx = (uint32_t)yc * P1b + (uint32_t)ix * 4u;
+    const uint32_t erowf = (uint32_t)yec * Pb + (uint32_t)ix * 4u;
+    const uint32_t erowm = (uint32_t)yec * (uint32_t)a.pitch + (uint32_t)ix;
+
+    auto clampz = [&](int p) { return p < 0 ? 0 : (p >= nz ? nz - 1 : p); };
+    auto load_row = [&](bool on, int p, uint32_t rf, uint32_t rm, f4 (&m)[NC], uint32_t& kw) {
+        if (on) {
+            const uint32_t pf = (uint32_t)clampz(p) * plane_b, pm = (uint32_t)clampz(p) * plane_m;
 #pragma unroll
-    for (int q = 0; q < NC; q++) {
-        double v = acc[q];
+            for (int q = 0; q < NC; q++) m[q] = ld4_nt(a.min[q], pf + rf);
+            kw = ldw_nt(a.mask, pm + rm);
+            if (xend) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        acc[q] = v;
+                for (int q = 0; q < NC; q++) m[q] = clamp_tail(m[q], xlast);
+                kw = clamp_tail_w(kw, xlast);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) m[q] = f4{0.f, 0.f, 0.f, 0.f};
+            kw = 0;
+        }
+    };
+    // write a chunk (and its x ghosts, and the y ghost row when on the
+    // domain edge) into an LDS plane
+    auto stage = [&](float (*S)[LW], uint8_t (*K)[LW], int hrows, int row, const f4* v, uint32_t w, int yy,
+                     int sq) {
+        (void)sq;
+#pragma unroll
+        for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&S[q * hrows + row][c]) = v[q];
+        *reinterpret_cast<uint32_t*>(&K[row][c]) = w;
+        if (xend) {
+            const int xl = xlast < 3 ? xlast : 3;
+#pragma unroll
+            for (int q = 0; q < NC; q++) S[q * hrows + row][c + 4] = v[q][xl];
+            K[row][c + 4] = (uint8_t)((w >> (8 * xl)) & 0xffu);
+        }
+        if (xbeg) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) S[q * hrows + row][c - 1] = v[q].x;
+            K[row][c - 1] = (uint8_t)(w & 0xffu);
+        }
+        if (yy == 0 && row - 1 >= 0) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&S[q * hrows + row - 1][c]) = v[q];
+            *reinterpret_cast<uint32_t*>(&K[row - 1][c]) = w;
+        }
+        if (yy == ny - 1 && row + 1 < hrows) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&S[q * hrows + row + 1][c]) = v[q];
+            *reinterpret_cast<uint32_t*>(&K[row + 1][c]) = w;
+        }
+    };
+    // neighbours of this thread's chunk in an LDS plane
+    auto gather = [&](float (*S)[LW], uint8_t (*K)[LW], int hrows, int row, uint32_t kc, f4 (&ym)[NC],
+                      f4 (&yp)[NC], float (&xl)[NC], float (&xr)[NC], uint32_t& kym, uint32_t& kyp, uint32_t& kxm,
+                      uint32_t& kxp) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            ym[q] = *reinterpret_cast<const f4*>(&S[q * hrows + row - 1][c]);
+            yp[q] = *reinterpret_cast<const f4*>(&S[q * hrows + row + 1][c]);
+            xl[q] = S[q * hrows + row][c - 1];
+            xr[q] = S[q * hrows + row][c + 4];
+        }
+        kym = *reinterpret_cast<const uint32_t*>(&K[row - 1][c]);
+        kyp = *reinterpret_cast<const uint32_t*>(&K[row + 1][c]);
+        kxm = (kc << 8) | (uint32_t)K[row][c - 1];
+        kxp = (kc >> 8) | ((uint32_t)K[row][c + 4] << 24);
+    };
+
+    float (*SA)[LW] = &sA[0][0];
+    float (*SB)[LW] = &sB[0][0];
+
+    // register pipeline: M^n planes p-1, p, p+1 (n0, n1, n2) and prefetch p+2
+    f4 n0[NC], n1[NC], n2[NC], n3[NC], e1[NC], e2[NC];
+    uint32_t kn0, kn1, kn2, kn3, ke1, ke2, kq = 0;
+    // level-1 results of planes p-2, p-1 and p (r0, r1, rn)
+    f4 r0[NC], r1[NC], rn[NC];
+    // velocities of the current level-1 plane and of the previous one
+    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, vz0 = vx4, vz1 = vx4;
+    f4 pvx4 = vx4, pvy4 = vx4, pvy4n = vx4, pvz0 = vx4, pvz1 = vx4;
+    float vxe = 0.f, pvxe = 0.f;
+#pragma unroll
+    for (int q = 0; q < NC; q++) r0[q] = r1[q] = rn[q] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int p0 = k0 - 1;
+    load_row(act, p0 - 1, rowf, rowm, n0, kn0);
+    load_row(act, p0, rowf, rowm, n1, kn1);
+    load_row(act, p0 + 1, rowf, rowm, n2, kn2);
+    load_row(eact, p0, erowf, erowm, e1, ke1);
+    if constexpr (VEL) { // Vz face of the first level-1 plane (face 0 when the chunk starts at z = 0)
+        if (act) vz1 = ld4_nt(a.vz, (uint32_t)(p0 > 0 ? p0 : 0) * plane_b + rowf);
     }
-    const int wave = tid >> 6, lane = tid & 63;
-    if (lane == 0) {
+
+    double acc1[NC], acc2[NC];
 #pragma unroll
-        for (int q = 0; q < NC; q++) sred[wave][q] = acc[q];
+    for (int q = 0; q < NC; q++) acc1[q] = acc2[q] = 0.0;
+    bool trim_x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) trim_x[e] = (ix + e >= a.bx0) && (ix + e < a.bx1);
+    const bool trim_y = y >= a.by0 && y < a.by1;
+    const Coef c1 = a.c1, c2 = a.c2;
+    const float thy1 = c1.bgy * ((float)y + 0.5f), thy2 = c2.bgy * ((float)y + 0.5f);
+
+    for (int p = p0; p <= k1; p++) {
+        const bool l1 = p >= 0 && p < nz;
+        const bool l2 = p - 1 >= k0;
+        // prefetch M^n plane p+2 (clamped) and the extra row of plane p+1
+        load_row(act, p + 2, rowf, rowm, n3, kn3);
+        load_row(eact, p + 1, erowf, erowm, e2, ke2);
+        if constexpr (VEL) {
+            vz0 = vz1;
+            if (act && l1) {
+                const uint32_t px = (uint32_t)p * planex_b, py = (uint32_t)p * planey_b;
+                vx4 = ld4(a.vx, px + rowx);
+                vxe = (xlast >= 3) ? ld1(a.vx, px + rowx + 16u) : 0.f;
+                vy4 = ld4(a.vy, py + rowf);
+                vy4n = ld4(a.vy, py + rowf + Pb);
+                vz1 = ld4_nt(a.vz, (uint32_t)(p + 1) * plane_b + rowf);
+            }
+        }
+        // 1. stage M^n(p)
+        if (l1) {
+            if (act) stage(SA, kA, HA, aR, n1, kn1, y, 0);
+            if (eact) stage(SA, kA, HA, eR, e1, ke1, ye, 0);
+        }
+        __syncthreads();
+        // 2. level 1 at plane p (step n -> n+1)
+        if (l1 && act) {
+            f4 ym[NC], yp[NC];
+            float xl[NC], xr[NC];
+            uint32_t kym, kyp, kxm, kxp;
+            gather(SA, kA, HA, aR, kn1, ym, yp, xl, xr, kym, kyp, kxm, kxp);
+            const float thz = c1.bgz * ((float)p + 0.5f);
+            float sn, cm1;
+            plane_rotation<EXACT>(thz, sn, cm1);
+            update_chunk<NC, VEL, EXACT>(c1, n1, n0, n2, ym, yp, xl, xr, kn1, kxm, kxp, kym, kyp, kn0, kn2, vx4, vxe,
+                                         vy4, vy4n, vz0, vz1, ix, thy1, thz, sn, cm1, rn);
+            if (xend) {
+#pragma unroll
+                for (int q = 0; q < NC; q++) rn[q] = clamp_tail(rn[q], xlast);
+            }
+            if (own && trim_y && p >= k0 && p < k1 && p >= a.bz0 && p < a.bz1) accumulate<NC>(acc1, rn, kn1, trim_x);
+        }
+        // 3. level 2 at plane p-1 (step n+1 -> n+2)
+        if (l2 && own) {
+            const int pm = p - 1;
+            f4 ym[NC], yp[NC], zm[NC], zp[NC];
+            float xl[NC], xr[NC];
+            uint32_t kym, kyp, kxm, kxp;
+            gather(SB, kB, HB, bR, kn0, ym, yp, xl, xr, kym, kyp, kxm, kxp);
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                zm[q] = (pm == 0) ? r1[q] : r0[q];
+                zp[q] = (p >= nz) ? r1[q] : rn[q];
+            }
+            const float thz = c2.bgz * ((float)pm + 0.5f);
+            float sn, cm1;
+            plane_rotation<EXACT>(thz, sn, cm1);
+            f4 out[NC];
+            update_chunk<NC, VEL, EXACT>(c2, r1, zm, zp, ym, yp, xl, xr, kn0, kxm, kxp, kym, kyp, kq, kn1, pvx4,
+                                         pvxe, pvy4, pvy4n, pvz0, pvz1, ix, thy2, thz, sn, cm1, out);
+            const uint32_t of = (uint32_t)pm * plane_b + rowf;
+#pragma unroll
+            for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
+            if (trim_y && pm >= a.bz0 && pm < a.bz1) accumulate<NC>(acc2, out, kn0, trim_x);
+        }
+        __syncthreads();
+        // publish M^{n+1}(p) for the level-2 step of the next plane
+        if (l1 && act) stage(SB, kB, HB, bR, rn, kn1, y, 1);
+        // rotate
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            n0[q] = n1[q];
+            n1[q] = n2[q];
+            n2[q] = n3[q];
+            e1[q] = e2[q];
+            r0[q] = r1[q];
+            r1[q] = rn[q];
+        }
+        kq = kn0;
+        kn0 = kn1;
+        kn1 = kn2;
+        kn2 = kn3;
+        ke1 = ke2;
+        if constexpr (VEL) {
+            pvx4 = vx4;
+            pvxe = vxe;
+            pvy4 = vy4;
+            pvy4n = vy4n;
+            pvz0 = vz0;
+            pvz1 = vz1;
+        }
     }
+    block_reduce<NC, NT>(acc1, sred, a.partials1);
     __syncthreads();
-    if (tid < NC) {
-        double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < NT / 64; w++) s += sred[w][tid];
-        a.partials[(size_t)blockIdx.x * NC + tid] = s;
+    block_reduce<NC, NT>(acc2, sred, a.partials2);
+}
+
+// Sets *flag when some face between two in-domain cells carries velocity and
+// touches a solid cell (then solid magnetization can reach a pore through the
+// upwind terms and the sweeps must load it).  One thread per cell checks the
+// faces on its minus side; racing writers all store 1.
+__global__ void __launch_bounds__(256) inert_check_kernel(InertArgs a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const size_t P = a.pitch, P1 = a.pitch1;
+    const size_t c = ((size_t)z * a.ny + y) * P + x;
+    const bool pc = a.mask[c] != 0;
+    bool bad = false;
+    if (x > 0) {
+        const float v = a.vx[((size_t)z * a.ny + y) * P1 + x];
+        bad |= v != 0.f && !(pc && a.mask[c - 1] != 0);
     }
+    if (y > 0) {
+        const float v = a.vy[((size_t)z * (a.ny + 1) + y) * P + x];
+        bad |= v != 0.f && !(pc && a.mask[c - P] != 0);
+    }
+    if (z > 0) {
+        const float v = a.vz[((size_t)z * a.ny + y) * P + x];
+        bad |= v != 0.f && !(pc && a.mask[c - (size_t)a.ny * P] != 0);
+    }
+    if (bad) *a.flag = 1;
+}
+
+hipError_t launch_inert_check(const InertArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(inert_check_kernel, dim3((a.nx + 255) / 256, a.ny, a.nz), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 // Sum of the per-workgroup partials of nsteps steps: out[s*NC + q], fixed order.
@@ -488,17 +878,15 @@ __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) acc[q] = 0.0;
-    const bool kin = k >= a.bz0 && k < a.bz1;
-    if (kin) {
+    if (k >= a.bz0 && k < a.bz1) {
         const size_t base = (size_t)k * (size_t)a.ny * (size_t)a.pitch;
-        const int n = a.nx * a.ny;
-        for (int t = threadIdx.x; t < n; t += 256) {
-            const int y = t / a.nx, x = t - y * a.nx;
-            if (y < a.by0 || y >= a.by1 || x < a.bx0 || x >= a.bx1) continue;
-            const size_t o = base + (size_t)y * a.pitch + x;
-            if (a.mask[o] == 0) continue;
+        for (int y = a.by0; y < a.by1; y++) {
+            const size_t row = base + (size_t)y * a.pitch;
+            for (int x = a.bx0 + (int)threadIdx.x; x < a.bx1; x += 256) {
+                if (a.mask[row + x] == 0) continue;
 #pragma unroll
-            for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][o];
+                for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
+            }
         }
     }
     for (int q = 0; q < NC; q++) {
@@ -529,16 +917,28 @@ static const VariantShape kVariants[] = {
     {8, 1},  // TX 32,  TY 32
 };
 static const int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+// two-step sweeps: lanes per row (tile width 4*lx covers the whole x extent)
+static const int kVariants2[] = {64, 32, 16};
+static const int kNumVariants2 = (int)(sizeof(kVariants2) / sizeof(kVariants2[0]));
 
-int num_variants() { return kNumVariants; }
+int num_variants() { return kNumVariants + kNumVariants2; }
 
-bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty)
+bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps)
 {
-    if (v < 0 || v >= kNumVariants) return false;
-    *lx = kVariants[v].lx;
-    *rpt = kVariants[v].rpt;
-    *tx = 4 * kVariants[v].lx;
-    *ty = (256 / kVariants[v].lx) * kVariants[v].rpt;
+    if (v < 0 || v >= num_variants()) return false;
+    if (v < kNumVariants) {
+        *lx = kVariants[v].lx;
+        *rpt = kVariants[v].rpt;
+        *tx = 4 * kVariants[v].lx;
+        *ty = (256 / kVariants[v].lx) * kVariants[v].rpt;
+        *steps = 1;
+    } else {
+        *lx = kVariants2[v - kNumVariants];
+        *rpt = 1;
+        *tx = 4 * *lx;
+        *ty = 512 / *lx - 2;
+        *steps = 2;
+    }
     return true;
 }
 
@@ -567,6 +967,30 @@ hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepAr
         return vel ? launch_nc<2, true, false>(variant, a, s) : launch_nc<2, false, false>(variant, a, s);
     }
     if (nc == 3) return vel ? launch_nc<3, true, false>(variant, a, s) : launch_nc<3, false, false>(variant, a, s);
+    return hipErrorInvalidValue;
+}
+
+template <int NC, bool VEL, bool EXACT>
+static hipError_t launch2_nc(int lx, const Sweep2Args& a, hipStream_t s)
+{
+    const dim3 grid(a.nblocks), block(512);
+    switch (lx) {
+    case 64: hipLaunchKernelGGL((sweep2_kernel<NC, VEL, EXACT, 64>), grid, block, 0, s, a); break;
+    case 32: hipLaunchKernelGGL((sweep2_kernel<NC, VEL, EXACT, 32>), grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL((sweep2_kernel<NC, VEL, EXACT, 16>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s)
+{
+    if (nc == 1) return vel ? launch2_nc<1, true, false>(lx, a, s) : launch2_nc<1, false, false>(lx, a, s);
+    if (nc == 2) {
+        if (exact) return vel ? launch2_nc<2, true, true>(lx, a, s) : launch2_nc<2, false, true>(lx, a, s);
+        return vel ? launch2_nc<2, true, false>(lx, a, s) : launch2_nc<2, false, false>(lx, a, s);
+    }
+    if (nc == 3) return vel ? launch2_nc<3, true, false>(lx, a, s) : launch2_nc<3, false, false>(lx, a, s);
     return hipErrorInvalidValue;
 }
 

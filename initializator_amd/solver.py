@@ -147,12 +147,12 @@ class HipSolver:
 
 
 def variants():
-    """[(index, lanes_x, rows_per_thread, tile_x, tile_y)] of the sweep kernel."""
+    """[(index, lanes_x, rows_per_thread, tile_x, tile_y, steps_per_pass)] of
+    the sweep kernels."""
     lib = _lib.load()
     out = []
     for v in range(lib.nmr_hip_num_variants()):
-        a, b, c, d = (ctypes.c_int32() for _ in range(4))
-        check(lib.nmr_hip_variant_info(v, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)),
-              "nmr_hip_variant_info")
-        out.append((v, a.value, b.value, c.value, d.value))
+        f = [ctypes.c_int32() for _ in range(5)]
+        check(lib.nmr_hip_variant_info(v, *[ctypes.byref(x) for x in f]), "nmr_hip_variant_info")
+        out.append((v,) + tuple(x.value for x in f))
     return out

@@ -111,6 +111,12 @@ def test_strerror_and_variants(lib):
     n = lib.nmr_hip_num_variants()
     assert n >= 4
     a = ctypes.c_int32()
-    assert lib.nmr_hip_variant_info(n, None, None, None, None) == _lib.NMR_ERR_INVALID
-    assert lib.nmr_hip_variant_info(0, ctypes.byref(a), None, None, None) == _lib.NMR_OK
+    assert lib.nmr_hip_variant_info(n, None, None, None, None, None) == _lib.NMR_ERR_INVALID
+    assert lib.nmr_hip_variant_info(0, ctypes.byref(a), None, None, None, None) == _lib.NMR_OK
     assert a.value in (8, 16, 32, 64)
+    steps = []
+    for v in range(n):
+        st = ctypes.c_int32()
+        assert lib.nmr_hip_variant_info(v, None, None, None, None, ctypes.byref(st)) == _lib.NMR_OK
+        steps.append(st.value)
+    assert set(steps) == {1, 2}

@@ -95,13 +95,41 @@ def test_general_three_components():
     assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
 
 
+def test_velocity_on_solid_faces_against_oracle():
+    """Velocity on every face, pore-solid faces included: solid magnetization
+    (here nonzero) is advected into pores by the upwind terms, so the sweeps
+    must load it; every tile variant must still match the oracle."""
+    shape = (14, 19, 23)
+    rng = np.random.default_rng(12)
+    mask = (rng.random(shape) < 0.6).astype(np.uint8)
+    vx = rng.uniform(-1e-3, 1e-3, (14, 19, 24))
+    vy = rng.uniform(-1e-3, 1e-3, (14, 20, 23))
+    vz = rng.uniform(-1e-3, 1e-3, (15, 19, 23))
+    case = generators.Case(shape, mask, vx, vy, vz, 1e-6, generators.stable_dt(1e-6, 2.3e-9), 2.3e-9, 0.5,
+                           2.5e10, 1e-5, 60, None, 2)
+    m0 = [rng.random(shape), rng.random(shape) - 0.5, np.zeros(shape)]  # solids nonzero too
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    for v, lx, rpt, tx, ty, steps in variants():
+        if steps == 2 and shape[2] > tx:
+            continue
+        s = HipSolver.from_case(case)
+        s.set_tile(v, 5)
+        s.upload_magnetization(*m0[:2])
+        sig = s.run(case.nsteps)
+        f = s.download_magnetization()
+        s.close()
+        e_sig, e_vox = _report(f"random velocity variant {v}", sig, sig_ref[:, :2], f, f_ref[:2])
+        assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (v, e_sig, e_vox)
+
+
 def test_tile_shapes_bitwise_identical():
-    """Every tile variant and z-chunk gives bit-identical fields (the tiling
-    does not change any voxel's arithmetic) and signals equal to 1e-12."""
-    case = _flow_case((70, 29, 33), 40, seed=9)
+    """Every one-step tile variant, every two-step (temporally blocked) variant
+    and every z-chunk gives bit-identical fields -- a two-step pass equals two
+    one-step sweeps -- and signals equal to 1e-12."""
+    case = _flow_case((70, 29, 33), 41, seed=9)
     m0 = [case.mask.astype(np.float64), 0.5 * case.mask, np.zeros(case.shape)]
     base_sig, base_f = None, None
-    for v, lx, rpt, tx, ty in variants():
+    for v, lx, rpt, tx, ty, steps in variants():
         for lz in (1, 7, 70):
             s = HipSolver.from_case(case)
             s.set_tile(v, lz)
@@ -113,8 +141,32 @@ def test_tile_shapes_bitwise_identical():
                 base_sig, base_f = sig, f
                 continue
             for a, b in zip(f, base_f):
-                assert np.array_equal(a, b), (v, lz)
+                assert np.array_equal(a, b), (v, lz, steps)
             np.testing.assert_allclose(sig, base_sig, rtol=1e-12, atol=1e-12)
+
+
+def test_two_step_sweep_on_domain_edges():
+    """Two-step pass on shapes that put tile and chunk edges on the domain
+    boundary (ny a multiple of the tile height, nz = 1, 2, nx = 1) matches the
+    one-step sweep bit for bit."""
+    for shape in [(1, 12, 20), (2, 6, 64), (9, 14, 1), (5, 31, 128), (17, 30, 57)]:
+        case = _flow_case(shape, 6, seed=4, phi=0.5)
+        m0 = [case.mask.astype(np.float64), 0.25 * case.mask]
+        res = []
+        for v, lx, rpt, tx, ty, steps in variants():
+            if steps == 2 and shape[2] > tx:
+                continue
+            for lz in (1, 3):
+                s = HipSolver.from_case(case)
+                s.set_tile(v, lz)
+                s.upload_magnetization(*m0)
+                sig = s.run(case.nsteps, case.grad_sched)
+                res.append((v, lz, sig, s.download_magnetization()))
+                s.close()
+        for v, lz, sig, f in res[1:]:
+            for a, b in zip(f, res[0][3]):
+                assert np.array_equal(a, b), (shape, v, lz)
+            np.testing.assert_allclose(sig, res[0][2], rtol=1e-12, atol=1e-12)
 
 
 def test_deterministic_repeat():
