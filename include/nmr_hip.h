@@ -116,6 +116,21 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
  * after n updates. */
 int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal);
 
+/* Instantaneous pulse on every pore voxel, AddImpulse semantics
+ * (src/NmrRelaxationSolve.cpp:2552-2608, rotation OrtTransform cpp:2536-2550):
+ *   flag 1: gradient pulse, angle = param*g*((i-1)dx*ax + (j-1)dy*ay + (k-1)dz*az)
+ *           about the unit vector of (ax, ay, az);
+ *   flag 2: RF pulse, angle = param about (ax, ay, az) as given;
+ *   flag 3: multi-gradient pulse (angle g*|a|*|(r - 1e13 d).a_hat|);
+ *   flag 4: radial phase -g*|(r - r_centre) * a| about (ax, ay, az).
+ * Replaces the host-side AddImpulse and its 6 PCIe copies in Sequence_gpu
+ * (cpp:2681-3146).  ncomp 3, or ncomp 2 with a rotation axis along z. */
+int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay, double az);
+
+/* Device selection before nmr_hip_create (main.cpp:32 picks (rank+shift)%NGPU). */
+int nmr_hip_device_count(int32_t* count);
+int nmr_hip_set_device(int32_t device);
+
 /* Pore sums of the current state: out[ncomp]. */
 int nmr_hip_signal(nmr_ctx* ctx, double* out);
 

@@ -44,6 +44,9 @@ EXPORTS = (
     "nmr_hip_download_magnetization",
     "nmr_hip_run",
     "nmr_hip_signal",
+    "nmr_hip_pulse",
+    "nmr_hip_device_count",
+    "nmr_hip_set_device",
     "nmr_hip_last_run_ms",
     "nmr_hip_set_tile",
     "nmr_hip_num_variants",
@@ -124,6 +127,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_download_magnetization": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_run": (c_int, [vp, c_int64, dp, dp]),
         "nmr_hip_signal": (c_int, [vp, dp]),
+        "nmr_hip_pulse": (c_int, [vp, c_int32, c_double, c_double, c_double, c_double]),
+        "nmr_hip_device_count": (c_int, [POINTER(c_int32)]),
+        "nmr_hip_set_device": (c_int, [c_int32]),
         "nmr_hip_last_run_ms": (c_int, [vp, dp, POINTER(c_int64)]),
         "nmr_hip_set_tile": (c_int, [vp, c_int32, c_int32]),
         "nmr_hip_num_variants": (c_int, []),

@@ -592,6 +592,54 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out)
     return NMR_OK;
 }
 
+int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay, double az)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_pulse: null context");
+    if (flag < 1 || flag > 4) return fail(NMR_ERR_INVALID, "pulse flag %d", (int)flag);
+    if (!std::isfinite(param) || !std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az))
+        return fail(NMR_ERR_INVALID, "pulse parameters must be finite");
+    if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "pulse before mask and magnetization upload");
+    if (ctx->nc == 1) return fail(NMR_ERR_UNSUPPORTED, "pulses need ncomp >= 2");
+    if (ctx->nc == 2 && (ax != 0.0 || ay != 0.0))
+        return fail(NMR_ERR_UNSUPPORTED, "ncomp 2 takes rotations about z only");
+    const nmr_problem& p = ctx->prob;
+    nmr::PulseArgs a{};
+    for (int q = 0; q < 3; q++) a.m[q] = ctx->m[ctx->cur][q];
+    a.mask = ctx->mask;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.pitch = ctx->pitch;
+    a.nc = ctx->nc;
+    a.flag = flag;
+    a.param = param;
+    a.ax = ax;
+    a.ay = ay;
+    a.az = az;
+    a.dx = p.dx;
+    a.dy = p.dy;
+    a.dz = p.dz;
+    a.gamma = p.gamma;
+    HIP_TRY(nmr::launch_pulse(a, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return NMR_OK;
+}
+
+int nmr_hip_device_count(int32_t* count)
+{
+    if (!count) return fail(NMR_ERR_NULL, "nmr_hip_device_count: null argument");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    *count = n;
+    return NMR_OK;
+}
+
+int nmr_hip_set_device(int32_t device)
+{
+    HIP_TRY(hipSetDevice(device));
+    return NMR_OK;
+}
+
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps)
 {
     if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_last_run_ms: null context");

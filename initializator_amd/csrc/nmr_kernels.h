@@ -62,6 +62,15 @@ struct Sweep2Args {
     Coef c1, c2;            // coefficients of the two steps
 };
 
+struct PulseArgs {
+    float* m[3];
+    const uint8_t* mask;
+    int nx, ny, nz, pitch, nc;
+    int flag;
+    double param, ax, ay, az;
+    double dx, dy, dz, gamma;
+};
+
 struct FieldSumArgs {
     const float* f[3];
     const uint8_t* mask;
@@ -77,6 +86,7 @@ hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepAr
 hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
+hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s);
 
 } // namespace nmr

@@ -847,6 +847,58 @@ hipError_t launch_inert_check(const InertArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Instantaneous RF / gradient pulse: every pore voxel's M is rotated by an
+// angle about an axis, as AddImpulse + OrtTransform do it
+// (src/NmrRelaxationSolve.cpp:2536-2608).  Angles and the rotation are
+// evaluated in FP64 in the reference's order of operations; the state stays
+// FP32.  Reference index i maps to x + 1 here (no ghost layer: ghost cells
+// are refilled from the interior after every pulse in the reference anyway).
+__global__ void __launch_bounds__(256) pulse_kernel(PulseArgs a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const size_t o = ((size_t)z * a.ny + y) * a.pitch + x;
+    if (a.mask[o] == 0) return;
+    double len = sqrt(a.ax * a.ax + a.ay * a.ay + a.az * a.az);
+    if (len == 0.0) len = 1.0;
+    const double n0x = a.ax / len, n0y = a.ay / len, n0z = a.az / len;
+    const double i = (double)(x + 1), j = (double)(y + 1), k = (double)(z + 1);
+    double angle, rx, ry, rz;
+    if (a.flag == 1) { // narrow gradient pulse, axis = unit gradient direction
+        angle = a.param * a.gamma * ((i - 1) * a.dx * a.ax + (j - 1) * a.dy * a.ay + (k - 1) * a.dz * a.az);
+        rx = n0x; ry = n0y; rz = n0z;
+    } else if (a.flag == 2) { // RF pulse of angle param about the (unnormalised) axis
+        angle = a.param;
+        rx = a.ax; ry = a.ay; rz = a.az;
+    } else if (a.flag == 3) { // multi-gradient pulse about a far-away origin
+        const double distance = 10e+12;
+        const double eff = sqrt(a.ax * a.ax + a.ay * a.ay + a.az * a.az);
+        const double t = (i - 1.0 - distance) * a.dx * n0x + (j - 1.0 - distance) * a.dy * n0y +
+                         (k - 1.0 - distance) * a.dz * n0z;
+        angle = a.gamma * eff * sqrt(t * t);
+        rx = n0x; ry = n0y; rz = n0z;
+    } else { // flag 4: radial phase about the domain centre
+        const double tx = (i - 0.5) * a.dx * a.ax - 0.5 * a.nx * a.dx * a.ax;
+        const double ty = (j - 0.5) * a.dy * a.ay - 0.5 * a.ny * a.dy * a.ay;
+        const double tz = (k - 0.5) * a.dz * a.az - 0.5 * a.nz * a.dz * a.az;
+        angle = 0.0 - a.gamma * sqrt(tx * tx + ty * ty + tz * tz);
+        rx = a.ax; ry = a.ay; rz = a.az;
+    }
+    const double c = cos(angle), s = sin(angle);
+    const double vx = a.m[0][o], vy = a.m[1][o], vz = a.nc == 3 ? (double)a.m[2][o] : 0.0;
+    const double scal = rx * vx + ry * vy + rz * vz;
+    a.m[0][o] = (float)(vx * c + rx * (1.0 - c) * scal + s * (ry * vz - rz * vy));
+    a.m[1][o] = (float)(vy * c + ry * (1.0 - c) * scal + s * (rz * vx - rx * vz));
+    if (a.nc == 3) a.m[2][o] = (float)(vz * c + rz * (1.0 - c) * scal + s * (rx * vy - ry * vx));
+}
+
+hipError_t launch_pulse(const PulseArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(pulse_kernel, dim3((a.nx + 255) / 256, a.ny, a.nz), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 // Sum of the per-workgroup partials of nsteps steps: out[s*NC + q], fixed order.
 __global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ part,
                                                               int nwg, int nc, double* __restrict__ out)

@@ -129,6 +129,11 @@ class HipSolver:
         check(self.lib.nmr_hip_run(self.ctx, int(nsteps), _dptr(g), _dptr(sig)), "nmr_hip_run")
         return sig
 
+    def pulse(self, flag: int, param: float, axis) -> None:
+        """Rotate every pore voxel (AddImpulse flags 1-4)."""
+        check(self.lib.nmr_hip_pulse(self.ctx, int(flag), float(param), float(axis[0]), float(axis[1]),
+                                     float(axis[2])), "nmr_hip_pulse")
+
     def signal(self) -> np.ndarray:
         out = np.empty(self.ncomp)
         check(self.lib.nmr_hip_signal(self.ctx, _dptr(out)), "nmr_hip_signal")

@@ -66,6 +66,7 @@ def lib():
         L.or_explicit_simple.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_output.restype = c_double
         L.or_output.argtypes = [P, cp, c_int, c_int, c_int, dp, POINTER(c_int32)]
+        L.or_add_impulse.argtypes = [P, cp, c_int, c_double, c_double, c_double, c_double, dp, dp, dp]
         L.or_run.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, c_int64, dp, dp, c_int, c_int, c_int,
                              dp, dp, dp, dp]
         _lib = L
@@ -143,6 +144,12 @@ def explicit_simple(p, act, D, t1, t2, vel, nf, m, a):
                              *[_d(x) for x in nf], *[_d(x) for x in m], *[_d(x) for x in a])
 
 
+def add_impulse(p, act, flag, param, axis, m):
+    """AddImpulse in place on m (list of 3 padded arrays)."""
+    lib().or_add_impulse(ctypes.byref(p), _c(act), int(flag), float(param), float(axis[0]), float(axis[1]),
+                         float(axis[2]), *[_d(x) for x in m])
+
+
 def output(p, act, buf, f):
     n = c_int32()
     s = lib().or_output(ctypes.byref(p), _c(act), buf[0], buf[1], buf[2], _d(f), ctypes.byref(n))
@@ -190,3 +197,150 @@ def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0)):
     gs = None if case.grad_sched is None else case.grad_sched[:n]
     sig = run(p, act, D, t1, t2, vel, n, m, grad_sched=gs, buf=buf)
     return sig, [layout.unpad(x) for x in m]
+
+
+# ---------------------------------------------------------------------------
+# The reference's driver loops, restated for checking the drop-in executable.
+
+PI = 3.1415926535897932384626433832795028841971
+SMALL = 1e-50
+
+
+def sequence_events(gpu, nseq, t, dt, T1, T2, d, gp, mg, flags):
+    """Pulses (flag, param, axis) at time t: Sequence_gpu (cpp:2681-3146) when
+    gpu, else the CPU Sequence (cpp:2610-2679).  flags: 5-element list."""
+    ev = []
+
+    def win(t0):
+        return (t < t0 + dt) and (t >= t0)
+
+    ninety = (2, 0.5 * PI, (d[2], d[0], d[1]))
+    if nseq == 0:
+        if win(0.0):
+            ev.append(ninety)
+    elif nseq == 1:
+        if win(0.0):
+            ev.append(ninety)
+        for i in range(20 if gpu else 100):
+            if win((2 * i + 1) * T1):
+                ev.append((2, PI, (d[2], d[0], d[1])))
+    elif nseq == 2:
+        neg = (2, 0.5 * PI, (-d[2], d[0], d[1])) if gpu else ninety
+        gpl = (1, gp, (d[0], d[1], d[2]))
+        for n, (t0, e) in enumerate([(0.0, neg), (T1, gpl), (2.0 * T1, ninety), (2.0 * T1 + T2, neg),
+                                     (3.0 * T1 + T2, gpl)]):
+            if win(t0) and flags[n] == 0:
+                ev.append(e)
+                flags[n] = 1
+    elif gpu and nseq == 9:
+        gpl = (1, gp, (d[0], d[1], d[2]))
+        for n, (t0, e) in enumerate([(0.0, ninety), (T1, gpl), (T1 + T2 / 2, (2, PI, (d[2], d[0], d[1]))),
+                                     (T2 + T1, gpl)]):
+            if win(t0) and flags[n] == 0:
+                ev.append(e)
+                flags[n] = 1
+    elif gpu and nseq == 5:
+        if win(0.0):
+            ev.append((2, PI, (d[2], d[0], d[1])))
+    elif gpu and nseq in (98, 99):
+        seq = [(0.0, (2, 0.5 * PI, (0, -1, 0)) if nseq == 98 else None), (T1, (3, 1, tuple(mg))),
+               (T1 + T2 / 2, (2, PI, (0, 0, 1)) if nseq == 98 else None),
+               (T2 + T1, (3, 1, tuple(mg) if nseq == 98 else tuple(-x for x in mg)))]
+        for n, (t0, e) in enumerate(seq):
+            if win(t0) and flags[n] == 0:
+                if e is not None:
+                    ev.append(e)
+                flags[n] = 1
+    return ev
+
+
+def _chain(dx, dy, dz, f):
+    """The spacing selection of cpp:1585-1587 / 1853-1855 (last true wins)."""
+    out = None
+    if dx <= dy and dx <= dz:
+        out = f(dx)
+    if dy <= dy and dy <= dz:
+        out = f(dy)
+    if dz <= dy and dz <= dz:
+        out = f(dz)
+    return out
+
+
+def run_dropin(cfg, mask, vel_raw, gpu=True):
+    """The reference's solve_gpu (gpu=True, cu:1187-1227) or solve (cpp:35-66)
+    loop in FP64 for one fluid: returns (rows (n, 4), dTime, nStep).
+    cfg: dict of the data-file values (see initializator_amd.refio)."""
+    import math
+
+    from initializator_amd import layout
+
+    nz, ny, nx = mask.shape
+    dx, dy, dz = cfg["DX"], cfg["DY"], cfg["DZ"]
+    fluid = cfg.get("NFLUID", 1)
+    c1, c2 = (1.0, 0.0) if fluid == 1 else (0.0, 1.0)
+    Dw = c1 * cfg["D_WAT"] + c2 * cfg["D_OIL"]
+    T1b = 1.0 / (c1 / cfg["T1_WAT"] + c2 / cfg["T1_OIL"])
+    T2b = 1.0 / (c1 / cfg["T2_WAT"] + c2 / cfg["T2_OIL"])
+    r1 = c1 * cfg["R1_WAT"] + c2 * cfg["R1_OIL"]
+    r2 = c1 * cfg["R2_WAT"] + c2 * cfg["R2_OIL"]
+    ndf = cfg["NDIRECT_FIELD"]
+    bs = (cfg["BX0"], cfg["BY0"], cfg["BZ0"])
+    b0 = [bs[i] if (ndf == i + 1 or ndf == 0) else 0.0 for i in range(3)]
+    btot = math.sqrt(sum(x * x for x in bs))
+    d = [(1.0 if ndf == i + 1 else 0.0) if ndf != 0 else bs[i] / btot for i in range(3)]
+    buf = (cfg["bufferX"], cfg["bufferY"], cfg["bufferZ"])
+    peq = 1.0 / ((nx - 2 * buf[0]) * (ny - 2 * buf[1]) * (nz - 2 * buf[2]))
+    p = params(mask.shape, dx, 1.0, dy=dy, dz=dz, precession=cfg["NVECTOR"] == 1, velocity=cfg["NVELOC"] == 1,
+               diffusion=cfg["NDIFFUS"] == 1, p_eq=peq, b0=b0)
+    act = layout.pad_mask(mask)
+    D, t1, t2, (dmax, tmin) = fill_coefficients(p, act, Dw, T1b, T2b, r1, r2)
+    # step limits (cpp:1583-1587, 1849-1864, 1980-1988, 2519-2534)
+    cfl, cfl1, g = cfg["CFL"], cfg["CFL1"], G_REF
+    if cfg["NVELOC"] == 1:
+        vmax = max(0.0, max(float(v.max()) for v in vel_raw))
+        vmin = min(0.0, min(float(v.min()) for v in vel_raw))
+        vm = vmax * cfg["VEL_MULT"] if vmax > abs(vmin) else abs(vmin) * cfg["VEL_MULT"]
+        dt_vel = _chain(dx, dy, dz, lambda h: cfl * h / (vm + SMALL))
+    else:
+        dt_vel = 1.0
+    dt_diff = _chain(dx, dy, dz, lambda h: 2.0 * cfl * h * h / dmax) if cfg["NDIFFUS"] == 1 else 1.0
+    dt_rel = cfl1 * tmin
+    grads = (cfg["GRAD_X"], cfg["GRAD_Y"], cfg["GRAD_Z"])
+    ga = SMALL
+    for gval, h, n in zip(grads, (dx, dy, dz), (nx, ny, nz)):
+        if gval != 0:
+            ga = abs(gval * h * n)
+    noise = cfg["AMPL_NOISE"]
+    dt_noise = cfl1 / (g * noise) if noise != 0 else math.inf
+    dt_grad = cfl1 / (g * ga)
+    T1s, T2s = cfg["TIME1"], cfg["TIME2"]
+    dt = step_time(dt_noise, dt_grad, dt_vel, dt_diff, dt_rel, T1s, T2s)
+    nseq = cfg["NSEQUENCE"]
+    end = {0: 2 * cfg["T2_WAT"], 1: 2 * cfg["T2_WAT"], 2: 4.0 * T1s + T2s}.get(nseq, T2s + 2.0 * T1s)
+    nstep = int(end / dt) + 1
+    p.dt = dt
+    if cfg["NVELOC"] == 1:
+        vel = [np.ascontiguousarray(x * cfg["VEL_MULT"]) for x in layout.pad_velocity(*vel_raw, mask.shape)]
+    else:
+        vel = [np.zeros(padded_shape(p)) for _ in range(3)]
+    pad_pore = act == ord("1")
+    m = [np.where(pad_pore, peq * d[q], 0.0) for q in range(3)]
+    nf = fill_gradient(p, grads if cfg["NVECTOR"] == 1 else (0.0, 0.0, 0.0))
+    a = [x.copy() for x in m]
+    flags = [0] * 5
+    gp = cfg.get("gradientPulse", 0.0)
+    mg = cfg.get("multipleGradients", (0.0, 0.0, 0.0))
+    rows = []
+    ndp = cfg["Printing on"]
+    for i in range(nstep + 1):
+        t = i * dt
+        if not gpu and i % ndp == 0:
+            rows.append([t] + [output(p, act, buf, m[q])[0] for q in range(3)])
+        explicit_simple(p, act, D, t1, t2, vel, nf, m, a)
+        for fl, par, ax in sequence_events(gpu, nseq, t, dt, T1s, T2s, d, gp, mg, flags):
+            add_impulse(p, act, fl, par, ax, m)
+        for x in m:
+            bc_field(p, x)
+        if gpu and i % ndp == 0:
+            rows.append([t] + [output(p, act, buf, m[q])[0] for q in range(3)])
+    return np.array(rows), dt, nstep

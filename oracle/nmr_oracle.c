@@ -21,8 +21,9 @@
  *   or_fill_gradient     pinned: closed form G*d*(idx-1/2)
  *   or_output            pinned: brute-force numpy sum with trimming
  *   or_step_time         pinned: formula arithmetic
- *   or_maccormack        pinned: reduces to Heun for relaxation-only and
- *                        diffusion-only closed forms
+ *   or_add_impulse       pinned: Rodrigues rotation closed form (sign
+ *                        convention), 2*pi identity, norm preservation,
+ *                        gradient phase linear in position
  */
 #include <math.h>
 #include <stdint.h>
@@ -357,4 +358,49 @@ void or_run(const or_params* p, const char* act, const double* D, const double* 
     }
     free(ax); free(ay); free(az);
     free(nfx); free(nfy); free(nfz);
+}
+
+/* Rotation of one cell's M by `angle` about (nx, ny, nz)
+ * (NmrRelaxationSolve.cpp:2536-2550). */
+static void or_ort(double* mx, double* my, double* mz, long c, double nx, double ny, double nz, double angle)
+{
+    const double cs = cos(angle), sn = sin(angle);
+    const double v0 = mx[c], v1 = my[c], v2 = mz[c];
+    const double scal = nx * v0 + ny * v1 + nz * v2;
+    mx[c] = v0 * cs + nx * (1.0 - cs) * scal + sn * (ny * v2 - nz * v1);
+    my[c] = v1 * cs + ny * (1.0 - cs) * scal + sn * (nz * v0 - nx * v2);
+    mz[c] = v2 * cs + nz * (1.0 - cs) * scal + sn * (nx * v1 - ny * v0);
+}
+
+/* Instantaneous pulse on every pore cell of the padded domain, ghosts
+ * included (NmrRelaxationSolve.cpp:2552-2608); flags 1-4 as there. */
+void or_add_impulse(const or_params* p, const char* act, int flag, double grad, double nx, double ny,
+                    double nz, double* mx, double* my, double* mz)
+{
+    double length = pow(nx * nx + ny * ny + nz * nz, 0.5);
+    if (length == 0.0) length = 1.0;
+    const double nx0 = nx / length, ny0 = ny / length, nz0 = nz / length;
+    const double distance = 10e+12;
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int j = 0; j < p->jk + 2; j++)
+            for (int i = 0; i < p->ik + 2; i++) {
+                const long c = IX(p, i, j, k);
+                if (act[c] != '1') continue;
+                if (flag == 1) {
+                    const double angle = grad * p->gamma * ((i - 1) * p->dx * nx + (j - 1) * p->dy * ny + (k - 1) * p->dz * nz);
+                    or_ort(mx, my, mz, c, nx0, ny0, nz0, angle);
+                } else if (flag == 2) {
+                    or_ort(mx, my, mz, c, nx, ny, nz, grad);
+                } else if (flag == 3) {
+                    const double eff = pow(nx * nx + ny * ny + nz * nz, 0.5);
+                    const double t = ((double)i - 1.0 - distance) * p->dx * nx0 + ((double)j - 1.0 - distance) * p->dy * ny0 +
+                                     ((double)k - 1.0 - distance) * p->dz * nz0;
+                    or_ort(mx, my, mz, c, nx0, ny0, nz0, p->gamma * eff * pow(pow(t, 2), 0.5));
+                } else if (flag == 4) {
+                    const double angle = 0.0 - p->gamma * sqrt(pow((i - 0.5) * p->dx * nx - 0.5 * p->ik * p->dx * nx, 2.0) +
+                                                               pow((j - 0.5) * p->dy * ny - 0.5 * p->jk * p->dy * ny, 2.0) +
+                                                               pow((k - 0.5) * p->dz * nz - 0.5 * p->kk * p->dz * nz, 2.0));
+                    or_ort(mx, my, mz, c, nx, ny, nz, angle);
+                }
+            }
 }

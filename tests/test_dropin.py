@@ -1,0 +1,186 @@
+"""The drop-in NmrRelaxationSolve class (initializator_amd/host) driven by the
+reference's own input files: parsing (CPU), refusal of configurations the GPU
+solver does not implement (CPU), and results_gpu.csv / results_cpu.csv
+against closed forms and the FP64 restatement of the reference's loops (GPU).
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from initializator_amd import generators, refio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "initializator_amd", "nmr_exe")
+REFDATA = os.path.join(ROOT, "tests", "golden", "reference_data")
+
+
+@pytest.fixture(scope="module")
+def exe():
+    if not os.path.exists(EXE):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "initializator_amd", "csrc")], check=True,
+                       stdout=subprocess.DEVNULL)
+    return EXE
+
+
+def _run(exe, cwd, *args, env=None, check=True):
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([exe, *args], cwd=cwd, capture_output=True, text=True, env=e, timeout=600)
+    if check and r.returncode != 0:
+        raise AssertionError(f"nmr_exe failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return r
+
+
+def _config(exe, cwd, *args, full=False):
+    r = _run(exe, cwd, *args, *([] if full else ["--print-config"]), env={"NMR_INIT_CONFIG": "1"} if full else None)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_parses_the_reference_data_files(exe, tmp_path):
+    """The shipped data/Nmr_data.dat (61 substring-keyed lines) and
+    Nmr_grad.dat are read as the reference reads them (cpp:175-810)."""
+    os.makedirs(tmp_path / "data")
+    for f in ("Nmr_data.dat", "Nmr_grad.dat", "Nmr_start.dat"):
+        shutil.copy(os.path.join(REFDATA, f), tmp_path / "data" / f)
+    c = _config(exe, tmp_path, "200", "1", "399")
+    assert (c["ik"], c["jk"], c["kk"]) == (10, 10, 10)
+    assert c["ndPrint"] == 500 and c["nComponents"] == 2 and c["nSequence"] == 98
+    assert c["nVelocity"] == 1 and c["nDirectionField"] == 1 and c["nVectorMult"] == 0
+    assert c["dx"] == 2.67e-6 and c["bsx"] == 1.0 and c["time2"] == 70e-3 and c["time1"] == 0.0
+    assert c["velMult"] == 1e-2 and c["dWater"] == 2.7e-9 and c["dOil"] == 2.1e-10
+    assert c["cfl"] == 0.002 and c["cfl1"] == 0.002 and c["t2Water"] == 2.5e10
+    assert c["pEq"] == 1e-3  # the 0.31 = 310 pores x 1/1000 of results_gpu.csv
+    assert c["endTime"] == pytest.approx(70e-3)  # TIME2 + 2 TIME1 for sequence 98
+    grad = [float(x) for x in open(os.path.join(REFDATA, "Nmr_grad.dat")).read().split()[1:]]
+    assert c["gradientPulse"] == grad[199] == 0.0
+    assert c["multipleGradients"] == [grad[199], grad[0], grad[398]]
+
+
+def test_refuses_two_phase_before_touching_a_device(exe, tmp_path):
+    """NCOMP=2 is not implemented on the GPU: init() says so and exits 1 (the
+    reference would run on with missing inputs)."""
+    os.makedirs(tmp_path / "data")
+    for f in ("Nmr_data.dat", "Nmr_grad.dat", "Nmr_start.dat"):
+        shutil.copy(os.path.join(REFDATA, f), tmp_path / "data" / f)
+    r = _run(exe, tmp_path, "1", "1", "1", check=False)
+    assert r.returncode == 1 and "NCOMP=2" in r.stderr
+
+
+def test_written_inputs_round_trip(exe, tmp_path):
+    """refio.write_inputs produces files the drop-in reads back, including the
+    step size and step count of StepTime (cpp:2519-2534)."""
+    shape = (6, 5, 7)
+    mask = generators.porosity(shape, 0.5, seed=1, sigma=1.0)
+    vx, vy, vz = generators.velocity(mask, seed=1, sigma=1.5, vmax=2e-3)
+    cfg = _write_case(tmp_path, shape, mask, (vx, vy, vz), NSEQUENCE=2, TIME1=2e-4, TIME2=5e-4, R2_WAT=1e-5)
+    c = _config(exe, tmp_path, "5", "0", "0", full=True)
+    _, dt, nstep = _limits_only(cfg, mask, (vx, vy, vz))
+    assert c["dTime"] == dt and c["nStep"] == nstep
+    assert c["nPoro"] == 1 and c["nVelocity"] == 1 and c["nSequence"] == 2 and c["time1"] == 2e-4
+
+
+def _limits_only(cfg, mask, vel):
+    """dTime and nStep as the FP64 restatement computes them (no time loop)."""
+    import math
+
+    dx, dy, dz = cfg["DX"], cfg["DY"], cfg["DZ"]
+    from initializator_amd import layout
+
+    p = oracle.params(mask.shape, dx, 1.0, dy=dy, dz=dz)
+    act = layout.pad_mask(mask)
+    _, _, _, (dmax, tmin) = oracle.fill_coefficients(p, act, cfg["D_WAT"], cfg["T1_WAT"], cfg["T2_WAT"],
+                                                     cfg["R1_WAT"], cfg["R2_WAT"])
+    vmax = max(0.0, max(float(v.max()) for v in vel)) if cfg["NVELOC"] else 0.0
+    vmin = min(0.0, min(float(v.min()) for v in vel)) if cfg["NVELOC"] else 0.0
+    vm = vmax * cfg["VEL_MULT"] if vmax > abs(vmin) else abs(vmin) * cfg["VEL_MULT"]
+    dt_vel = oracle._chain(dx, dy, dz, lambda h: cfg["CFL"] * h / (vm + oracle.SMALL)) if cfg["NVELOC"] else 1.0
+    dt_diff = oracle._chain(dx, dy, dz, lambda h: 2.0 * cfg["CFL"] * h * h / dmax)
+    dt = oracle.step_time(math.inf, cfg["CFL1"] / (oracle.G_REF * oracle.SMALL), dt_vel, dt_diff,
+                          cfg["CFL1"] * tmin, cfg["TIME1"], cfg["TIME2"])
+    end = {0: 2 * cfg["T2_WAT"], 1: 2 * cfg["T2_WAT"], 2: 4.0 * cfg["TIME1"] + cfg["TIME2"]}.get(
+        cfg["NSEQUENCE"], cfg["TIME2"] + 2.0 * cfg["TIME1"])
+    return None, dt, int(end / dt) + 1
+
+
+def _write_case(root, shape, mask, vel, grad_table=None, **ov):
+    cfg = dict(refio.DATA_KEYS)
+    cfg = {k.split(" ")[0] if not k.startswith("Printing") else "Printing on": v for k, v in cfg.items()}
+    cfg.update({k.replace("_", " ") if k.startswith("Printing") else k: v for k, v in ov.items()})
+    if mask is not None:
+        cfg["NPOR"] = 1
+    if vel is not None:
+        cfg["NVELOC"] = 1
+    refio.write_inputs(str(root), shape, mask, *(vel if vel is not None else (None, None, None)),
+                       grad_table=grad_table, **ov)
+    return cfg
+
+
+@pytest.mark.gpu
+def test_fid_closed_form(exe, tmp_path):
+    """FID on an all-pore box with bulk T2 only: the 90 deg pulse about x turns
+    pEq*N = 1 along z into -1 along y, which then decays by (1 - dt/T2)^i
+    (results_gpu.csv row i holds i+1 updates and the t = 0 pulse)."""
+    shape = (10, 6, 8)
+    cfg = _write_case(tmp_path, shape, None, None, NPOR=0, NSEQUENCE=0, T2_WAT=0.01, Printing_on=50)
+    _run(exe, tmp_path, "1", "0", "0")
+    rows = refio.read_results(tmp_path / "results_gpu.csv")
+    c = _config(exe, tmp_path, "1", "0", "0", full=True)
+    dt = c["dTime"]
+    i = np.arange(0, c["nStep"] + 1, 50)
+    assert len(rows) == len(i)
+    np.testing.assert_allclose(rows[:, 0], i * dt, rtol=1e-6)
+    np.testing.assert_allclose(rows[:, 2], -(1 - dt / 0.01) ** i, rtol=5e-5)  # FP32 rounding of a uniform field
+    assert np.abs(rows[:, 1]).max() < 1e-6 and np.abs(rows[:, 3]).max() < 1e-6
+    assert os.path.exists(tmp_path / "results_NMR" / "results_grad_1_0_0.csv")
+
+
+@pytest.mark.gpu
+def test_cpmg_echo_train_closed_form(exe, tmp_path):
+    """CPMG (20 echoes on the GPU path): each 180 deg pulse about x flips My."""
+    shape = (6, 6, 6)
+    T1s, T2 = 2e-4, 2e-3
+    _write_case(tmp_path, shape, None, None, NPOR=0, NSEQUENCE=1, TIME1=T1s, T2_WAT=T2, Printing_on=7)
+    _run(exe, tmp_path, "1", "0", "0")
+    rows = refio.read_results(tmp_path / "results_gpu.csv")
+    c = _config(exe, tmp_path, "1", "0", "0", full=True)
+    dt = c["dTime"]
+    # every step whose time lies in a window [(2e+1) T1, (2e+1) T1 + dt) gets a
+    # 180 deg pulse (no flag gating in the CPMG branch): a window that rounds
+    # to two steps pulses twice, as the reference does
+    flips = [k for e in range(20) for k in range(c["nStep"] + 1)
+             if (k * dt < (2 * e + 1) * T1s + dt) and (k * dt >= (2 * e + 1) * T1s)]
+    assert len(flips) >= 5
+    i = np.arange(0, c["nStep"] + 1, 7)
+    sign = np.array([(-1) ** sum(1 for m in flips if m <= k) for k in i])
+    np.testing.assert_allclose(rows[:, 2], -sign * (1 - dt / T2) ** i, rtol=5e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cadence", ["gpu", "cpu"])
+def test_pfg_flow_against_fp64_loop(exe, tmp_path, cadence):
+    """Porous box with divergence-free flow, surface relaxation and a pulsed-
+    gradient stimulated echo (sequence 2): every CSV row of solve_gpu()
+    (results_gpu.csv) or solve() (results_cpu.csv) matches the FP64
+    restatement of the reference's loop to the CSV's 7 digits."""
+    shape = (12, 10, 14)
+    mask = generators.porosity(shape, 0.55, seed=6, sigma=1.2)
+    vel = generators.velocity(mask, seed=6, sigma=1.5, vmax=5e-3)
+    table = np.linspace(-0.03, 0.03, 399)
+    cfg = _write_case(tmp_path, shape, mask, vel, grad_table=table, NSEQUENCE=2, TIME1=2e-4, TIME2=4e-4,
+                      R2_WAT=2e-5, R1_WAT=1e-5, T1_WAT=0.05, Printing_on=5, BX0=0.0, BZ0=1.0)
+    cfg["gradientPulse"] = float(table[349])
+    args = ["350", "0", "0"] + (["--cpu-cadence"] if cadence == "cpu" else [])
+    _run(exe, tmp_path, *args)
+    rows = refio.read_results(tmp_path / f"results_{cadence}.csv")
+    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=(cadence == "gpu"))
+    assert rows.shape == ref.shape, (rows.shape, ref.shape)
+    np.testing.assert_allclose(rows[:, 0], ref[:, 0], rtol=2e-6)
+    scale = np.abs(ref[:, 1:]).max()
+    err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
+    print(f"[dropin] sequence 2, {cadence} cadence: {len(rows)} rows, max error {err:.2e} of the signal scale")
+    assert err < 5e-6

@@ -339,3 +339,63 @@ def test_results_gpu_csv_invariant_fixture():
     np.testing.assert_allclose(total, 0.31, rtol=5e-7)
     assert np.all(np.diff(vals[:, 0]) > 0)
     np.testing.assert_allclose(np.diff(vals[:, 0]), 500 * 1.078552e-5, rtol=1e-5)
+
+
+def _rot(v, n, ang):
+    """Closed-form rotation of the reference's OrtTransform: right-handed
+    rotation by ang about the unit axis n (Rodrigues)."""
+    n = np.asarray(n, float)
+    v = np.asarray(v, float)
+    c, s = np.cos(ang), np.sin(ang)
+    return v * c + np.cross(n, v) * s + n * (n @ v) * (1 - c)
+
+
+def test_rf_pulse_rotation_closed_form():
+    """Flag 2 (RF pulse): every pore rotates by the given angle about the given
+    axis; solids are untouched (cpp:2536-2550, 2578-2582)."""
+    shape = (3, 4, 5)
+    rng = np.random.default_rng(2)
+    mask = (rng.random(shape) < 0.6).astype(np.uint8)
+    p = oracle.params(shape, 1e-6, 1e-5)
+    act = layout.pad_mask(mask)
+    m0 = [layout.pad_field(rng.standard_normal(shape)) for _ in range(3)]
+    axis = np.array([0.3, -0.4, np.sqrt(1 - 0.25)])
+    for ang in (0.5 * np.pi, np.pi, 1.234):
+        m = [x.copy() for x in m0]
+        oracle.add_impulse(p, act, 2, ang, axis, m)
+        for (k, j, i) in [(0, 0, 0), (2, 3, 4), (1, 2, 3)]:
+            c = (k + 1, j + 1, i + 1)
+            v0 = [x[c] for x in m0]
+            got = [x[c] for x in m]
+            if mask[k, j, i]:
+                np.testing.assert_allclose(got, _rot(v0, axis, ang), atol=1e-13)
+            else:
+                assert got == v0
+    # 90 deg about x takes z to -y (sign convention of the reference)
+    m = [np.zeros(oracle.padded_shape(p)) for _ in range(3)]
+    m[2][:] = 1.0
+    oracle.add_impulse(p, act, 2, 0.5 * np.pi, (1.0, 0.0, 0.0), m)
+    c = tuple(np.argwhere(layout.pad_mask(mask) == ord("1"))[0])
+    assert abs(m[0][c]) < 1e-15 and abs(m[1][c] + 1) < 1e-15 and abs(m[2][c]) < 1e-15
+
+
+def test_gradient_pulse_phase_linear_in_position():
+    """Flag 1: angle = G g ((i-1)dx nx + (j-1)dy ny + (k-1)dz nz) about the unit
+    gradient direction; two opposite pulses cancel (cpp:2573-2577)."""
+    shape = (4, 3, 6)
+    dx, dy, dz = 1e-6, 2e-6, 3e-6
+    p = oracle.params(shape, dx, 1e-5, dy=dy, dz=dz)
+    act = layout.pad_mask(np.ones(shape, np.uint8))
+    g = np.array([0.2, 0.0, 0.1])
+    G = 1e-3
+    m = [np.full(oracle.padded_shape(p), 0.6), np.full(oracle.padded_shape(p), 0.0),
+         np.full(oracle.padded_shape(p), 0.8)]
+    m0 = [x.copy() for x in m]
+    oracle.add_impulse(p, act, 1, G, g, m)
+    n = g / np.linalg.norm(g)
+    for (k, j, i) in [(1, 1, 1), (4, 3, 6), (2, 1, 5)]:
+        ang = G * oracle.G_REF * ((i - 1) * dx * g[0] + (j - 1) * dy * g[1] + (k - 1) * dz * g[2])
+        np.testing.assert_allclose([x[k, j, i] for x in m], _rot([0.6, 0.0, 0.8], n, ang), atol=1e-13)
+    oracle.add_impulse(p, act, 1, -G, g, m)
+    for a, b in zip(m, m0):
+        np.testing.assert_allclose(a, b, atol=1e-13)
