@@ -150,6 +150,46 @@ int nmr_hip_variant_info(int32_t variant, int32_t* lanes_x, int32_t* rows_per_th
 /* Bytes of device memory the context holds. */
 int nmr_hip_device_bytes(nmr_ctx* ctx, int64_t* bytes);
 
+/* ---- batched solves (many uneven instances per launch) ----
+ * No reference counterpart: the reference solves one domain per process.
+ * All instances share the physics of `common` (ncomp, flags, phase mode,
+ * dx/dy/dz, dt, D, bulk relaxation, g, pEq, B0 direction, signal trim);
+ * each has its own grid, surface relaxivities and pulsed-gradient spin
+ * echo along z (+G on steps [0, nd), -G on [nD, nD+nd), the second lobe
+ * taking precedence where they overlap; nd = lround(delta/dt),
+ * nD = lround(big_delta/dt)).  One persistent launch advances every
+ * instance by one step; chunks of steps are captured in a HIP graph and
+ * replayed.  Arrays are in the COMPACT layout. */
+typedef struct nmr_batch_instance {
+    int32_t nx, ny, nz, reserved;
+    double rho1, rho2;      /* surface relaxivities (m/s) */
+    double grad_amp;        /* pulsed gradient amplitude along z (T/m) */
+    double delta, big_delta;/* lobe length and lobe spacing (s) */
+} nmr_batch_instance;
+
+typedef struct nmr_batch nmr_batch;
+
+int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_batch_instance* inst,
+                         nmr_batch** out);
+int nmr_hip_batch_destroy(nmr_batch* b);
+/* Mask (0/1), face velocities (nullable when the flags have no velocity)
+ * and magnetization (components beyond ncomp may be NULL) of instance i. */
+int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const double* vx, const double* vy,
+                         const double* vz, const double* mx, const double* my, const double* mz);
+int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, double* mz);
+/* signal (nullable): [ninst][nsteps+1][ncomp]; step counting continues
+ * across calls (the gradient schedule sees steps 0, 1, ... since create). */
+int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal);
+/* variant (-1 automatic / autotuned), z-chunk (0 automatic), steps per
+ * captured graph (0: direct launches, no graph; must be even). */
+int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps);
+/* Times every one-step tile variant and z-chunk on this batch (repeated
+ * single steps from the current state, which stays unchanged) and keeps the
+ * fastest; the choice is cached per set of instance shapes. */
+int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, int32_t* lz);
+/* GPU ms of the last run's steps and the voxel-steps they advanced. */
+int nmr_hip_batch_last_run_ms(nmr_batch* b, double* ms, int64_t* voxel_steps);
+
 const char* nmr_hip_strerror(int code);
 const char* nmr_hip_last_error(void);
 int nmr_hip_abi_version(void);

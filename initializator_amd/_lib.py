@@ -52,6 +52,14 @@ EXPORTS = (
     "nmr_hip_num_variants",
     "nmr_hip_variant_info",
     "nmr_hip_device_bytes",
+    "nmr_hip_batch_create",
+    "nmr_hip_batch_destroy",
+    "nmr_hip_batch_upload",
+    "nmr_hip_batch_download",
+    "nmr_hip_batch_run",
+    "nmr_hip_batch_set_tile",
+    "nmr_hip_batch_autotune",
+    "nmr_hip_batch_last_run_ms",
     "nmr_hip_strerror",
     "nmr_hip_last_error",
     "nmr_hip_abi_version",
@@ -84,6 +92,22 @@ class NmrProblemC(ctypes.Structure):
         ("p_eq", c_double),
         ("b0_dir", c_double * 3),
         ("grad", c_double * 3),
+    ]
+
+
+class NmrBatchInstanceC(ctypes.Structure):
+    """Mirror of ``nmr_batch_instance`` in include/nmr_hip.h."""
+
+    _fields_ = [
+        ("nx", c_int32),
+        ("ny", c_int32),
+        ("nz", c_int32),
+        ("reserved", c_int32),
+        ("rho1", c_double),
+        ("rho2", c_double),
+        ("grad_amp", c_double),
+        ("delta", c_double),
+        ("big_delta", c_double),
     ]
 
 
@@ -138,6 +162,14 @@ def load(path: str | None = None) -> ctypes.CDLL:
             [c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)],
         ),
         "nmr_hip_device_bytes": (c_int, [vp, POINTER(c_int64)]),
+        "nmr_hip_batch_create": (c_int, [POINTER(NmrProblemC), c_int32, POINTER(NmrBatchInstanceC), POINTER(vp)]),
+        "nmr_hip_batch_destroy": (c_int, [vp]),
+        "nmr_hip_batch_upload": (c_int, [vp, c_int32, POINTER(c_uint8), dp, dp, dp, dp, dp, dp]),
+        "nmr_hip_batch_download": (c_int, [vp, c_int32, dp, dp, dp]),
+        "nmr_hip_batch_run": (c_int, [vp, c_int64, dp]),
+        "nmr_hip_batch_set_tile": (c_int, [vp, c_int32, c_int32, c_int32]),
+        "nmr_hip_batch_autotune": (c_int, [vp, c_int32, POINTER(c_int32), POINTER(c_int32)]),
+        "nmr_hip_batch_last_run_ms": (c_int, [vp, dp, POINTER(c_int64)]),
         "nmr_hip_strerror": (c_char_p, [c_int]),
         "nmr_hip_last_error": (c_char_p, []),
         "nmr_hip_abi_version": (c_int, []),

@@ -779,3 +779,577 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched solves: one persistent launch per step over all instances.
+
+#include <map>
+#include <mutex>
+
+struct nmr_batch {
+    nmr_problem prob{};
+    int ninst = 0, nc = 1;
+    bool vel = false, exact = false;
+    std::vector<nmr_batch_instance> spec;
+    struct Inst {
+        int pitch = 0, pitch1 = 0;
+        size_t n_m = 0, n_vx = 0, n_vy = 0, n_vz = 0;
+        float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+        uint8_t* mask = nullptr;
+        float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+        bool uploaded = false;
+        int inert = 0;
+    };
+    std::vector<Inst> inst;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int variant = -1, lz = 0, graph_steps = 64;
+    int parity = 0;
+    int64_t steps_done = 0;
+    // launch state for the current variant / lz
+    int cur_variant = -2, cur_lz = -1, grid = 0, ntiles = 0;
+    nmr::BatchInst* d_inst = nullptr;
+    int2* d_tiles = nullptr;
+    int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
+    double* d_part = nullptr;
+    size_t part_cap = 0;
+    double* d_signal = nullptr;
+    size_t signal_cap = 0;
+    double* d_fpart = nullptr; // field-sum partials (max nz)
+    int* d_flag = nullptr;
+    double last_ms = 0.0;
+    int64_t last_voxel_steps = 0;
+};
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::vector<int>, std::pair<int, int>> g_tune_cache; // shapes -> (variant, lz)
+
+void batch_free(nmr_batch* b)
+{
+    for (auto& I : b->inst) {
+        for (int p = 0; p < 2; p++)
+            for (int q = 0; q < 3; q++)
+                if (I.m[p][q]) (void)hipFree(I.m[p][q]);
+        if (I.mask) (void)hipFree(I.mask);
+        if (I.vx) (void)hipFree(I.vx);
+        if (I.vy) (void)hipFree(I.vy);
+        if (I.vz) (void)hipFree(I.vz);
+    }
+    for (void* p : {(void*)b->d_inst, (void*)b->d_tiles, (void*)b->d_off, (void*)b->d_ids, (void*)b->d_step,
+                    (void*)b->d_part, (void*)b->d_signal, (void*)b->d_fpart, (void*)b->d_flag})
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+template <typename T>
+int balloc(T** p, size_t count)
+{
+    HIP_TRY(hipMalloc((void**)p, std::max<size_t>(count * sizeof(T), 256)));
+    HIP_TRY(hipMemset(*p, 0, std::max<size_t>(count * sizeof(T), 256)));
+    return NMR_OK;
+}
+
+// Tile list, descriptors and partials for (variant, lz).
+int batch_prepare(nmr_batch* b, int variant, int lz)
+{
+    if (variant == b->cur_variant && lz == b->cur_lz) return NMR_OK;
+    int lx, rpt, tx, ty, st;
+    if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1)
+        return fail(NMR_ERR_INVALID, "batch variant %d is not a one-step tile", variant);
+    const nmr_problem& p = b->prob;
+    std::vector<nmr::BatchInst> hi(b->ninst);
+    struct T { int inst, t; long cost; };
+    std::vector<T> tiles;
+    for (int i = 0; i < b->ninst; i++) {
+        const nmr_batch_instance& s = b->spec[i];
+        auto& I = b->inst[i];
+        nmr::BatchInst& d = hi[i];
+        for (int q = 0; q < 3; q++) {
+            d.m[0][q] = I.m[0][q];
+            d.m[1][q] = I.m[1][q];
+        }
+        d.mask = I.mask;
+        d.vx = I.vx;
+        d.vy = I.vy;
+        d.vz = I.vz;
+        d.nx = s.nx;
+        d.ny = s.ny;
+        d.nz = s.nz;
+        d.pitch = I.pitch;
+        d.pitch1 = I.pitch1;
+        d.lz = std::min(std::max(lz, 1), s.nz);
+        d.ntx = (s.nx + tx - 1) / tx;
+        d.nty = (s.ny + ty - 1) / ty;
+        d.ntz = (s.nz + d.lz - 1) / d.lz;
+        d.bx0 = p.buffer[0]; d.bx1 = s.nx - p.buffer[0];
+        d.by0 = p.buffer[1]; d.by1 = s.ny - p.buffer[1];
+        d.bz0 = p.buffer[2]; d.bz1 = s.nz - p.buffer[2];
+        d.inert = I.inert;
+        nmr_problem q = p;
+        q.rho1 = s.rho1;
+        q.rho2 = s.rho2;
+        q.flags &= ~NMR_F_PRECESSION; // the gradient is chosen per step in the kernel
+        nmr_ctx tmp;
+        tmp.prob = q;
+        const double g0[3] = {0.0, 0.0, 0.0};
+        d.c = make_coef(&tmp, g0);
+        d.nd = (int)std::lround(s.delta / p.dt);
+        d.nD = (int)std::lround(s.big_delta / p.dt);
+        // same rounding as make_coef for a gradient (0, 0, G)
+        d.bgz = (p.flags & NMR_F_PRECESSION) ? (float)(p.dt * p.gamma * s.grad_amp * p.dz) : 0.f;
+        const int n = d.ntx * d.nty * d.ntz;
+        for (int t = 0; t < n; t++) {
+            const int tz = t / (d.ntx * d.nty);
+            const long planes = std::min(d.lz, s.nz - tz * d.lz);
+            tiles.push_back({i, t, planes * (long)tx * ty});
+        }
+    }
+    // longest tiles first (stable: ties keep instance/tile order)
+    std::stable_sort(tiles.begin(), tiles.end(), [](const T& a, const T& c) { return a.cost > c.cost; });
+    std::vector<int2> ht(tiles.size());
+    std::vector<std::vector<int>> per(b->ninst);
+    for (size_t k = 0; k < tiles.size(); k++) {
+        ht[k] = make_int2(tiles[k].inst, tiles[k].t);
+        per[tiles[k].inst].push_back((int)k);
+    }
+    std::vector<int> off(b->ninst + 1, 0), ids;
+    for (int i = 0; i < b->ninst; i++) {
+        std::sort(per[i].begin(), per[i].end()); // fixed summation order
+        off[i + 1] = off[i] + (int)per[i].size();
+        ids.insert(ids.end(), per[i].begin(), per[i].end());
+    }
+    for (void* q : {(void*)b->d_inst, (void*)b->d_tiles, (void*)b->d_off, (void*)b->d_ids})
+        if (q) HIP_TRY(hipFree(q));
+    b->d_inst = nullptr;
+    b->d_tiles = nullptr;
+    b->d_off = nullptr;
+    b->d_ids = nullptr;
+    int rc;
+    if ((rc = balloc(&b->d_inst, hi.size())) || (rc = balloc(&b->d_tiles, ht.size())) ||
+        (rc = balloc(&b->d_off, off.size())) || (rc = balloc(&b->d_ids, ids.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(b->d_inst, hi.data(), hi.size() * sizeof(nmr::BatchInst), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_tiles, ht.data(), ht.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    b->ntiles = (int)tiles.size();
+    const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, variant);
+    if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
+    b->grid = std::min(res, b->ntiles);
+    b->cur_variant = variant;
+    b->cur_lz = lz;
+    return NMR_OK;
+}
+
+int batch_partials(nmr_batch* b, size_t steps)
+{
+    const size_t need = steps * (size_t)b->ntiles * b->nc;
+    if (need <= b->part_cap) return NMR_OK;
+    if (b->d_part) HIP_TRY(hipFree(b->d_part));
+    b->d_part = nullptr;
+    int rc = balloc(&b->d_part, need);
+    if (rc) return rc;
+    b->part_cap = need;
+    return NMR_OK;
+}
+
+int batch_launch_step(nmr_batch* b, int local, int parity, double* part)
+{
+    nmr::BatchArgs a{};
+    a.inst = b->d_inst;
+    a.tiles = b->d_tiles;
+    a.ntiles = b->ntiles;
+    a.parity = parity;
+    a.local_step = local;
+    a.step_base = b->d_step;
+    a.partials = part;
+    HIP_TRY(nmr::launch_batch(b->nc, b->vel, b->exact, b->cur_variant, b->grid, a, b->stream));
+    return NMR_OK;
+}
+
+void batch_auto(const nmr_batch* b, int* variant, int* lz)
+{
+    int mx = 0;
+    for (const auto& s : b->spec) mx = std::max(mx, s.nx);
+    *variant = mx > 64 ? 2 : (mx > 32 ? 4 : 6);
+    *lz = 8;
+}
+
+std::vector<int> batch_key(const nmr_batch* b)
+{
+    std::vector<int> k = {b->nc, (int)b->vel, (int)b->exact};
+    for (const auto& s : b->spec) {
+        k.push_back(s.nx);
+        k.push_back(s.ny);
+        k.push_back(s.nz);
+    }
+    return k;
+}
+
+} // namespace
+
+extern "C" {
+
+int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_batch_instance* inst, nmr_batch** out)
+{
+    if (!common || !inst || !out) return fail(NMR_ERR_NULL, "nmr_hip_batch_create: null argument");
+    *out = nullptr;
+    if (ninst < 1 || ninst > 65535) return fail(NMR_ERR_INVALID, "ninst %d (1..65535)", (int)ninst);
+    nmr_problem p = *common;
+    if (p.ncomp < 1 || p.ncomp > 3) return fail(NMR_ERR_INVALID, "ncomp %d", p.ncomp);
+    if (p.scheme != NMR_SCHEME_UPWIND) return fail(NMR_ERR_UNSUPPORTED, "scheme %d", p.scheme);
+    if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION)) return fail(NMR_ERR_INVALID, "flags");
+    if (!(p.dx > 0) || !(p.dy > 0) || !(p.dz > 0) || !(p.dt > 0)) return fail(NMR_ERR_INVALID, "dx, dy, dz, dt");
+    if (p.ncomp < 3 && (p.b0_dir[0] != 0.0 || p.b0_dir[1] != 0.0 || p.b0_dir[2] == 0.0))
+        return fail(NMR_ERR_UNSUPPORTED, "ncomp < 3 needs the main field along z");
+    if (p.ncomp == 1 && (p.flags & NMR_F_PRECESSION)) return fail(NMR_ERR_UNSUPPORTED, "precession needs ncomp >= 2");
+    if (p.phase_mode == NMR_PHASE_EXACT && p.ncomp != 2) return fail(NMR_ERR_UNSUPPORTED, "exact phase needs ncomp 2");
+    for (int i = 0; i < ninst; i++) {
+        const nmr_batch_instance& s = inst[i];
+        if (s.nx < 1 || s.ny < 1 || s.nz < 1 || (long)s.nx * s.ny * s.nz > (1L << 30))
+            return fail(NMR_ERR_INVALID, "instance %d grid %d x %d x %d", i, s.nx, s.ny, s.nz);
+        if (!(s.delta >= 0) || !(s.big_delta >= 0) || !std::isfinite(s.grad_amp))
+            return fail(NMR_ERR_INVALID, "instance %d gradient schedule", i);
+        for (int a = 0; a < 3; a++)
+            if (p.buffer[a] < 0) return fail(NMR_ERR_INVALID, "buffer");
+    }
+    nmr_batch* b = new (std::nothrow) nmr_batch();
+    if (!b) return fail(NMR_ERR_NOMEM, "host allocation");
+    auto bail = [&](int rc) {
+        batch_free(b);
+        delete b;
+        return rc;
+    };
+    b->prob = p;
+    b->ninst = ninst;
+    b->nc = p.ncomp;
+    b->vel = (p.flags & NMR_F_VELOCITY) != 0;
+    b->exact = p.phase_mode == NMR_PHASE_EXACT;
+    b->spec.assign(inst, inst + ninst);
+    b->inst.resize(ninst);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+        return bail(fail(NMR_ERR_DEVICE, "stream/event creation failed"));
+    int rc;
+    int maxnz = 1;
+    for (int i = 0; i < ninst; i++) {
+        const nmr_batch_instance& s = inst[i];
+        auto& I = b->inst[i];
+        I.pitch = round_up4(s.nx);
+        I.pitch1 = round_up4(s.nx + 1);
+        I.n_m = (size_t)s.nz * s.ny * I.pitch;
+        maxnz = std::max(maxnz, s.nz);
+        for (int par = 0; par < 2; par++)
+            for (int q = 0; q < b->nc; q++)
+                if ((rc = balloc(&I.m[par][q], I.n_m))) return bail(rc);
+        if ((rc = balloc(&I.mask, I.n_m))) return bail(rc);
+        if (b->vel) {
+            I.n_vx = (size_t)s.nz * s.ny * I.pitch1;
+            I.n_vy = (size_t)s.nz * (s.ny + 1) * I.pitch;
+            I.n_vz = (size_t)(s.nz + 1) * s.ny * I.pitch;
+            if ((rc = balloc(&I.vx, I.n_vx)) || (rc = balloc(&I.vy, I.n_vy)) || (rc = balloc(&I.vz, I.n_vz)))
+                return bail(rc);
+        }
+    }
+    if ((rc = balloc(&b->d_step, 1)) || (rc = balloc(&b->d_fpart, (size_t)maxnz * 3)) || (rc = balloc(&b->d_flag, 16)))
+        return bail(rc);
+    *out = b;
+    return NMR_OK;
+}
+
+int nmr_hip_batch_destroy(nmr_batch* b)
+{
+    if (!b) return NMR_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+    return NMR_OK;
+}
+
+int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const double* vx, const double* vy,
+                         const double* vz, const double* mx, const double* my, const double* mz)
+{
+    if (!b || !mask) return fail(NMR_ERR_NULL, "nmr_hip_batch_upload: null argument");
+    if (i < 0 || i >= b->ninst) return fail(NMR_ERR_INVALID, "instance %d", (int)i);
+    const double* src[3] = {mx, my, mz};
+    for (int q = 0; q < b->nc; q++)
+        if (!src[q]) return fail(NMR_ERR_NULL, "magnetization component %d is NULL", q);
+    const nmr_batch_instance& s = b->spec[i];
+    auto& I = b->inst[i];
+    const int P = I.pitch, P1 = I.pitch1;
+    {
+        std::vector<uint8_t> h(I.n_m, 0);
+        for (int k = 0; k < s.nz; k++)
+            for (int j = 0; j < s.ny; j++)
+                for (int x = 0; x < s.nx; x++) {
+                    const uint8_t v = mask[(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)];
+                    if (v > 1) return fail(NMR_ERR_INVALID, "mask value %d (want 0/1)", (int)v);
+                    h[(size_t)k * s.ny * P + (size_t)j * P + x] = v;
+                }
+        HIP_TRY(hipMemcpy(I.mask, h.data(), h.size(), hipMemcpyHostToDevice));
+    }
+    if (b->vel) {
+        std::vector<float> h(I.n_vx, 0.f);
+        if (vx)
+            for (int k = 0; k < s.nz; k++)
+                for (int j = 0; j < s.ny; j++)
+                    for (int x = 0; x <= s.nx; x++)
+                        h[(size_t)k * s.ny * P1 + (size_t)j * P1 + x] =
+                            (float)vx[(size_t)x + (size_t)(s.nx + 1) * ((size_t)j + (size_t)s.ny * k)];
+        HIP_TRY(hipMemcpy(I.vx, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        h.assign(I.n_vy, 0.f);
+        if (vy)
+            for (int k = 0; k < s.nz; k++)
+                for (int j = 0; j <= s.ny; j++)
+                    for (int x = 0; x < s.nx; x++)
+                        h[(size_t)k * (s.ny + 1) * P + (size_t)j * P + x] =
+                            (float)vy[(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)(s.ny + 1) * k)];
+        HIP_TRY(hipMemcpy(I.vy, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        h.assign(I.n_vz, 0.f);
+        if (vz)
+            for (int k = 0; k <= s.nz; k++)
+                for (int j = 0; j < s.ny; j++)
+                    for (int x = 0; x < s.nx; x++)
+                        h[(size_t)k * s.ny * P + (size_t)j * P + x] =
+                            (float)vz[(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)];
+        HIP_TRY(hipMemcpy(I.vz, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    std::vector<float> h(I.n_m, 0.f);
+    for (int q = 0; q < b->nc; q++) {
+        for (int k = 0; k < s.nz; k++)
+            for (int j = 0; j < s.ny; j++)
+                for (int x = 0; x < s.nx; x++)
+                    h[(size_t)k * s.ny * P + (size_t)j * P + x] =
+                        (float)src[q][(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)];
+        for (int par = 0; par < 2; par++) HIP_TRY(hipMemcpy(I.m[par][q], h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    // inertness of this instance (no velocity on faces touching solid)
+    if (!b->vel) {
+        I.inert = 1;
+    } else {
+        HIP_TRY(hipMemset(b->d_flag, 0, sizeof(int)));
+        nmr::InertArgs a{};
+        a.mask = I.mask;
+        a.vx = I.vx;
+        a.vy = I.vy;
+        a.vz = I.vz;
+        a.flag = b->d_flag;
+        a.nx = s.nx;
+        a.ny = s.ny;
+        a.nz = s.nz;
+        a.pitch = P;
+        a.pitch1 = P1;
+        HIP_TRY(nmr::launch_inert_check(a, b->stream));
+        int f = 0;
+        HIP_TRY(hipMemcpyAsync(&f, b->d_flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        I.inert = f == 0;
+    }
+    I.uploaded = true;
+    b->cur_variant = -2; // descriptors carry the inert flag: rebuild
+    return NMR_OK;
+}
+
+int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, double* mz)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_download: null batch");
+    if (i < 0 || i >= b->ninst) return fail(NMR_ERR_INVALID, "instance %d", (int)i);
+    double* dst[3] = {mx, my, mz};
+    const nmr_batch_instance& s = b->spec[i];
+    auto& I = b->inst[i];
+    std::vector<float> h(I.n_m);
+    for (int q = 0; q < b->nc; q++) {
+        if (!dst[q]) continue;
+        HIP_TRY(hipMemcpy(h.data(), I.m[b->parity][q], h.size() * 4, hipMemcpyDeviceToHost));
+        for (int k = 0; k < s.nz; k++)
+            for (int j = 0; j < s.ny; j++)
+                for (int x = 0; x < s.nx; x++)
+                    dst[q][(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)] =
+                        h[(size_t)k * s.ny * I.pitch + (size_t)j * I.pitch + x];
+    }
+    return NMR_OK;
+}
+
+int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_set_tile: null batch");
+    int lx, rpt, tx, ty, st;
+    if (variant != -1 && (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
+        return fail(NMR_ERR_INVALID, "batch variant %d", (int)variant);
+    if (lz < 0 || graph_steps < 0 || (graph_steps & 1)) return fail(NMR_ERR_INVALID, "lz %d / graph_steps %d", lz, graph_steps);
+    b->variant = variant;
+    b->lz = lz;
+    b->graph_steps = graph_steps;
+    return NMR_OK;
+}
+
+int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, int32_t* lz)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_autotune: null batch");
+    for (const auto& I : b->inst)
+        if (!I.uploaded) return fail(NMR_ERR_STATE, "autotune before every instance is uploaded");
+    const std::vector<int> key = batch_key(b);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            b->variant = it->second.first;
+            b->lz = it->second.second;
+            if (variant) *variant = b->variant;
+            if (lz) *lz = b->lz;
+            return NMR_OK;
+        }
+    }
+    const int trials = std::max(1, (int)trial_steps);
+    double best = 1e300;
+    int bv = 0, bl = 8;
+    for (int v = 0; v < nmr::num_variants(); v++) {
+        int lx, rpt, tx, ty, st;
+        nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st);
+        if (st != 1) continue;
+        for (int l : {4, 8, 16}) {
+            int rc = batch_prepare(b, v, l);
+            if (rc) return rc;
+            if ((rc = batch_partials(b, 1))) return rc;
+            // warm-up, then timed: every trial reads the current buffer and
+            // writes the other one, so the state is unchanged
+            if ((rc = batch_launch_step(b, 0, b->parity, b->d_part))) return rc;
+            HIP_TRY(hipEventRecord(b->ev0, b->stream));
+            for (int r = 0; r < trials; r++)
+                if ((rc = batch_launch_step(b, 0, b->parity, b->d_part))) return rc;
+            HIP_TRY(hipEventRecord(b->ev1, b->stream));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best) {
+                best = ms;
+                bv = v;
+                bl = l;
+            }
+        }
+    }
+    b->variant = bv;
+    b->lz = bl;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = {bv, bl};
+    }
+    if (variant) *variant = bv;
+    if (lz) *lz = bl;
+    return NMR_OK;
+}
+
+int nmr_hip_batch_last_run_ms(nmr_batch* b, double* ms, int64_t* voxel_steps)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_last_run_ms: null batch");
+    if (ms) *ms = b->last_ms;
+    if (voxel_steps) *voxel_steps = b->last_voxel_steps;
+    return NMR_OK;
+}
+
+int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_run: null batch");
+    if (nsteps < 0 || b->steps_done + nsteps > (int64_t)1 << 30) return fail(NMR_ERR_INVALID, "nsteps");
+    for (const auto& I : b->inst)
+        if (!I.uploaded) return fail(NMR_ERR_STATE, "run before every instance is uploaded");
+    int v = b->variant, l = b->lz;
+    {
+        int av, al;
+        batch_auto(b, &av, &al);
+        if (v < 0) v = av;
+        if (l <= 0) l = al;
+    }
+    int rc = batch_prepare(b, v, l);
+    if (rc) return rc;
+    const int nc = b->nc;
+    const int64_t rows = nsteps + 1;
+    const int S = b->graph_steps;
+    if ((rc = batch_partials(b, (size_t)std::max(S, 2)))) return rc;
+    const size_t need = (size_t)b->ninst * rows * nc;
+    if (need > b->signal_cap) {
+        if (b->d_signal) HIP_TRY(hipFree(b->d_signal));
+        b->d_signal = nullptr;
+        if ((rc = balloc(&b->d_signal, need))) return rc;
+        b->signal_cap = need;
+    }
+    // row 0: signal of every instance before the run
+    for (int i = 0; i < b->ninst; i++) {
+        const nmr_batch_instance& s = b->spec[i];
+        nmr::FieldSumArgs a{};
+        for (int q = 0; q < 3; q++) a.f[q] = b->inst[i].m[b->parity][q];
+        a.mask = b->inst[i].mask;
+        a.part = b->d_fpart;
+        a.nx = s.nx;
+        a.ny = s.ny;
+        a.nz = s.nz;
+        a.pitch = b->inst[i].pitch;
+        a.bx0 = b->prob.buffer[0]; a.bx1 = s.nx - b->prob.buffer[0];
+        a.by0 = b->prob.buffer[1]; a.by1 = s.ny - b->prob.buffer[1];
+        a.bz0 = b->prob.buffer[2]; a.bz1 = s.nz - b->prob.buffer[2];
+        HIP_TRY(nmr::launch_field_sum(nc, a, s.nz, b->stream));
+        HIP_TRY(nmr::launch_reduce_partials(b->d_fpart, 1, s.nz, nc, b->d_signal + (size_t)i * rows * nc, b->stream));
+    }
+    // the device step counter holds the global step (the gradient schedule
+    // reads it); signal rows count from this run's first step
+    const int step0 = (int)b->steps_done;
+    HIP_TRY(hipMemcpyAsync(b->d_step, &step0, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int64_t done = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    auto enqueue_chunk = [&](int n) -> int {
+        for (int s = 0; s < n; s++) {
+            int r = batch_launch_step(b, s, (b->parity + s) & 1, b->d_part + (size_t)s * b->ntiles * nc);
+            if (r) return r;
+        }
+        HIP_TRY(nmr::launch_batch_reduce(b->d_part, b->ntiles, nc, b->d_off, b->d_ids, b->d_step, step0, rows,
+                                         b->d_signal, n, b->ninst, b->stream));
+        HIP_TRY(nmr::launch_step_advance(b->d_step, n, b->stream));
+        return NMR_OK;
+    };
+    if (S > 0 && nsteps >= S) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int r = enqueue_chunk(S);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        HIP_TRY(e);
+        graph = g;
+        HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        while (nsteps - done >= S) {
+            HIP_TRY(hipGraphLaunch(exec, b->stream));
+            done += S; // S is even: the parity is unchanged
+        }
+    }
+    while (done < nsteps) {
+        const int n = (int)std::min<int64_t>(std::max(S, 2), nsteps - done);
+        if ((rc = enqueue_chunk(n))) return rc;
+        b->parity = (b->parity + n) & 1;
+        done += n;
+    }
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    if (signal)
+        HIP_TRY(hipMemcpyAsync(signal, b->d_signal, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    int64_t vox = 0;
+    for (const auto& s : b->spec) vox += (int64_t)s.nx * s.ny * s.nz;
+    b->last_voxel_steps = vox * nsteps;
+    b->steps_done += nsteps;
+    return NMR_OK;
+}
+
+} // extern "C"

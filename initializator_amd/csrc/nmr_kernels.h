@@ -35,6 +35,33 @@ struct SweepArgs {
     Coef c;
 };
 
+// One instance of a batch (device memory).
+struct BatchInst {
+    float* m[2][3];         // ping-pong magnetization buffers
+    const uint8_t* mask;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    int nx, ny, nz, pitch, pitch1;
+    int ntx, nty, ntz, lz;
+    int bx0, bx1, by0, by1, bz0, bz1;
+    int inert;
+    int nd, nD;             // PGSE lobe length and start of the second lobe (steps)
+    float bgz;              // dt*g*G*dz of the +G lobe, rounded once on the host
+                            // (the -G lobe negates it, which is exact)
+    Coef c;                 // coefficients without the gradient
+};
+
+struct BatchArgs {
+    const BatchInst* inst;
+    const int2* tiles;      // (instance, tile index within the instance)
+    int ntiles;
+    int parity;             // which ping-pong buffer holds the current state
+    int local_step;         // step within the captured chunk
+    const int* step_base;   // device step counter (first step of the chunk)
+    double* partials;       // [ntiles][nc] of this step
+};
+
 struct InertArgs {
     const uint8_t* mask;
     const float* vx;
@@ -84,6 +111,12 @@ int num_variants();
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);
 hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s);
 hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
+hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s);
+int batch_resident_blocks(int nc, bool vel, bool exact, int variant);
+hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
+                               const int* step_base, int row0, int64_t rows, double* signal, int nsteps, int ninst,
+                               hipStream_t s);
+hipError_t launch_step_advance(int* step_base, int n, hipStream_t s);
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
 hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);

@@ -215,7 +215,7 @@ __device__ __forceinline__ void accumulate(double (&acc)[NC], const f4 (&out)[NC
 
 // Workgroup sum of acc into part[blockIdx.x*NC + q] (fixed order).
 template <int NC, int NT>
-__device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[NC], double* part)
+__device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[NC], double* dst)
 {
 #pragma unroll
     for (int q = 0; q < NC; q++) {
@@ -234,12 +234,14 @@ __device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[N
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; w++) s += sred[w][tid];
-        part[(size_t)blockIdx.x * NC + tid] = s;
+        dst[tid] = s;
     }
 }
 
+// One tile of the one-step sweep: tile index t (x fastest, then y, then the
+// z-chunk).  Adds the signal of the tile's updated pore voxels into acc.
 template <int NC, bool VEL, bool EXACT, int LX, int RPT>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+__device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, double (&acc)[NC])
 {
     constexpr int NT = 256;
     constexpr int RY = NT / LX;  // thread rows
@@ -253,13 +255,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     // without bounds tests.
     __shared__ __attribute__((aligned(16))) float sm[2][NC][LH][LW];
     __shared__ __attribute__((aligned(16))) uint8_t sk[2][LH][LW];
-    __shared__ double sred[NT / 64][NC];
 
     const int tid = threadIdx.x;
     const int lx = tid % LX;
     const int ry = tid / LX;
 
-    const int t = xcd_tile(blockIdx.x, a.nblocks);
     const int tx = t % a.ntx;
     const int tyz = t / a.ntx;
     const int ty = tyz % a.nty;
@@ -423,10 +423,6 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     load_hm(k0, hkc, hc);
     load_xhalo(k0, xc, kxc);
 
-    double acc[NC];
-#pragma unroll
-    for (int q = 0; q < NC; q++) acc[q] = 0.0;
-
     const Coef cf = a.c;
     bool trim_x[4];
 #pragma unroll
@@ -548,8 +544,98 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
         hkc = hkn;
         hkn = hkq;
     }
+}
 
-    block_reduce<NC, NT>(acc, sred, a.partials);
+template <int NC, bool VEL, bool EXACT, int LX, int RPT>
+__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+{
+    __shared__ double sred[4][NC];
+    double acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) acc[q] = 0.0;
+    sweep_tile<NC, VEL, EXACT, LX, RPT>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
+}
+
+// Persistent batched sweep: one launch advances every instance of a batch by
+// one step.  The grid is sized to the resident workgroups; workgroup w takes
+// tiles w, w + gridDim.x, ... of a list sorted by cost, each tile naming its
+// instance.  Instance descriptors and the step counter live in device memory
+// so that a chunk of launches can be captured in a graph and replayed.
+template <int NC, bool VEL, bool EXACT, int LX, int RPT>
+__global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
+{
+    __shared__ double sred[4][NC];
+    const int step = *b.step_base + b.local_step;
+    for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) {
+        const int2 tl = b.tiles[t];
+        const BatchInst* I = b.inst + tl.x;
+        SweepArgs a;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a.min[q] = I->m[b.parity][q];
+            a.mout[q] = I->m[b.parity ^ 1][q];
+        }
+        a.mask = I->mask;
+        a.vx = I->vx;
+        a.vy = I->vy;
+        a.vz = I->vz;
+        a.partials = nullptr;
+        a.nx = I->nx;
+        a.ny = I->ny;
+        a.nz = I->nz;
+        a.pitch = I->pitch;
+        a.pitch1 = I->pitch1;
+        a.ntx = I->ntx;
+        a.nty = I->nty;
+        a.ntz = I->ntz;
+        a.lz = I->lz;
+        a.nblocks = 0;
+        a.bx0 = I->bx0; a.bx1 = I->bx1; a.by0 = I->by0; a.by1 = I->by1; a.bz0 = I->bz0; a.bz1 = I->bz1;
+        a.inert = I->inert;
+        a.c = I->c;
+        // pulsed gradient along z: +G on [0, nd), -G on [nD, nD + nd) (the
+        // second lobe wins where they overlap)
+        a.c.bgz = (step >= I->nD && step < I->nD + I->nd) ? -I->bgz : (step < I->nd ? I->bgz : 0.f);
+        double acc[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc[q] = 0.0;
+        sweep_tile<NC, VEL, EXACT, LX, RPT>(a, tl.y, acc);
+        block_reduce<NC, 256>(acc, sred, b.partials + (size_t)t * NC);
+        __syncthreads();
+    }
+}
+
+// Per-instance signal rows of a chunk of batched steps: block (s, i) sums the
+// partials of instance i's tiles (fixed order) into signal[i][row][q],
+// row = *step_base - row0 + s + 1 (row0: the step counter at the run's start).
+__global__ void __launch_bounds__(64) batch_reduce_kernel(const double* __restrict__ part, int ntiles, int nc,
+                                                          const int* __restrict__ off, const int* __restrict__ ids,
+                                                          const int* __restrict__ step_base, int row0, int64_t rows,
+                                                          double* __restrict__ signal)
+{
+    __shared__ double sh[64];
+    const int s = blockIdx.x, i = blockIdx.y;
+    const double* p = part + (size_t)s * ntiles * nc;
+    const int64_t row = (int64_t)(*step_base - row0) + s + 1;
+    for (int q = 0; q < nc; q++) {
+        double v = 0.0;
+        for (int k = off[i] + (int)threadIdx.x; k < off[i + 1]; k += 64) v += p[(size_t)ids[k] * nc + q];
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int h = 32; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) signal[((size_t)i * rows + row) * nc + q] = sh[0];
+        __syncthreads();
+    }
+}
+
+// Advances the device step counter after a chunk (one lane, vector store).
+__global__ void step_advance_kernel(int* step_base, int n)
+{
+    if (threadIdx.x == 0) step_base[0] += n;
 }
 
 // Two time steps per pass (temporal blocking).  The workgroup owns TY full-
@@ -808,9 +894,9 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
             pvz1 = vz1;
         }
     }
-    block_reduce<NC, NT>(acc1, sred, a.partials1);
+    block_reduce<NC, NT>(acc1, sred, a.partials1 + (size_t)blockIdx.x * NC);
     __syncthreads();
-    block_reduce<NC, NT>(acc2, sred, a.partials2);
+    block_reduce<NC, NT>(acc2, sred, a.partials2 + (size_t)blockIdx.x * NC);
 }
 
 // Sets *flag when some face between two in-domain cells carries velocity and
@@ -1044,6 +1130,77 @@ hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args&
     }
     if (nc == 3) return vel ? launch2_nc<3, true, false>(lx, a, s) : launch2_nc<3, false, false>(lx, a, s);
     return hipErrorInvalidValue;
+}
+
+template <int NC, bool VEL, bool EXACT>
+static hipError_t launchb_nc(int v, int grid, const BatchArgs& b, hipStream_t s)
+{
+    const dim3 g(grid), block(256);
+    switch (v) {
+    case 0: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 64, 1>), g, block, 0, s, b); break;
+    case 1: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 64, 2>), g, block, 0, s, b); break;
+    case 2: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 32, 1>), g, block, 0, s, b); break;
+    case 3: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 32, 2>), g, block, 0, s, b); break;
+    case 4: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 16, 1>), g, block, 0, s, b); break;
+    case 5: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 16, 2>), g, block, 0, s, b); break;
+    case 6: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 8, 1>), g, block, 0, s, b); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s)
+{
+    if (nc == 1) return vel ? launchb_nc<1, true, false>(variant, grid, b, s) : launchb_nc<1, false, false>(variant, grid, b, s);
+    if (nc == 2) {
+        if (exact) return vel ? launchb_nc<2, true, true>(variant, grid, b, s) : launchb_nc<2, false, true>(variant, grid, b, s);
+        return vel ? launchb_nc<2, true, false>(variant, grid, b, s) : launchb_nc<2, false, false>(variant, grid, b, s);
+    }
+    if (nc == 3) return vel ? launchb_nc<3, true, false>(variant, grid, b, s) : launchb_nc<3, false, false>(variant, grid, b, s);
+    return hipErrorInvalidValue;
+}
+
+int batch_resident_blocks(int nc, bool vel, bool exact, int variant)
+{
+    const void* f = nullptr;
+#define NMR_PICK(NC_, V_, E_)                                                                   \
+    switch (variant) {                                                                          \
+    case 0: f = (const void*)batch_kernel<NC_, V_, E_, 64, 1>; break;                           \
+    case 1: f = (const void*)batch_kernel<NC_, V_, E_, 64, 2>; break;                           \
+    case 2: f = (const void*)batch_kernel<NC_, V_, E_, 32, 1>; break;                           \
+    case 3: f = (const void*)batch_kernel<NC_, V_, E_, 32, 2>; break;                           \
+    case 4: f = (const void*)batch_kernel<NC_, V_, E_, 16, 1>; break;                           \
+    case 5: f = (const void*)batch_kernel<NC_, V_, E_, 16, 2>; break;                           \
+    case 6: f = (const void*)batch_kernel<NC_, V_, E_, 8, 1>; break;                            \
+    default: return -1;                                                                         \
+    }
+    if (nc == 1) { if (vel) { NMR_PICK(1, true, false) } else { NMR_PICK(1, false, false) } }
+    else if (nc == 2 && exact) { if (vel) { NMR_PICK(2, true, true) } else { NMR_PICK(2, false, true) } }
+    else if (nc == 2) { if (vel) { NMR_PICK(2, true, false) } else { NMR_PICK(2, false, false) } }
+    else if (nc == 3) { if (vel) { NMR_PICK(3, true, false) } else { NMR_PICK(3, false, false) } }
+    else return -1;
+#undef NMR_PICK
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) return -1;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    return per_cu * ncu;
+}
+
+hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
+                               const int* step_base, int row0, int64_t rows, double* signal, int nsteps, int ninst,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(nsteps, ninst), dim3(64), 0, s, part, ntiles, nc, off, ids,
+                       step_base, row0, rows, signal);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(int* step_base, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, step_base, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s)

@@ -161,3 +161,112 @@ def variants():
         check(lib.nmr_hip_variant_info(v, *[ctypes.byref(x) for x in f]), "nmr_hip_variant_info")
         out.append((v,) + tuple(x.value for x in f))
     return out
+
+
+class BatchSolver:
+    """Many independent problems of uneven size advanced together: one
+    persistent launch per step covers every instance, chunks of steps are
+    replayed from a captured HIP graph.  The instances share the physics
+    (ncomp, D, bulk relaxation, dx, dt, ...) and each has its own grid,
+    surface relaxivity and pulsed-gradient spin echo along z (+G for the
+    first round(delta/dt) steps, -G from step round(big_delta/dt) for as
+    many)."""
+
+    def __init__(self, instances, ncomp: int, dx: float, dt: float, diffusivity: float, r2_bulk: float,
+                 r1_bulk: float = 0.0, gamma: float = G_REF, p_eq: float = 0.0, diffusion: bool = True,
+                 velocity: bool = False, precession: bool = False, phase_exact: bool = False, buffer=(0, 0, 0)):
+        """instances: iterable of dicts with shape (nz, ny, nx), rho1, rho2,
+        grad_amp, delta, big_delta (missing keys default to 0)."""
+        self.lib = _lib.load()
+        inst = list(instances)
+        self.ninst = len(inst)
+        self.ncomp = int(ncomp)
+        self.shapes = [tuple(int(x) for x in d["shape"]) for d in inst]
+        arr = (_lib.NmrBatchInstanceC * max(1, self.ninst))()
+        for i, d in enumerate(inst):
+            nz, ny, nx = self.shapes[i]
+            arr[i].nx, arr[i].ny, arr[i].nz = nx, ny, nz
+            arr[i].rho1 = float(d.get("rho1", 0.0))
+            arr[i].rho2 = float(d.get("rho2", d.get("rho1", 0.0)))
+            arr[i].grad_amp = float(d.get("grad_amp", 0.0))
+            arr[i].delta = float(d.get("delta", 0.0))
+            arr[i].big_delta = float(d.get("big_delta", 0.0))
+        p = NmrProblemC()
+        p.nx, p.ny, p.nz = 1, 1, 1
+        p.ncomp = self.ncomp
+        p.flags = ((_lib.NMR_F_DIFFUSION if diffusion else 0) | (_lib.NMR_F_VELOCITY if velocity else 0)
+                   | (_lib.NMR_F_PRECESSION if precession else 0))
+        p.scheme = _lib.NMR_SCHEME_UPWIND
+        p.phase_mode = _lib.NMR_PHASE_EXACT if phase_exact else _lib.NMR_PHASE_LINEAR
+        for a in range(3):
+            p.buffer[a] = int(buffer[a])
+        p.b0_dir[2] = 1.0
+        p.dx = p.dy = p.dz = float(dx)
+        p.dt = float(dt)
+        p.diffusivity = float(diffusivity)
+        p.r1_bulk = float(r1_bulk)
+        p.r2_bulk = float(r2_bulk)
+        p.gamma = float(gamma)
+        p.p_eq = float(p_eq)
+        self.problem = p
+        self.velocity = velocity
+        h = ctypes.c_void_p()
+        check(self.lib.nmr_hip_batch_create(ctypes.byref(p), self.ninst, arr, ctypes.byref(h)),
+              "nmr_hip_batch_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            check(self.lib.nmr_hip_batch_destroy(self.h), "nmr_hip_batch_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, i: int, mask, mag, vel=None) -> None:
+        """mask (nz, ny, nx) 0/1; mag: list of ncomp arrays; vel: (vx, vy, vz)
+        face arrays or None."""
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        v = [None] * 3 if vel is None else [np.ascontiguousarray(a, dtype=np.float64) for a in vel]
+        g = [np.ascontiguousarray(a, dtype=np.float64) for a in mag] + [None] * (3 - len(mag))
+        check(self.lib.nmr_hip_batch_upload(self.h, int(i), m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                            *[_dptr(a) for a in v], *[_dptr(a) for a in g]),
+              "nmr_hip_batch_upload")
+
+    def download(self, i: int):
+        outs = [np.empty(self.shapes[i]) if q < self.ncomp else None for q in range(3)]
+        check(self.lib.nmr_hip_batch_download(self.h, int(i), *[_dptr(a) for a in outs]), "nmr_hip_batch_download")
+        return outs[: self.ncomp]
+
+    def set_tile(self, variant: int = -1, lz: int = 0, graph_steps: int = 64) -> None:
+        check(self.lib.nmr_hip_batch_set_tile(self.h, int(variant), int(lz), int(graph_steps)),
+              "nmr_hip_batch_set_tile")
+
+    def autotune(self, trial_steps: int = 5):
+        v = ctypes.c_int32()
+        lz = ctypes.c_int32()
+        check(self.lib.nmr_hip_batch_autotune(self.h, int(trial_steps), ctypes.byref(v), ctypes.byref(lz)),
+              "nmr_hip_batch_autotune")
+        return v.value, lz.value
+
+    def run(self, nsteps: int, want_signal: bool = True):
+        """Advance every instance nsteps; returns (ninst, nsteps+1, ncomp)."""
+        sig = np.empty((self.ninst, nsteps + 1, self.ncomp)) if want_signal else None
+        check(self.lib.nmr_hip_batch_run(self.h, int(nsteps), _dptr(sig)), "nmr_hip_batch_run")
+        return sig
+
+    def last_run_ms(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.lib.nmr_hip_batch_last_run_ms(self.h, ctypes.byref(ms), ctypes.byref(n)),
+              "nmr_hip_batch_last_run_ms")
+        return ms.value, n.value

@@ -53,6 +53,20 @@ def test_struct_layout_matches_header(lib, tmp_path):
     assert got == want
 
 
+def test_batch_instance_layout_matches_header(lib, tmp_path):
+    fields = [f[0] for f in _lib.NmrBatchInstanceC._fields_]
+    src = tmp_path / "layout_b.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"nmr_hip.h\"\nint main(void){\n"
+        + "".join(f'printf("%zu\\n", offsetof(nmr_batch_instance, {f}));\n' for f in fields)
+        + 'printf("%zu\\n", sizeof(nmr_batch_instance));return 0;}\n')
+    exe = tmp_path / "layout_b"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [getattr(_lib.NmrBatchInstanceC, f).offset for f in fields] + [ctypes.sizeof(_lib.NmrBatchInstanceC)]
+    assert got == want
+
+
 def _problem(**kw):
     p = _lib.NmrProblemC()
     p.nx = p.ny = p.nz = 8
@@ -120,3 +134,36 @@ def test_strerror_and_variants(lib):
         assert lib.nmr_hip_variant_info(v, None, None, None, None, ctypes.byref(st)) == _lib.NMR_OK
         steps.append(st.value)
     assert set(steps) == {1, 2}
+
+
+def _inst(n, **kw):
+    arr = (_lib.NmrBatchInstanceC * max(n, 1))()
+    for i in range(n):
+        arr[i].nx, arr[i].ny, arr[i].nz = 8, 9, 10
+        for k, v in kw.items():
+            setattr(arr[i], k, v)
+    return arr
+
+
+@pytest.mark.parametrize("pkw,ikw,n,code", [
+    (dict(), dict(), 0, _lib.NMR_ERR_INVALID),
+    (dict(), dict(), 70000, _lib.NMR_ERR_INVALID),
+    (dict(), dict(nx=0), 3, _lib.NMR_ERR_INVALID),
+    (dict(), dict(delta=-1.0), 2, _lib.NMR_ERR_INVALID),
+    (dict(scheme=3), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
+    (dict(ncomp=1, flags=_lib.NMR_F_PRECESSION), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
+    (dict(dt=-1.0), dict(), 2, _lib.NMR_ERR_INVALID),
+])
+def test_batch_create_rejects_bad_arguments(lib, pkw, ikw, n, code):
+    h = ctypes.c_void_p(77)
+    rc = lib.nmr_hip_batch_create(ctypes.byref(_problem(**pkw)), n, _inst(min(n, 4), **ikw), ctypes.byref(h))
+    assert rc == code, lib.nmr_hip_last_error().decode()
+    assert h.value is None
+
+
+def test_batch_null_arguments(lib):
+    assert lib.nmr_hip_batch_create(None, 1, None, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_batch_destroy(None) == _lib.NMR_OK
+    assert lib.nmr_hip_batch_run(None, 1, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_batch_upload(None, 0, None, None, None, None, None, None, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_batch_autotune(None, 1, None, None) == _lib.NMR_ERR_NULL
