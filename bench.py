@@ -79,6 +79,116 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def _batch_instance(i: int):
+    """Instance i of the batched workload: cube edge U{32..128}, porosity
+    U(0.15, 0.40), rho U(1, 50) um/s, b log-uniform on [1e7, 1e10] s/m^2 with
+    delta 2 ms, Delta 20 ms, divergence-free velocity (max 1 mm/s)."""
+    from initializator_amd import generators
+
+    seed = 0x4E4D52 + 4000 + i
+    r = np.random.default_rng(seed)
+    n = int(r.integers(32, 129))
+    phi = float(r.uniform(0.15, 0.40))
+    rho = float(r.uniform(1e-6, 50e-6))
+    b = float(10.0 ** r.uniform(7.0, 10.0))
+    shape = (n, n, n)
+    mask = generators.porosity(shape, phi, seed)
+    vel = generators.velocity(mask, seed, vmax=1e-3, d=(1e-6, 1e-6, 1e-6))
+    g = generators.gradient_for_b(b, 2e-3, 20e-3)
+    return dict(shape=shape, phi=phi, rho1=rho, rho2=rho, grad_amp=g, delta=2e-3, big_delta=20e-3,
+                mask=mask, vel=vel)
+
+
+def batch_main(args) -> int:
+    """Batched workload: many uneven instances advanced together."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from initializator_amd import generators
+    from initializator_amd.solver import BatchSolver
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ProcessPoolExecutor(workers) as ex:  # before any GPU call in this process
+        specs = list(ex.map(_batch_instance, range(args.instances)))
+    print(f"[bench] generated {len(specs)} instances", file=sys.stderr, flush=True)
+    dx, D = 1e-6, 2.3e-9
+    dt = generators.stable_dt(dx, D)
+    b = BatchSolver(specs, 2, dx, dt, D, 1.0 / 2.0, r1_bulk=1.0 / 2.5e10, velocity=True, precession=True)
+    for i, sp in enumerate(specs):
+        b.upload(i, sp["mask"], [sp["mask"].astype(np.float64), np.zeros(sp["shape"])], sp["vel"])
+    if args.variant >= 0:
+        b.set_tile(args.variant, args.lz, args.graph_steps, not args.full_grid)
+        v, lz, per = args.variant, args.lz, not args.full_grid
+    else:
+        b.set_tile(-1, 0, args.graph_steps)
+        v, lz, per = b.autotune(3)
+        b.set_tile(v, lz, args.graph_steps, per)
+    b.run(args.warmup, want_signal=False)
+    t0 = time.perf_counter()
+    b.run(args.steps, want_signal=True)
+    wall = time.perf_counter() - t0
+    gpu_ms, vox_steps = b.last_run_ms()
+    b.close()
+    nvox = sum(int(np.prod(sp["shape"])) for sp in specs)
+    # the same instances solved one after another, one context each (the
+    # unbatched path), on a shorter run: GPU time of their runs only
+    loop = None
+    if args.single_steps > 0:
+        from initializator_amd.solver import HipSolver
+
+        loop_ms = 0.0
+        for sp in specs:
+            h = HipSolver(sp["shape"], 2, dx, dt, D, 1.0 / 2.0, r1_bulk=1.0 / 2.5e10, rho1=sp["rho1"],
+                          rho2=sp["rho2"], velocity=True, precession=True)
+            h.upload_mask(sp["mask"])
+            h.upload_velocity(*sp["vel"])
+            h.upload_magnetization(sp["mask"].astype(np.float64), np.zeros(sp["shape"]))
+            sched = generators.pgse_schedule(args.single_steps, dt, sp["grad_amp"], 2e-3, 20e-3)
+            h.run(args.single_steps, sched)
+            loop_ms += h.last_run_ms()[0]
+            h.close()
+        loop = {"value": nvox * args.single_steps / (loop_ms * 1e-3), "unit": "voxel-steps/s",
+                "steps": args.single_steps, "gpu_ms": loop_ms}
+    kernel_s = gpu_ms * 1e-3 / args.steps
+    achieved = BYTES_PER_VOXEL_STEP * nvox / kernel_s / 1e9
+    out = {
+        "metric": "voxel-steps/s (batched uneven instances)",
+        "value": nvox * args.steps / wall,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded porosity, divergence-free velocity, PGSE per instance)",
+        "config": {
+            "workload": f"batched FP32 upwind Bloch-Torrey steps, {len(specs)} cubes of edge 32..128, "
+                        "phi U(0.15,0.4), rho U(1,50) um/s, b log-U(1e7,1e10) s/m^2, complex (Mx,My)",
+            "voxels": nvox,
+            "variant": v,
+            "lz": lz,
+            "persistent_grid": per,
+            "graph_steps": args.graph_steps,
+            "bytes_per_voxel_step": BYTES_PER_VOXEL_STEP,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel_us": kernel_s * 1e6,
+        },
+        "cpu_baseline": None,
+        "single_instance_loop": loop,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +200,14 @@ def main() -> int:
     ap.add_argument("--lz", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("plain", "batch"), default="plain",
+                    help="plain: one 256^3 grid (default); batch: many uneven instances per launch")
+    ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
+    ap.add_argument("--graph-steps", type=int, default=64, help="batch workload: steps per captured graph")
+    ap.add_argument("--full-grid", action="store_true",
+                    help="batch workload with --variant: one workgroup per tile instead of a resident grid")
+    ap.add_argument("--single-steps", type=int, default=200,
+                    help="batch workload: steps of the one-instance-at-a-time comparison (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="time a plain device-to-device copy of MIB MiB instead (HBM ceiling / counter scale)")
     args = ap.parse_args()
@@ -99,6 +217,8 @@ def main() -> int:
         print("bench.py: this build measures one GPU; the solve keeps the whole domain on one MI355X "
               "(multi-GPU z-slab sharding is a design note, not built)", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return batch_main(args)
 
     from initializator_amd import generators
     from initializator_amd.solver import HipSolver

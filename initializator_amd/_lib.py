@@ -167,8 +167,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_batch_upload": (c_int, [vp, c_int32, POINTER(c_uint8), dp, dp, dp, dp, dp, dp]),
         "nmr_hip_batch_download": (c_int, [vp, c_int32, dp, dp, dp]),
         "nmr_hip_batch_run": (c_int, [vp, c_int64, dp]),
-        "nmr_hip_batch_set_tile": (c_int, [vp, c_int32, c_int32, c_int32]),
-        "nmr_hip_batch_autotune": (c_int, [vp, c_int32, POINTER(c_int32), POINTER(c_int32)]),
+        "nmr_hip_batch_set_tile": (c_int, [vp, c_int32, c_int32, c_int32, c_int32]),
+        "nmr_hip_batch_autotune": (c_int, [vp, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
         "nmr_hip_batch_last_run_ms": (c_int, [vp, dp, POINTER(c_int64)]),
         "nmr_hip_strerror": (c_char_p, [c_int]),
         "nmr_hip_last_error": (c_char_p, []),

@@ -803,11 +803,11 @@ struct nmr_batch {
     std::vector<Inst> inst;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int variant = -1, lz = 0, graph_steps = 64;
+    int variant = -1, lz = 0, graph_steps = 64, persistent = 1;
     int parity = 0;
     int64_t steps_done = 0;
-    // launch state for the current variant / lz
-    int cur_variant = -2, cur_lz = -1, grid = 0, ntiles = 0;
+    // launch state for the current variant / lz / grid kind
+    int cur_variant = -2, cur_lz = -1, cur_persistent = -1, grid = 0, ntiles = 0;
     nmr::BatchInst* d_inst = nullptr;
     int2* d_tiles = nullptr;
     int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
@@ -824,7 +824,8 @@ struct nmr_batch {
 namespace {
 
 std::mutex g_tune_mu;
-std::map<std::vector<int>, std::pair<int, int>> g_tune_cache; // shapes -> (variant, lz)
+struct Tuned { int variant, lz, persistent; };
+std::map<std::vector<int>, Tuned> g_tune_cache; // shapes -> fastest launch
 
 void batch_free(nmr_batch* b)
 {
@@ -853,10 +854,10 @@ int balloc(T** p, size_t count)
     return NMR_OK;
 }
 
-// Tile list, descriptors and partials for (variant, lz).
-int batch_prepare(nmr_batch* b, int variant, int lz)
+// Tile list, descriptors and grid for (variant, lz, grid kind).
+int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
 {
-    if (variant == b->cur_variant && lz == b->cur_lz) return NMR_OK;
+    if (variant == b->cur_variant && lz == b->cur_lz && persistent == b->cur_persistent) return NMR_OK;
     int lx, rpt, tx, ty, st;
     if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1)
         return fail(NMR_ERR_INVALID, "batch variant %d is not a one-step tile", variant);
@@ -937,11 +938,16 @@ int batch_prepare(nmr_batch* b, int variant, int lz)
     HIP_TRY(hipMemcpy(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     b->ntiles = (int)tiles.size();
-    const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, variant);
-    if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
-    b->grid = std::min(res, b->ntiles);
+    if (persistent) {
+        const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, variant);
+        if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
+        b->grid = std::min(res, b->ntiles);
+    } else {
+        b->grid = b->ntiles;
+    }
     b->cur_variant = variant;
     b->cur_lz = lz;
+    b->cur_persistent = persistent;
     return NMR_OK;
 }
 
@@ -1174,20 +1180,22 @@ int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, doub
     return NMR_OK;
 }
 
-int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps)
+int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps, int32_t persistent)
 {
     if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_set_tile: null batch");
     int lx, rpt, tx, ty, st;
     if (variant != -1 && (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
         return fail(NMR_ERR_INVALID, "batch variant %d", (int)variant);
-    if (lz < 0 || graph_steps < 0 || (graph_steps & 1)) return fail(NMR_ERR_INVALID, "lz %d / graph_steps %d", lz, graph_steps);
+    if (lz < 0 || graph_steps < 0 || (graph_steps & 1) || (persistent != 0 && persistent != 1))
+        return fail(NMR_ERR_INVALID, "lz %d / graph_steps %d / persistent %d", lz, graph_steps, persistent);
     b->variant = variant;
     b->lz = lz;
     b->graph_steps = graph_steps;
+    b->persistent = persistent;
     return NMR_OK;
 }
 
-int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, int32_t* lz)
+int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, int32_t* lz, int32_t* persistent)
 {
     if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_autotune: null batch");
     for (const auto& I : b->inst)
@@ -1197,22 +1205,25 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            b->variant = it->second.first;
-            b->lz = it->second.second;
+            b->variant = it->second.variant;
+            b->lz = it->second.lz;
+            b->persistent = it->second.persistent;
             if (variant) *variant = b->variant;
             if (lz) *lz = b->lz;
+            if (persistent) *persistent = b->persistent;
             return NMR_OK;
         }
     }
     const int trials = std::max(1, (int)trial_steps);
     double best = 1e300;
-    int bv = 0, bl = 8;
+    int bv = 0, bl = 8, bp = 1;
     for (int v = 0; v < nmr::num_variants(); v++) {
         int lx, rpt, tx, ty, st;
         nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st);
         if (st != 1) continue;
-        for (int l : {4, 8, 16}) {
-            int rc = batch_prepare(b, v, l);
+        for (int cfg = 0; cfg < 10; cfg++) {
+            const int l = 4 << (cfg >> 1), per = cfg & 1; // lz 4..64, grid kind
+            int rc = batch_prepare(b, v, l, per);
             if (rc) return rc;
             if ((rc = batch_partials(b, 1))) return rc;
             // warm-up, then timed: every trial reads the current buffer and
@@ -1229,17 +1240,20 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
                 best = ms;
                 bv = v;
                 bl = l;
+                bp = per;
             }
         }
     }
     b->variant = bv;
     b->lz = bl;
+    b->persistent = bp;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tune_cache[key] = {bv, bl};
+        g_tune_cache[key] = {bv, bl, bp};
     }
     if (variant) *variant = bv;
     if (lz) *lz = bl;
+    if (persistent) *persistent = bp;
     return NMR_OK;
 }
 
@@ -1264,7 +1278,7 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
         if (v < 0) v = av;
         if (l <= 0) l = al;
     }
-    int rc = batch_prepare(b, v, l);
+    int rc = batch_prepare(b, v, l, b->persistent);
     if (rc) return rc;
     const int nc = b->nc;
     const int64_t rows = nsteps + 1;

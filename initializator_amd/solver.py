@@ -247,16 +247,18 @@ class BatchSolver:
         check(self.lib.nmr_hip_batch_download(self.h, int(i), *[_dptr(a) for a in outs]), "nmr_hip_batch_download")
         return outs[: self.ncomp]
 
-    def set_tile(self, variant: int = -1, lz: int = 0, graph_steps: int = 64) -> None:
-        check(self.lib.nmr_hip_batch_set_tile(self.h, int(variant), int(lz), int(graph_steps)),
+    def set_tile(self, variant: int = -1, lz: int = 0, graph_steps: int = 64, persistent: bool = True) -> None:
+        check(self.lib.nmr_hip_batch_set_tile(self.h, int(variant), int(lz), int(graph_steps), int(persistent)),
               "nmr_hip_batch_set_tile")
 
     def autotune(self, trial_steps: int = 5):
+        """Returns (variant, lz, persistent) of the fastest launch."""
         v = ctypes.c_int32()
         lz = ctypes.c_int32()
-        check(self.lib.nmr_hip_batch_autotune(self.h, int(trial_steps), ctypes.byref(v), ctypes.byref(lz)),
-              "nmr_hip_batch_autotune")
-        return v.value, lz.value
+        per = ctypes.c_int32()
+        check(self.lib.nmr_hip_batch_autotune(self.h, int(trial_steps), ctypes.byref(v), ctypes.byref(lz),
+                                              ctypes.byref(per)), "nmr_hip_batch_autotune")
+        return v.value, lz.value, bool(per.value)
 
     def run(self, nsteps: int, want_signal: bool = True):
         """Advance every instance nsteps; returns (ninst, nsteps+1, ncomp)."""

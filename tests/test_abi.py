@@ -166,4 +166,5 @@ def test_batch_null_arguments(lib):
     assert lib.nmr_hip_batch_destroy(None) == _lib.NMR_OK
     assert lib.nmr_hip_batch_run(None, 1, None) == _lib.NMR_ERR_NULL
     assert lib.nmr_hip_batch_upload(None, 0, None, None, None, None, None, None, None) == _lib.NMR_ERR_NULL
-    assert lib.nmr_hip_batch_autotune(None, 1, None, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_batch_autotune(None, 1, None, None, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_batch_set_tile(None, 0, 0, 0, 1) == _lib.NMR_ERR_NULL

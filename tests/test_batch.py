@@ -59,9 +59,9 @@ def _schedule(s, nsteps, start=0):
     return g
 
 
-def _batch(specs, graph_steps=16, variant=-1, lz=0):
+def _batch(specs, graph_steps=16, variant=-1, lz=0, persistent=True):
     b = BatchSolver(specs, 2, DX, DT, D, 1.0 / T2, r1_bulk=1.0 / T1, velocity=True, precession=True)
-    b.set_tile(variant, lz, graph_steps)
+    b.set_tile(variant, lz, graph_steps, persistent)
     for i, s in enumerate(specs):
         b.upload(i, s["mask"], s["m0"], s["vel"])
     return b
@@ -135,19 +135,19 @@ def test_split_runs_equal_one_run():
 def test_autotune_keeps_state_and_tile_invariance():
     specs = _instances()
     tuned = _batch(specs)
-    v, lz = tuned.autotune(3)
-    assert 0 <= v <= 6 and lz in (4, 8, 16)
+    v, lz, per = tuned.autotune(3)
+    assert 0 <= v <= 6 and lz in (4, 8, 16, 32, 64)
     for i, s in enumerate(specs):
         f = tuned.download(i)
         for q in range(2):
             assert np.array_equal(f[q], s["m0"][q].astype(np.float32).astype(np.float64))
-    plain = _batch(specs, variant=0, lz=5)
+    plain = _batch(specs, variant=0, lz=5, persistent=not per)
     st, sp = tuned.run(60), plain.run(60)
     for i in range(len(specs)):
         for q in range(2):
             assert np.array_equal(tuned.download(i)[q], plain.download(i)[q])
     np.testing.assert_allclose(st, sp, rtol=1e-11, atol=1e-14)
-    print(f"[batch] autotuned variant {v}, lz {lz}", flush=True)
+    print(f"[batch] autotuned variant {v}, lz {lz}, persistent {per}", flush=True)
     tuned.close()
     plain.close()
 
@@ -177,6 +177,8 @@ def test_batch_state_errors():
         b.run(2)
     with pytest.raises(Exception):
         b.set_tile(0, 0, 3)  # odd graph chunk
+    with pytest.raises(Exception):
+        b.set_tile(0, 0, 2, 5)  # grid kind
     with pytest.raises(Exception):
         b.set_tile(8, 0, 0)  # a two-step variant
     b.close()
