@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 METRIC = "voxel-steps/s & %HBM roofline @256³; learned vs plain time-to-signal @1e-3"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + Vx,Vy,Vz (12)
+# DRAM bytes per sweep launch of the default workload (256^3, phi 0.20,
+# automatic tile): rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per
+# dispatch, scaled by what the same counters read for a plain 1 GiB copy
+# (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
+DEFAULT_TRAFFIC = 326.9e6 + 50.4e6
 
 
 def cpu_baseline(case, target_s: float):
@@ -264,7 +269,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (DEFAULT_TRAFFIC if (args.n, args.phi, args.variant, args.lz) == (256, 0.20, -1, 0)
+                        else None),
             "kernel_us": kernel_s * 1e6,
         },
         "cpu_baseline": None,
