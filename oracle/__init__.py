@@ -39,7 +39,13 @@ class OrParams(ctypes.Structure):
         ("gamma", c_double),
         ("p_eq", c_double),
         ("b0", c_double * 3),
+        ("scheme", c_int32),
+        ("bc", c_int32 * 3),
     ]
+
+
+UPWIND, MACCORMACK = 0, 1
+BC_NEUMANN, BC_PERIODIC, BC_DIRICHLET = 0, 1, 2
 
 
 _lib = None
@@ -69,6 +75,7 @@ def lib():
         L.or_add_impulse.argtypes = [P, cp, c_int, c_double, c_double, c_double, c_double, dp, dp, dp]
         L.or_run.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, c_int64, dp, dp, c_int, c_int, c_int,
                              dp, dp, dp, dp]
+        L.or_maccormack.argtypes = [P, cp] + [dp] * 18
         _lib = L
     return _lib
 
@@ -86,9 +93,12 @@ def _c(a):
 
 
 def params(shape, dx, dt, *, dy=None, dz=None, precession=False, velocity=False, diffusion=True,
-           gamma=G_REF, p_eq=0.0, b0=(0.0, 0.0, 1.0)) -> OrParams:
+           gamma=G_REF, p_eq=0.0, b0=(0.0, 0.0, 1.0), scheme=UPWIND, bc=(0, 0, 0)) -> OrParams:
     nz, ny, nx = shape
     p = OrParams()
+    p.scheme = int(scheme)
+    for i in range(3):
+        p.bc[i] = int(bc[i])
     p.ik, p.jk, p.kk = nx, ny, nz
     p.precession, p.velocity, p.diffusion = int(precession), int(velocity), int(diffusion)
     p.dx = dx
@@ -144,6 +154,13 @@ def explicit_simple(p, act, D, t1, t2, vel, nf, m, a):
                              *[_d(x) for x in nf], *[_d(x) for x in m], *[_d(x) for x in a])
 
 
+def maccormack(p, act, D, t1, t2, vel, nf, m, pc):
+    """One MacCormack step in place on m; pc: six zero-initialised padded
+    scratch arrays (predictor, corrector) kept across steps."""
+    lib().or_maccormack(ctypes.byref(p), _c(act), _d(D), _d(t1), _d(t2), *[_d(v) for v in vel],
+                        *[_d(x) for x in nf], *[_d(x) for x in m], *[_d(x) for x in pc])
+
+
 def add_impulse(p, act, flag, param, axis, m):
     """AddImpulse in place on m (list of 3 padded arrays)."""
     lib().or_add_impulse(ctypes.byref(p), _c(act), int(flag), float(param), float(axis[0]), float(axis[1]),
@@ -167,14 +184,14 @@ def run(p, act, D, t1, t2, vel, nsteps, m, grad_sched=None, grad_const=None, buf
     return sig
 
 
-def setup_case(case, p_eq=0.0, m0=None):
+def setup_case(case, p_eq=0.0, m0=None, scheme=UPWIND, bc=(0, 0, 0)):
     """Padded oracle inputs for a generators.Case (one fluid, uniform
     relaxivity): params, mask, D, T1, T2, velocity, initial M."""
     from initializator_amd import layout  # conversions only
 
     has_v = case.vx is not None
     prec = case.grad_sched is not None
-    p = params(case.shape, case.dx, case.dt, precession=prec, velocity=has_v, p_eq=p_eq)
+    p = params(case.shape, case.dx, case.dt, precession=prec, velocity=has_v, p_eq=p_eq, scheme=scheme, bc=bc)
     act = layout.pad_mask(case.mask)
     D, t1, t2, _ = fill_coefficients(p, act, case.diffusivity, case.t1_bulk, case.t2_bulk, case.rho, case.rho)
     if has_v:
@@ -188,12 +205,12 @@ def setup_case(case, p_eq=0.0, m0=None):
     return p, act, D, t1, t2, vel, m
 
 
-def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0)):
+def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0), scheme=UPWIND, bc=(0, 0, 0)):
     """Oracle run of a Case: (signal (nsteps+1, 3), compact final fields)."""
     from initializator_amd import layout
 
     n = case.nsteps if nsteps is None else nsteps
-    p, act, D, t1, t2, vel, m = setup_case(case, p_eq=p_eq, m0=m0)
+    p, act, D, t1, t2, vel, m = setup_case(case, p_eq=p_eq, m0=m0, scheme=scheme, bc=bc)
     gs = None if case.grad_sched is None else case.grad_sched[:n]
     sig = run(p, act, D, t1, t2, vel, n, m, grad_sched=gs, buf=buf)
     return sig, [layout.unpad(x) for x in m]

@@ -8,6 +8,10 @@
  * side:  idx = i + (ik+2)*(j + (jk+2)*k)   (NmrRelaxationSolve.cpp:3284-3287).
  * Masks are the reference's ASCII '0'/'1' chars.
  *
+ * Schemes: or_params.scheme 0 is ExplicitSimple (upwind, the host path),
+ * 1 is the MacCormack step of the GPU path (ExplicitMakUpgrade); bc[] selects
+ * the ghost fill per axis (zero derivative, periodic, zero value).
+ *
  * Parity status of each function (how it is pinned without a reference
  * build -- the reference's host path needs cuda_runtime.h and MPI, which
  * this image does not provide, so it cannot be compiled here):
@@ -15,7 +19,16 @@
  *                        surface rate, Neumann cosine eigenmode, 1-D upwind
  *                        brute force (both signs), precession (1-i*theta)^n,
  *                        conservation / fixed-point invariants
- *   or_bc_field          pinned: Neumann invariants + brute copy check
+ *   or_maccormack        pinned: Heun factors of the relaxation-only,
+ *                        precession-only and diffusion-only (Neumann cosine
+ *                        and zero-value sine eigenmodes) problems; the
+ *                        Lax-Wendroff amplification factor of periodic 1-D
+ *                        advection; uniform fixed point under flow next to
+ *                        solids of arbitrary value (the race resolution);
+ *                        solids zeroed after a step
+ *   or_bc_field          pinned: Neumann invariants + brute copy check;
+ *                        periodic advection (upwind amplification factor);
+ *                        zero-value diffusion sine eigenmode
  *   or_bc_mask           pinned: brute copy check
  *   or_fill_coefficients pinned: hand-computed rates on a 3^3 pore in 5^3
  *   or_fill_gradient     pinned: closed form G*d*(idx-1/2)
@@ -42,7 +55,14 @@ typedef struct or_params {
     double gamma;            /* g = 4.78941714e7 in the reference */
     double p_eq;             /* equilibriumMagnetization */
     double b0[3];            /* uniform mainField */
+    int32_t scheme;          /* 0: ExplicitSimple (upwind), 1: MacCormack (ExplicitMakUpgrade) */
+    int32_t bc[3];           /* per axis: 0 zero derivative, 1 periodic, 2 zero value */
 } or_params;
+
+#define OR_BC_NEUMANN 0
+#define OR_BC_PERIODIC 1
+#define OR_BC_DIRICHLET 2
+#define OR_MAK_EPS 1.0e-25 /* harmonic-mean guard of NmrRelaxationSolveKernels.cu:726-731 */
 
 static inline long IX(const or_params* p, int i, int j, int k)
 {
@@ -76,25 +96,40 @@ void or_bc_mask(const or_params* p, char* act)
         }
 }
 
-/* Zero-derivative ghost fill of one field component
- * (NmrRelaxationSolve.cpp:1664-1754 at one rank; same order: x faces over
- * all j,k, then y faces over all i,k, then the z faces). */
+/* Ghost fill of one field component, axis by axis in the order x, y, z
+ * over the full extent of the other two axes.  Per axis (bc[]):
+ *   zero derivative: ghost = adjacent interior cell
+ *       (BC_ZeroDer_X/Y/Z, NmrRelaxationSolveKernels.cu:969-1000; the host
+ *       path NmrRelaxationSolve.cpp:1664-1754 at one rank);
+ *   periodic: ghost 0 = cell n, ghost n+1 = cell 1 (BC_Inf_X/Y/Z, cu:1002-1033);
+ *   zero value: both ghosts 0 (BC_Advanced_XM/XP with Val = 0, cu:1035-1053,
+ *       which the reference offers on x only). */
 void or_bc_field(const or_params* p, double* f)
 {
-    for (int k = 0; k < p->kk + 2; k++)
-        for (int j = 0; j < p->jk + 2; j++) {
-            f[IX(p, 0, j, k)] = f[IX(p, 1, j, k)];
-            f[IX(p, p->ik + 1, j, k)] = f[IX(p, p->ik, j, k)];
+    const int ik = p->ik, jk = p->jk, kk = p->kk;
+    for (int k = 0; k < kk + 2; k++)
+        for (int j = 0; j < jk + 2; j++) {
+            double lo = f[IX(p, 1, j, k)], hi = f[IX(p, ik, j, k)];
+            if (p->bc[0] == OR_BC_PERIODIC) { const double t = lo; lo = hi; hi = t; }
+            if (p->bc[0] == OR_BC_DIRICHLET) lo = hi = 0.0;
+            f[IX(p, 0, j, k)] = lo;
+            f[IX(p, ik + 1, j, k)] = hi;
         }
-    for (int k = 0; k < p->kk + 2; k++)
-        for (int i = 0; i < p->ik + 2; i++) {
-            f[IX(p, i, 0, k)] = f[IX(p, i, 1, k)];
-            f[IX(p, i, p->jk + 1, k)] = f[IX(p, i, p->jk, k)];
+    for (int k = 0; k < kk + 2; k++)
+        for (int i = 0; i < ik + 2; i++) {
+            double lo = f[IX(p, i, 1, k)], hi = f[IX(p, i, jk, k)];
+            if (p->bc[1] == OR_BC_PERIODIC) { const double t = lo; lo = hi; hi = t; }
+            if (p->bc[1] == OR_BC_DIRICHLET) lo = hi = 0.0;
+            f[IX(p, i, 0, k)] = lo;
+            f[IX(p, i, jk + 1, k)] = hi;
         }
-    for (int j = 0; j < p->jk + 2; j++)
-        for (int i = 0; i < p->ik + 2; i++) {
-            f[IX(p, i, j, 0)] = f[IX(p, i, j, 1)];
-            f[IX(p, i, j, p->kk + 1)] = f[IX(p, i, j, p->kk)];
+    for (int j = 0; j < jk + 2; j++)
+        for (int i = 0; i < ik + 2; i++) {
+            double lo = f[IX(p, i, j, 1)], hi = f[IX(p, i, j, kk)];
+            if (p->bc[2] == OR_BC_PERIODIC) { const double t = lo; lo = hi; hi = t; }
+            if (p->bc[2] == OR_BC_DIRICHLET) lo = hi = 0.0;
+            f[IX(p, i, j, 0)] = lo;
+            f[IX(p, i, j, kk + 1)] = hi;
         }
 }
 
@@ -151,8 +186,11 @@ void or_fill_coefficients(const or_params* p, const char* act, double d_water,
                 if (tmin >= t1[c]) tmin = t1[c];
                 if (tmin >= t2[c]) tmin = t2[c];
             }
-    /* diffusion ghosts: x faces, y faces, then z faces */
-    or_bc_field(p, D);
+    /* diffusion ghosts copy the adjacent interior whatever the field's
+     * boundary condition (BoundaryConditionsForDiffusion, cpp:1895-1942) */
+    or_params pn = *p;
+    pn.bc[0] = pn.bc[1] = pn.bc[2] = OR_BC_NEUMANN;
+    or_bc_field(&pn, D);
     if (out2) {
         out2[0] = dmax;
         out2[1] = tmin;
@@ -284,6 +322,115 @@ void or_explicit_simple(const or_params* p, const char* act, const double* D,
             }
 }
 
+/* Relaxation toward pEq*n (T1 along n, T2 across) and the linearised
+ * precession dt*g*(m x Bg) of one cell: the increments both schemes add
+ * (NmrRelaxationSolve.cpp:3206-3235; NmrRelaxationSolveKernels.cu:701-733). */
+static void or_point_terms(const or_params* p, double tau1, double tau2, const double bg[3], const double m0[3],
+                           double inc[3])
+{
+    const double dt = p->dt;
+    const double nb = sqrt(p->b0[0] * p->b0[0] + p->b0[1] * p->b0[1] + p->b0[2] * p->b0[2]);
+    const double n[3] = {p->b0[0] / nb, p->b0[1] / nb, p->b0[2] / nb};
+    const double par = n[0] * m0[0] + n[1] * m0[1] + n[2] * m0[2];
+    for (int q = 0; q < 3; q++) {
+        const double mpar = n[q] * par;
+        inc[q] = dt * tau1 * (p->p_eq * n[q] - mpar) + dt * tau2 * (mpar - m0[q]);
+    }
+    if (p->precession) {
+        inc[0] += 0.0 - bg[1] * m0[2] + bg[2] * m0[1];
+        inc[1] += 0.0 + bg[0] * m0[2] - bg[2] * m0[0];
+        inc[2] += 0.0 - bg[0] * m0[1] + bg[1] * m0[0];
+    }
+}
+
+/* One stage of the MacCormack step for every pore (NMR_predictor_upgrade,
+ * NmrRelaxationSolveKernels.cu:637-801, and NMR_corrector_upgrade,
+ * cu:803-967).  The stage reads field f (M for the predictor, the predictor
+ * P for the corrector), adds relaxation, precession, harmonic-mean
+ * diffusion and advection with the cell-centred velocity (u = mean of the
+ * two x faces, ...): backward differences in the predictor, forward in the
+ * corrector.  out = f + increments (predictor) or
+ * 0.5*(M + P + increments(P)) (corrector).
+ *
+ * Race resolution: before differencing, the reference kernels copy the
+ * pore's own value into every solid neighbour (cu:765-782, 931-948), in
+ * place and without synchronisation, so the value a pore reads from a solid
+ * neighbour shared with other pores is whichever thread wrote last.  The
+ * value each writer intends is the reading pore's own, so here every read
+ * of a solid neighbour (advection and diffusion) returns the pore's own
+ * value.  Ghost cells carry the mask of the adjacent interior cell, so a
+ * pore on the domain edge reads its ghost as the boundary condition left it. */
+static void or_mak_stage(const or_params* p, int corrector, const char* act, const double* D, const double* t1,
+                         const double* t2, const double* vx, const double* vy, const double* vz,
+                         const double* nfx, const double* nfy, const double* nfz, double* const m[3],
+                         double* const f[3], double* const out[3])
+{
+    const double dt = p->dt;
+    const double hx = dt / p->dx, hy = dt / p->dy, hz = dt / p->dz;
+    const double hxx = dt / (p->dx * p->dx), hyy = dt / (p->dy * p->dy), hzz = dt / (p->dz * p->dz);
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k < p->kk + 1; k++)
+        for (int j = 1; j < p->jk + 1; j++)
+            for (int i = 1; i < p->ik + 1; i++) {
+                const long c = IX(p, i, j, k);
+                if (act[c] != '1') continue;
+                const long nbr[6] = {IX(p, i + 1, j, k), IX(p, i - 1, j, k), IX(p, i, j + 1, k),
+                                     IX(p, i, j - 1, k), IX(p, i, j, k + 1), IX(p, i, j, k - 1)};
+                const double dc = D[c];
+                double g[6];
+                for (int s = 0; s < 6; s++) g[s] = 2.0 * (D[nbr[s]] * dc) / (D[nbr[s]] + dc + OR_MAK_EPS);
+                const double u = 0.5 * (vx[c] + vx[nbr[0]]), v = 0.5 * (vy[c] + vy[nbr[2]]),
+                             w = 0.5 * (vz[c] + vz[nbr[4]]);
+                const double b = dt * p->gamma;
+                const double bg[3] = {b * nfx[c], b * nfy[c], b * nfz[c]};
+                const double f0[3] = {f[0][c], f[1][c], f[2][c]};
+                double inc[3];
+                or_point_terms(p, 1.0 / t1[c], 1.0 / t2[c], bg, f0, inc);
+                for (int q = 0; q < 3; q++) {
+                    double nv[6];
+                    for (int s = 0; s < 6; s++) nv[s] = act[nbr[s]] == '0' ? f0[q] : f[q][nbr[s]];
+                    double diff = 0.0, vel = 0.0;
+                    if (p->diffusion)
+                        diff = hxx * (g[0] * (nv[0] - f0[q]) - g[1] * (f0[q] - nv[1])) +
+                               hyy * (g[2] * (nv[2] - f0[q]) - g[3] * (f0[q] - nv[3])) +
+                               hzz * (g[4] * (nv[4] - f0[q]) - g[5] * (f0[q] - nv[5]));
+                    if (p->velocity) {
+                        if (!corrector)
+                            vel = 0.0 - u * hx * (f0[q] - nv[1]) - v * hy * (f0[q] - nv[3]) - w * hz * (f0[q] - nv[5]);
+                        else
+                            vel = 0.0 - u * hx * (nv[0] - f0[q]) - v * hy * (nv[2] - f0[q]) - w * hz * (nv[4] - f0[q]);
+                    }
+                    if (!corrector)
+                        out[q][c] = f0[q] + inc[q] + diff + vel;
+                    else
+                        out[q][c] = 0.5 * (m[q][c] + f0[q] + inc[q] + diff + vel);
+                }
+            }
+}
+
+/* One MacCormack step (ExplicitMakUpgrade, NmrRelaxationSolveKernels.cu:1473-1501):
+ * predictor into P, ghost fill of P, corrector into C, ghost fill of C, then
+ * M = C over the whole padded array.  C is only ever written at pores and at
+ * ghosts, so solid cells of M become C's solid value; the reference's C starts
+ * zeroed (cu:57-62) and so does the caller's. */
+void or_maccormack(const or_params* p, const char* act, const double* D, const double* t1, const double* t2,
+                   const double* vx, const double* vy, const double* vz, const double* nfx, const double* nfy,
+                   const double* nfz, double* mx, double* my, double* mz, double* px, double* py, double* pz,
+                   double* cx, double* cy, double* cz)
+{
+    double* const m[3] = {mx, my, mz};
+    double* const P[3] = {px, py, pz};
+    double* const C[3] = {cx, cy, cz};
+    or_mak_stage(p, 0, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, m, m, P);
+    for (int q = 0; q < 3; q++) or_bc_field(p, P[q]);
+    or_mak_stage(p, 1, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, m, P, C);
+    const long num = or_num_cells(p);
+    for (int q = 0; q < 3; q++) {
+        or_bc_field(p, C[q]);
+        memcpy(m[q], C[q], sizeof(double) * num);
+    }
+}
+
 /* Signal: sum of one component over pore cells of the buffer-trimmed
  * interior, one fluid (NmrRelaxationSolve.cpp:3301-3322).  Returns the
  * sum; *count receives the number of cells summed. */
@@ -325,6 +472,9 @@ void or_run(const or_params* p, const char* act, const double* D, const double* 
     double* nfx = (double*)malloc(sizeof(double) * num);
     double* nfy = (double*)malloc(sizeof(double) * num);
     double* nfz = (double*)malloc(sizeof(double) * num);
+    double* pc[6] = {NULL, NULL, NULL, NULL, NULL, NULL}; /* MacCormack P and C, zeroed */
+    if (p->scheme == 1)
+        for (int q = 0; q < 6; q++) pc[q] = (double*)calloc((size_t)num, sizeof(double));
     memcpy(ax, mx, sizeof(double) * num);
     memcpy(ay, my, sizeof(double) * num);
     memcpy(az, mz, sizeof(double) * num);
@@ -336,6 +486,8 @@ void or_run(const or_params* p, const char* act, const double* D, const double* 
     }
     or_fill_gradient(p, g_cur[0], g_cur[1], g_cur[2], nfx, nfy, nfz);
     double* ms[3] = {mx, my, mz};
+    /* ghosts of the initial state: the boundary condition of the initial field */
+    for (int q = 0; q < 3; q++) or_bc_field(p, ms[q]);
     if (signal)
         for (int q = 0; q < 3; q++) signal[q] = or_output(p, act, bx, by, bz, ms[q], NULL);
     for (int64_t n = 0; n < nsteps; n++) {
@@ -348,7 +500,11 @@ void or_run(const or_params* p, const char* act, const double* D, const double* 
                 or_fill_gradient(p, g_cur[0], g_cur[1], g_cur[2], nfx, nfy, nfz);
             }
         }
-        or_explicit_simple(p, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, mx, my, mz, ax, ay, az);
+        if (p->scheme == 1)
+            or_maccormack(p, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, mx, my, mz, pc[0], pc[1], pc[2], pc[3], pc[4],
+                          pc[5]);
+        else
+            or_explicit_simple(p, act, D, t1, t2, vx, vy, vz, nfx, nfy, nfz, mx, my, mz, ax, ay, az);
         or_bc_field(p, mx);
         or_bc_field(p, my);
         or_bc_field(p, mz);
@@ -358,6 +514,7 @@ void or_run(const or_params* p, const char* act, const double* D, const double* 
     }
     free(ax); free(ay); free(az);
     free(nfx); free(nfy); free(nfz);
+    for (int q = 0; q < 6; q++) free(pc[q]);
 }
 
 /* Rotation of one cell's M by `angle` about (nx, ny, nz)
