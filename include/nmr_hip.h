@@ -64,6 +64,17 @@ extern "C" {
 
 /* ---- update scheme ---- */
 #define NMR_SCHEME_UPWIND 0      /* ExplicitSimple, cpp:3148-3282 */
+#define NMR_SCHEME_MACCORMACK 1  /* ExplicitMakUpgrade, cu:637-967, 1473-1501; a
+                                    solid neighbour reads as the pore's own value
+                                    (the reference's racy in-place ghost fill,
+                                    cu:765-782, 931-948, resolved race-free);
+                                    solid voxels end each step at 0 */
+
+/* ---- boundary condition per axis (BoundaryConditionsForVectorField_gpu,
+ * cu:1321-1396; NB1/NB3 = x, NB2/NB4 = y, NB5/NB6 = z) ---- */
+#define NMR_BC_NEUMANN 0         /* ghost = adjacent interior (BC_ZeroDer, NB = 1) */
+#define NMR_BC_PERIODIC 1        /* ghost = opposite interior (BC_Inf, NB = 3) */
+#define NMR_BC_DIRICHLET 2       /* ghost = 0 (BC_Advanced with Val 0, NB = 2) */
 
 /* ---- gradient precession form ---- */
 #define NMR_PHASE_LINEAR 0       /* M += dt*g*(M x Bg), as cpp:3230-3235 */
@@ -76,6 +87,9 @@ typedef struct nmr_problem {
     int32_t scheme;         /* NMR_SCHEME_* */
     int32_t phase_mode;     /* NMR_PHASE_* */
     int32_t buffer[3];      /* signal trim per side (bufferSizeX/Y/Z) */
+    int32_t bc[3];          /* NMR_BC_* per axis x, y, z; ghost cells count as
+                               pores for diffusion and surface relaxation, as
+                               the reference's mask ghosts copy the interior */
     int32_t reserved;
     double dx, dy, dz;      /* voxel size (m) */
     double dt;              /* time step (s) */
@@ -138,8 +152,10 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out);
  * the kernels of the last nmr_hip_run, and the number of sweep launches. */
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
-/* Tile shape of the sweep kernel: variant index (-1 = automatic) and z-chunk
- * length (0 = automatic).  nmr_hip_num_variants() gives the variant count;
+/* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = the
+ * per-voxel kernel that MacCormack and non-zero-derivative boundaries use,
+ * which gives the same fields as the sweep) and z-chunk length
+ * (0 = automatic).  nmr_hip_num_variants() gives the variant count;
  * nmr_hip_variant_info() describes one: lanes per row, rows per thread, tile
  * width and height, and time steps per pass (2 = temporally blocked sweep,
  * usable when nx <= tile_x).  Every variant gives bit-identical fields. */

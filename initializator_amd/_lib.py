@@ -29,6 +29,11 @@ NMR_LAYOUT_COMPACT = 0
 NMR_LAYOUT_PADDED = 1
 
 NMR_SCHEME_UPWIND = 0
+NMR_SCHEME_MACCORMACK = 1
+
+NMR_BC_NEUMANN = 0
+NMR_BC_PERIODIC = 1
+NMR_BC_DIRICHLET = 2
 
 NMR_PHASE_LINEAR = 0
 NMR_PHASE_EXACT = 1
@@ -78,6 +83,7 @@ class NmrProblemC(ctypes.Structure):
         ("scheme", c_int32),
         ("phase_mode", c_int32),
         ("buffer", c_int32 * 3),
+        ("bc", c_int32 * 3),
         ("reserved", c_int32),
         ("dx", c_double),
         ("dy", c_double),

@@ -62,6 +62,8 @@ struct nmr_ctx {
     size_t n_m = 0;  // floats per magnetization component (nz*ny*pitch)
     size_t n_vx = 0, n_vy = 0, n_vz = 0;
     float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    float* pred[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor
+    bool general = false; // MacCormack or a non-zero-derivative boundary: per-voxel kernel
     uint8_t* mask = nullptr;
     float* vx = nullptr;
     float* vy = nullptr;
@@ -105,6 +107,8 @@ void free_all(nmr_ctx* c)
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < 3; q++)
             if (c->m[b][q]) (void)hipFree(c->m[b][q]);
+    for (int q = 0; q < 3; q++)
+        if (c->pred[q]) (void)hipFree(c->pred[q]);
     if (c->mask) (void)hipFree(c->mask);
     if (c->vx) (void)hipFree(c->vx);
     if (c->vy) (void)hipFree(c->vy);
@@ -265,6 +269,61 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
     return NMR_OK;
 }
 
+// Steps of the per-voxel kernel (MacCormack and/or non-zero-derivative
+// boundaries), each followed by the signal of the new state.
+int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
+{
+    const nmr_problem& p = ctx->prob;
+    const int nc = ctx->nc;
+    const int64_t rows = nsteps + 1;
+    int rc = ensure_signal(ctx, (size_t)rows * nc);
+    if (rc) return rc;
+    if ((rc = ensure_partials(ctx, (size_t)p.nz * nc))) return rc;
+    nmr::GenArgs a{};
+    a.mask = ctx->mask;
+    a.vx = ctx->vx;
+    a.vy = ctx->vy;
+    a.vz = ctx->vz;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.pitch = ctx->pitch;
+    a.pitch1 = ctx->pitch1;
+    for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
+    a.vel = ctx->vel ? 1 : 0;
+    const bool mak = p.scheme == NMR_SCHEME_MACCORMACK;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    for (int64_t s = 0; s < nsteps; s++) {
+        double g[3] = {p.grad[0], p.grad[1], p.grad[2]};
+        if (grad_sched)
+            for (int q = 0; q < 3; q++) g[q] = grad_sched[3 * s + q];
+        a.c = make_coef(ctx, g);
+        for (int q = 0; q < 3; q++) {
+            a.m[q] = ctx->m[ctx->cur][q];
+            a.p[q] = ctx->pred[q];
+            a.out[q] = mak ? ctx->pred[q] : ctx->m[ctx->cur ^ 1][q];
+        }
+        HIP_TRY(nmr::launch_general(nc, mak ? nmr::GEN_PREDICT : nmr::GEN_UPWIND, a, ctx->stream));
+        if (mak) {
+            for (int q = 0; q < 3; q++) a.out[q] = ctx->m[ctx->cur ^ 1][q];
+            HIP_TRY(nmr::launch_general(nc, nmr::GEN_CORRECT, a, ctx->stream));
+        }
+        ctx->cur ^= 1;
+        if ((rc = enqueue_field_sum(ctx, (size_t)s + 1))) return rc;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    if (signal)
+        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * nc, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    ctx->last_sweeps = nsteps * (mak ? 2 : 1);
+    return NMR_OK;
+}
+
 // Whether solid magnetization can reach a pore (a face touching a solid
 // carries velocity); recomputed on the device after mask/velocity uploads.
 int refresh_inert(nmr_ctx* c)
@@ -299,7 +358,7 @@ int refresh_inert(nmr_ctx* c)
 
 extern "C" {
 
-int nmr_hip_abi_version(void) { return 1; }
+int nmr_hip_abi_version(void) { return 2; }
 
 const char* nmr_hip_strerror(int code)
 {
@@ -342,7 +401,15 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     if ((long)p.nx * (long)p.ny * (long)p.nz > (1L << 31) - 1 || p.nx > (1 << 20) || p.ny > (1 << 20))
         return fail(NMR_ERR_INVALID, "grid %d x %d x %d exceeds 2^31 voxels", p.nx, p.ny, p.nz);
     if (p.ncomp < 1 || p.ncomp > 3) return fail(NMR_ERR_INVALID, "ncomp %d", p.ncomp);
-    if (p.scheme != NMR_SCHEME_UPWIND) return fail(NMR_ERR_UNSUPPORTED, "scheme %d", p.scheme);
+    if (p.scheme != NMR_SCHEME_UPWIND && p.scheme != NMR_SCHEME_MACCORMACK)
+        return fail(NMR_ERR_UNSUPPORTED, "scheme %d", p.scheme);
+    bool general = p.scheme == NMR_SCHEME_MACCORMACK;
+    for (int a = 0; a < 3; a++) {
+        if (p.bc[a] < NMR_BC_NEUMANN || p.bc[a] > NMR_BC_DIRICHLET) return fail(NMR_ERR_INVALID, "bc[%d] = %d", a, p.bc[a]);
+        general = general || p.bc[a] != NMR_BC_NEUMANN;
+    }
+    if (general && p.phase_mode == NMR_PHASE_EXACT)
+        return fail(NMR_ERR_UNSUPPORTED, "exact phase needs the upwind scheme with zero-derivative boundaries");
     if (p.phase_mode != NMR_PHASE_LINEAR && p.phase_mode != NMR_PHASE_EXACT)
         return fail(NMR_ERR_INVALID, "phase_mode %d", p.phase_mode);
     if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION))
@@ -365,6 +432,7 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     nmr_ctx* c = new (std::nothrow) nmr_ctx();
     if (!c) return fail(NMR_ERR_NOMEM, "host allocation");
     c->prob = p;
+    c->general = general;
     c->nc = p.ncomp;
     c->vel = (p.flags & NMR_F_VELOCITY) != 0;
     c->exact = p.phase_mode == NMR_PHASE_EXACT;
@@ -392,6 +460,9 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < c->nc; q++)
             if ((rc = dev_alloc(c, &c->m[b][q], c->n_m))) return bail(rc);
+    if (p.scheme == NMR_SCHEME_MACCORMACK)
+        for (int q = 0; q < c->nc; q++)
+            if ((rc = dev_alloc(c, &c->pred[q], c->n_m))) return bail(rc);
     if ((rc = dev_alloc(c, &c->mask, c->n_m))) return bail(rc);
     if ((rc = dev_alloc(c, &c->dflag, 64))) return bail(rc);
     if (c->vel) {
@@ -556,14 +627,23 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
                         dst[q][(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)] =
                             h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i];
         } else {
-            // interior plus the zero-derivative ghost copy (clamped indices)
+            // interior plus the ghost layer each axis's boundary condition
+            // leaves (copy of the adjacent cell, opposite cell, or 0)
+            auto ghost = [](int i, int n, int bc, bool* zero) {
+                if (i >= 1 && i <= n) return i - 1;
+                if (bc == NMR_BC_DIRICHLET) *zero = true;
+                if (bc == NMR_BC_PERIODIC) return i == 0 ? n - 1 : 0;
+                return i == 0 ? 0 : n - 1;
+            };
             for (int k = 0; k < p.nz + 2; k++)
                 for (int j = 0; j < p.ny + 2; j++)
                     for (int i = 0; i < p.nx + 2; i++) {
-                        const int ii = std::min(std::max(i - 1, 0), p.nx - 1);
-                        const int jj = std::min(std::max(j - 1, 0), p.ny - 1);
-                        const int kk = std::min(std::max(k - 1, 0), p.nz - 1);
-                        dst[q][pidx(p, i, j, k)] = h[(size_t)kk * p.ny * ctx->pitch + (size_t)jj * ctx->pitch + ii];
+                        bool zero = false;
+                        const int ii = ghost(i, p.nx, p.bc[0], &zero);
+                        const int jj = ghost(j, p.ny, p.bc[1], &zero);
+                        const int kk = ghost(k, p.nz, p.bc[2], &zero);
+                        dst[q][pidx(p, i, j, k)] =
+                            zero ? 0.0 : h[(size_t)kk * p.ny * ctx->pitch + (size_t)jj * ctx->pitch + ii];
                     }
         }
     }
@@ -573,7 +653,8 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
 int nmr_hip_set_tile(nmr_ctx* ctx, int32_t variant, int32_t lz)
 {
     if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_set_tile: null context");
-    if (variant < -1 || variant >= nmr::num_variants()) return fail(NMR_ERR_INVALID, "variant %d", (int)variant);
+    if (variant < -2 || variant >= nmr::num_variants()) return fail(NMR_ERR_INVALID, "variant %d", (int)variant);
+    if (variant == -2 && ctx->exact) return fail(NMR_ERR_UNSUPPORTED, "the per-voxel kernel has no exact phase");
     if (lz < 0) return fail(NMR_ERR_INVALID, "lz %d", (int)lz);
     ctx->variant = variant;
     ctx->lz = lz;
@@ -665,6 +746,7 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                 return fail(NMR_ERR_UNSUPPORTED, "precession needs ncomp >= 2");
         }
     }
+    if (ctx->general || ctx->variant == -2) return run_general(ctx, nsteps, grad_sched, signal);
     int rc0 = refresh_inert(ctx);
     if (rc0) return rc0;
     Launch L1{}, L2{};
@@ -1007,7 +1089,9 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
     if (ninst < 1 || ninst > 65535) return fail(NMR_ERR_INVALID, "ninst %d (1..65535)", (int)ninst);
     nmr_problem p = *common;
     if (p.ncomp < 1 || p.ncomp > 3) return fail(NMR_ERR_INVALID, "ncomp %d", p.ncomp);
-    if (p.scheme != NMR_SCHEME_UPWIND) return fail(NMR_ERR_UNSUPPORTED, "scheme %d", p.scheme);
+    if (p.scheme != NMR_SCHEME_UPWIND) return fail(NMR_ERR_UNSUPPORTED, "batched solves use the upwind scheme");
+    for (int a = 0; a < 3; a++)
+        if (p.bc[a] != NMR_BC_NEUMANN) return fail(NMR_ERR_UNSUPPORTED, "batched solves use zero-derivative boundaries");
     if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION)) return fail(NMR_ERR_INVALID, "flags");
     if (!(p.dx > 0) || !(p.dy > 0) || !(p.dz > 0) || !(p.dt > 0)) return fail(NMR_ERR_INVALID, "dx, dy, dz, dt");
     if (p.ncomp < 3 && (p.b0_dir[0] != 0.0 || p.b0_dir[1] != 0.0 || p.b0_dir[2] == 0.0))

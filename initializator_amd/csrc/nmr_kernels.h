@@ -89,6 +89,24 @@ struct Sweep2Args {
     Coef c1, c2;            // coefficients of the two steps
 };
 
+// Per-voxel step for the schemes and boundary conditions the z-sweep does
+// not cover (MacCormack, periodic and zero-value ghosts).
+enum { GEN_UPWIND = 0, GEN_PREDICT = 1, GEN_CORRECT = 2 };
+enum { BC_NEUMANN = 0, BC_PERIODIC = 1, BC_DIRICHLET = 2 };
+struct GenArgs {
+    const float* m[3];      // state at the start of the step
+    const float* p[3];      // predictor (corrector stage)
+    float* out[3];          // result of the stage
+    const uint8_t* mask;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    int nx, ny, nz, pitch, pitch1;
+    int bc[3];              // BC_* per axis
+    int vel;                // face velocities present
+    Coef c;
+};
+
 struct PulseArgs {
     float* m[3];
     const uint8_t* mask;
@@ -120,6 +138,7 @@ hipError_t launch_step_advance(int* step_base, int n, hipStream_t s);
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
 hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
+hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
 hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s);
 
 } // namespace nmr

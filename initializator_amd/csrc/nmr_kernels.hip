@@ -188,6 +188,147 @@ __device__ __forceinline__ void update_chunk(const Coef& cf, const f4 (&mc)[NC],
     }
 }
 
+// One voxel of a step for the schemes and boundary conditions outside the
+// z-sweep: the upwind step of ExplicitSimple (cpp:3148-3282, the same
+// face-weight arithmetic as update_chunk, so that with zero-derivative ghosts
+// the result is bit-identical to the sweep), or a stage of the MacCormack step
+// (NMR_predictor_upgrade / NMR_corrector_upgrade, cu:637-967): advection with
+// the cell-centred velocity u = (um + up)/2, backward differences in the
+// predictor and forward differences in the corrector, a solid neighbour
+// reading as the voxel's own value.  Neighbours outside the domain follow the
+// axis's boundary condition (own value, opposite edge, or 0) and count as
+// pores, as the reference's ghost mask copies the adjacent interior.
+// Threads: 64 x-voxels by 4 rows per workgroup, one plane per blockIdx.z.
+template <int NC, int ST>
+__global__ void __launch_bounds__(256) general_kernel(GenArgs a)
+{
+    const int x = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (x >= nx || y >= ny) return;
+    const size_t P = (size_t)a.pitch;
+    const size_t plane = (size_t)ny * P;
+    const size_t c = (size_t)z * plane + (size_t)y * P + (size_t)x;
+    if (a.mask[c] == 0) {
+        if constexpr (ST == GEN_CORRECT) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) a.out[q][c] = 0.f; // the reference's corrector holds 0 at solids
+        }
+        return; // upwind: solids keep their value (both buffers hold it)
+    }
+    const Coef& cf = a.c;
+    // neighbour cells: index, kind (0 load with its mask, 1 own value, 2 zero,
+    // 3 load as a pore), order x-, x+, y-, y+, z-, z+
+    size_t ni[6];
+    int kind[6];
+    {
+        const int lo[3] = {x, y, z}, n[3] = {nx, ny, nz};
+        const size_t st[3] = {1, P, plane};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++) {
+                const int s = 2 * ax + sd;
+                const bool out = sd == 0 ? lo[ax] == 0 : lo[ax] == n[ax] - 1;
+                if (!out) {
+                    ni[s] = sd == 0 ? c - st[ax] : c + st[ax];
+                    kind[s] = 0;
+                } else if (a.bc[ax] == BC_PERIODIC) {
+                    ni[s] = sd == 0 ? c + (size_t)(n[ax] - 1) * st[ax] : c - (size_t)(n[ax] - 1) * st[ax];
+                    kind[s] = 3;
+                } else {
+                    ni[s] = c;
+                    kind[s] = a.bc[ax] == BC_DIRICHLET ? 2 : 1;
+                }
+            }
+        }
+    }
+    float kn[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) kn[s] = kind[s] == 0 ? (a.mask[ni[s]] != 0 ? 1.f : 0.f) : 1.f;
+
+    // face weights on (f_neighbour - f_centre)
+    float w[6] = {cf.cdx * kn[0], cf.cdx * kn[1], cf.cdy * kn[2], cf.cdy * kn[3], cf.cdz * kn[4], cf.cdz * kn[5]};
+    if (a.vel) {
+        const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
+        const size_t P1 = (size_t)a.pitch1;
+        const size_t cx = (size_t)z * ny * P1 + (size_t)y * P1 + (size_t)x;
+        const size_t cy = ((size_t)z * (ny + 1) + (size_t)y) * P + (size_t)x;
+        const float um = a.vx[cx], up = a.vx[cx + 1];
+        const float vm = a.vy[cy], vp = a.vy[cy + P];
+        const float wm = a.vz[c], wp = a.vz[c + plane];
+        const float su = um + up, sv = vm + vp, sw = wm + wp;
+        if constexpr (ST == GEN_UPWIND) {
+            w[0] = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
+            w[1] = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
+            w[2] = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
+            w[3] = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
+            w[4] = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
+            w[5] = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
+        } else if constexpr (ST == GEN_PREDICT) { // + u hx (f_{i-1} - f_i)
+            w[0] = kn[0] * fmaf(hxh, su, cf.cdx);
+            w[2] = kn[2] * fmaf(hyh, sv, cf.cdy);
+            w[4] = kn[4] * fmaf(hzh, sw, cf.cdz);
+        } else { // - u hx (f_{i+1} - f_i)
+            w[1] = kn[1] * fmaf(-hxh, su, cf.cdx);
+            w[3] = kn[3] * fmaf(-hyh, sv, cf.cdy);
+            w[5] = kn[5] * fmaf(-hzh, sw, cf.cdz);
+        }
+    }
+    const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
+    const float rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+    const float* const* src = (ST == GEN_CORRECT) ? a.p : a.m;
+    float m0[NC], d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        const float m = src[q][c];
+        m0[q] = m;
+        float f[6];
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            if (kind[s] == 1) f[s] = m;
+            else if (kind[s] == 2) f[s] = 0.f;
+            else if (ST != GEN_UPWIND && kn[s] == 0.f) f[s] = m; // solid reads as own value
+            else f[s] = src[q][ni[s]];
+        }
+        float t = w[0] * (f[0] - m);
+        t = fmaf(w[1], f[1] - m, t);
+        t = fmaf(w[2], f[2] - m, t);
+        t = fmaf(w[3], f[3] - m, t);
+        t = fmaf(w[4], f[4] - m, t);
+        d[q] = fmaf(w[5], f[5] - m, t);
+    }
+    const float thy = cf.bgy * ((float)y + 0.5f), thz = cf.bgz * ((float)z + 0.5f);
+    if constexpr (NC == 3) {
+        const float rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
+        const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
+        const float nn[3] = {cf.n0, cf.n1, cf.n2};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float mpar = nn[q] * par;
+            d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
+            d[q] = fmaf(rate2, mpar - m0[q], d[q]);
+        }
+        const float thx = cf.bgx * ((float)x + 0.5f);
+        d[0] = fmaf(thz, m0[1], fmaf(-thy, m0[2], d[0]));
+        d[1] = fmaf(-thz, m0[0], fmaf(thx, m0[2], d[1]));
+        d[2] = fmaf(thy, m0[0], fmaf(-thx, m0[1], d[2]));
+    } else {
+#pragma unroll
+        for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
+        if constexpr (NC == 2) {
+            d[0] = fmaf(thz, m0[1], d[0]);
+            d[1] = fmaf(-thz, m0[0], d[1]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        if constexpr (ST == GEN_CORRECT) a.out[q][c] = 0.5f * (a.m[q][c] + (m0[q] + d[q]));
+        else a.out[q][c] = m0[q] + d[q];
+    }
+}
+
 // Per-plane scalars of the exact precession rotation.
 template <bool EXACT>
 __device__ __forceinline__ void plane_rotation(float thz, float& sn, float& cm1)
@@ -1207,6 +1348,24 @@ hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int n
 {
     if (nsteps <= 0) return hipSuccess;
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(nsteps), dim3(256), 0, s, part, nwg, nc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s)
+{
+    const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz), block(256);
+#define NMR_GEN(NC_)                                                                                 \
+    switch (stage) {                                                                                 \
+    case GEN_UPWIND: hipLaunchKernelGGL((general_kernel<NC_, GEN_UPWIND>), grid, block, 0, s, a); break; \
+    case GEN_PREDICT: hipLaunchKernelGGL((general_kernel<NC_, GEN_PREDICT>), grid, block, 0, s, a); break; \
+    case GEN_CORRECT: hipLaunchKernelGGL((general_kernel<NC_, GEN_CORRECT>), grid, block, 0, s, a); break; \
+    default: return hipErrorInvalidValue;                                                            \
+    }
+    if (nc == 1) { NMR_GEN(1) }
+    else if (nc == 2) { NMR_GEN(2) }
+    else if (nc == 3) { NMR_GEN(3) }
+    else return hipErrorInvalidValue;
+#undef NMR_GEN
     return hipGetLastError();
 }
 

@@ -28,7 +28,8 @@ class HipSolver:
                  r1_bulk: float = 0.0, rho1: float = 0.0, rho2: float = 0.0, gamma: float = G_REF,
                  p_eq: float = 0.0, b0_dir=(0.0, 0.0, 1.0), grad=(0.0, 0.0, 0.0), diffusion: bool = True,
                  velocity: bool = False, precession: bool = False, phase_exact: bool = False,
-                 buffer=(0, 0, 0), dy: float | None = None, dz: float | None = None):
+                 buffer=(0, 0, 0), dy: float | None = None, dz: float | None = None,
+                 scheme: int = _lib.NMR_SCHEME_UPWIND, bc=(0, 0, 0)):
         self.lib = _lib.load()
         nz, ny, nx = (int(s) for s in shape)
         self.shape = (nz, ny, nx)
@@ -38,10 +39,11 @@ class HipSolver:
         p.ncomp = self.ncomp
         p.flags = ((_lib.NMR_F_DIFFUSION if diffusion else 0) | (_lib.NMR_F_VELOCITY if velocity else 0)
                    | (_lib.NMR_F_PRECESSION if precession else 0))
-        p.scheme = _lib.NMR_SCHEME_UPWIND
+        p.scheme = int(scheme)
         p.phase_mode = _lib.NMR_PHASE_EXACT if phase_exact else _lib.NMR_PHASE_LINEAR
         for a in range(3):
             p.buffer[a] = int(buffer[a])
+            p.bc[a] = int(bc[a])
             p.b0_dir[a] = float(b0_dir[a])
             p.grad[a] = float(grad[a])
         p.dx = float(dx)

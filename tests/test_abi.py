@@ -82,6 +82,9 @@ def _problem(**kw):
         elif k == "grad":
             for i in range(3):
                 p.grad[i] = v[i]
+        elif k == "bc":
+            for i in range(3):
+                p.bc[i] = v[i]
         else:
             setattr(p, k, v)
     return p
@@ -101,6 +104,9 @@ def _problem(**kw):
     (dict(grad=(0.1, 0.0, 0.0)), _lib.NMR_ERR_UNSUPPORTED),
     (dict(phase_mode=_lib.NMR_PHASE_EXACT, ncomp=3), _lib.NMR_ERR_UNSUPPORTED),
     (dict(nx=4096, ny=4096, nz=4096), _lib.NMR_ERR_INVALID),
+    (dict(bc=(0, 3, 0)), _lib.NMR_ERR_INVALID),
+    (dict(scheme=_lib.NMR_SCHEME_MACCORMACK, phase_mode=_lib.NMR_PHASE_EXACT), _lib.NMR_ERR_UNSUPPORTED),
+    (dict(bc=(1, 0, 0), phase_mode=_lib.NMR_PHASE_EXACT), _lib.NMR_ERR_UNSUPPORTED),
 ])
 def test_create_rejects_bad_problems(lib, kw, code):
     ctx = ctypes.c_void_p(123)
@@ -121,7 +127,7 @@ def test_null_arguments(lib):
 def test_strerror_and_variants(lib):
     for code in range(-6, 1):
         assert lib.nmr_hip_strerror(code)
-    assert lib.nmr_hip_abi_version() == 1
+    assert lib.nmr_hip_abi_version() == 2
     n = lib.nmr_hip_num_variants()
     assert n >= 4
     a = ctypes.c_int32()
@@ -153,6 +159,8 @@ def _inst(n, **kw):
     (dict(scheme=3), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
     (dict(ncomp=1, flags=_lib.NMR_F_PRECESSION), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
     (dict(dt=-1.0), dict(), 2, _lib.NMR_ERR_INVALID),
+    (dict(scheme=_lib.NMR_SCHEME_MACCORMACK), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
+    (dict(bc=(1, 0, 0)), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
 ])
 def test_batch_create_rejects_bad_arguments(lib, pkw, ikw, n, code):
     h = ctypes.c_void_p(77)
