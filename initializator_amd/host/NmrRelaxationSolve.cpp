@@ -177,16 +177,38 @@ void NmrRelaxationSolve::ReadStart()
     flag1 = flag2 = flag3 = flag4 = flag5 = 0;
 }
 
+// Ghost fill per axis from the NB flags, with the rules of
+// BoundaryConditionsForVectorField_gpu (cu:1328-1396): x zero derivative when
+// NB1 = NB3 = 1, periodic when both are 3, zero value when either is 2 (that
+// rule runs last); y and z zero derivative or periodic when both flags agree.
+// Other combinations leave the reference's ghosts stale: -1.
+static void boundary_modes(const int nb[6], int out[3])
+{
+    out[0] = -1;
+    if (nb[0] == 1 && nb[2] == 1) out[0] = NMR_BC_NEUMANN;
+    if (nb[0] == 3 && nb[2] == 3) out[0] = NMR_BC_PERIODIC;
+    if (nb[0] == 2 || nb[2] == 2) out[0] = NMR_BC_DIRICHLET;
+    const int pairs[2][2] = {{1, 3}, {4, 5}};
+    for (int a = 0; a < 2; a++) {
+        const int lo = nb[pairs[a][0]], hi = nb[pairs[a][1]];
+        out[a + 1] = (lo == 1 && hi == 1) ? NMR_BC_NEUMANN : ((lo == 3 && hi == 3) ? NMR_BC_PERIODIC : -1);
+    }
+}
+
 // Inputs the GPU path implements (one fluid, uniform relaxivity, no random
-// noise field, zero-derivative boundaries).
+// noise field, boundary flags that define every ghost).
 void NmrRelaxationSolve::Validate() const
 {
     if (nComponents != 1) unsupported("NCOMP=2 (two-phase mixing) is not implemented by the GPU solver");
     if (nFluid != 1 && nFluid != 2) unsupported("NFLUID must be 1 or 2");
     if (nSurfRelWater != 0 || nSurfRelOil != 0) unsupported("face-wise surface relaxivity files are not implemented");
     if (noiseAmpl != 0.0 && nDirectionField != 0) unsupported("the random noise field (AMPL_NOISE) is not implemented");
-    for (int b = 0; b < 6; b++)
-        if (nBoundary[b] != 1) unsupported("only zero-derivative boundaries (NB=1) are implemented");
+    int modes[3];
+    boundary_modes(nBoundary, modes);
+    for (int a = 0; a < 3; a++)
+        if (modes[a] < 0)
+            unsupported(std::string("NB flags leave the ") + "xyz"[a] +
+                        " ghosts undefined (use 1 or 3 on both sides, or 2 on x)");
     if (ndPrint < 1) unsupported("'Printing on' must be >= 1");
     if (!(nSequence == 0 || nSequence == 1 || nSequence == 2 || nSequence == 5 || nSequence == 9 ||
           nSequence == 98 || nSequence == 99))
@@ -355,6 +377,7 @@ void NmrRelaxationSolve::init()
     ReadGridSize();
     ReadStart();
     Validate();
+    boundary_modes(nBoundary, bcMode);
     ReadActiveCells();
     ReadVelocity();
     Coefficients();
@@ -381,11 +404,14 @@ std::string NmrRelaxationSolve::config_json() const
       << ", \"gradientPulse\": " << gradientPulse << ", \"multipleGradients\": [" << multipleGradients[0] << ", "
       << multipleGradients[1] << ", " << multipleGradients[2] << "], \"endTime\": " << endTime
       << ", \"pEq\": " << equilibriumMagnetization << ", \"dTime\": " << dTime << ", \"nStep\": " << nStep
-      << ", \"dTimeVel\": " << dTimeVel << ", \"dTimeDiff\": " << dTimeDiff << ", \"dTimeRel\": " << dTimeRel << "}";
+      << ", \"dTimeVel\": " << dTimeVel << ", \"dTimeDiff\": " << dTimeDiff << ", \"dTimeRel\": " << dTimeRel
+      << ", \"nBoundary\": [" << nBoundary[0] << ", " << nBoundary[1] << ", " << nBoundary[2] << ", " << nBoundary[3]
+      << ", " << nBoundary[4] << ", " << nBoundary[5] << "], \"bc\": [" << bcMode[0] << ", " << bcMode[1] << ", "
+      << bcMode[2] << "]}";
     return o.str();
 }
 
-void NmrRelaxationSolve::create_context()
+void NmrRelaxationSolve::create_context(int scheme)
 {
     if (!initialised_) unsupported("init() must run before solve");
     if (ctx_) {
@@ -400,8 +426,11 @@ void NmrRelaxationSolve::create_context()
     p.ncomp = 3;
     p.flags = (nDiffusion == 1 ? NMR_F_DIFFUSION : 0) | (nVelocity == 1 ? NMR_F_VELOCITY : 0) |
               (nVectorMult == 1 ? NMR_F_PRECESSION : 0);
-    p.scheme = NMR_SCHEME_UPWIND;
+    p.scheme = scheme_ >= 0 ? scheme_ : scheme;
     p.phase_mode = NMR_PHASE_LINEAR;
+    p.bc[0] = bcMode[0];
+    p.bc[1] = bcMode[1];
+    p.bc[2] = bcMode[2];
     p.buffer[0] = bufferSizeX;
     p.buffer[1] = bufferSizeY;
     p.buffer[2] = bufferSizeZ;
@@ -519,7 +548,7 @@ void csv_row(std::ofstream& f, double t, const double* s)
 // i % ndPrint == 0 (the row stamped t = i*dt holds i+1 updates).
 void NmrRelaxationSolve::solve_gpu()
 {
-    create_context();
+    create_context(NMR_SCHEME_MACCORMACK);
     flag1 = flag2 = flag3 = flag4 = flag5 = 0;
     std::ofstream res(path("results_gpu.csv"));
     res << "Time; SumX; SumY; SumZ" << std::endl;
@@ -566,7 +595,7 @@ void NmrRelaxationSolve::solve_gpu()
 // == 0 (before this step's update), update, pulses of Sequence at t = i*dt.
 void NmrRelaxationSolve::solve()
 {
-    create_context();
+    create_context(NMR_SCHEME_UPWIND);
     flag1 = flag2 = flag3 = flag4 = flag5 = 0;
     std::ofstream res(path("results_cpu.csv"));
     res << "Time; SumX; SumY; SumZ" << std::endl;

@@ -7,17 +7,22 @@
 //                      data/Nmr_grad.dat, data/Poro.dat, data/V[xyz].dat
 //                      (NmrRelaxationSolve.cpp:20-33, 68-2534)
 //   void solve();      the CPU-cadence loop of NmrRelaxationSolve.cpp:35-66
-//                      (signal before each update, results_cpu.csv)
+//                      (signal before each update, results_cpu.csv) with the
+//                      upwind ExplicitSimple update (cpp:3148-3282)
 //   void solve_gpu();  the GPU-cadence loop of NmrRelaxationSolveKernels.cu:1056-1319
 //                      (update, pulses, signal; results_gpu.csv and
-//                      results_NMR/results_grad_<g>_<g>_<g>.csv)
+//                      results_NMR/results_grad_<g>_<g>_<g>.csv) with the
+//                      MacCormack update (ExplicitMakUpgrade, cu:1473-1501)
 //
 // Both loops run the update on the MI355X through the C ABI of nmr_hip.h;
 // there is no CPU solver behind them.  The whole domain lives on one GPU, so
 // MPI_size must be 1.  Configurations the GPU path does not implement
-// (two-phase NCOMP=2, face-wise relaxivity files, random noise field,
-// periodic/Dirichlet boundaries) are reported and end the program with
-// exit(1), where the reference would run on with uninitialised data.
+// (two-phase NCOMP=2, face-wise relaxivity files, random noise field, NB
+// flag combinations whose ghosts the reference never refreshes) are
+// reported and end the program with exit(1), where the reference would run
+// on with uninitialised data.  The NB flags select zero-derivative,
+// periodic or zero-value ghosts in both loops (the reference's host loop
+// ignores them).
 #ifndef NMR_RELAXATION_SOLVE_H
 #define NMR_RELAXATION_SOLVE_H
 
@@ -45,11 +50,14 @@ public:
 
     // Not in the reference: parse only the grid and Nmr_data.dat /
     // Nmr_grad.dat (no support checks, no field files), the parsed
-    // configuration as JSON (for tests and tools), and the directory the
-    // data/ files are read from.
+    // configuration as JSON (for tests and tools), the directory the data/
+    // files are read from, and the update scheme (-1: the reference's own,
+    // MacCormack in solve_gpu() and the upwind ExplicitSimple in solve();
+    // or NMR_SCHEME_UPWIND / NMR_SCHEME_MACCORMACK for both).
     void read_config();
     std::string config_json() const;
     void set_root(const std::string& dir) { root_ = dir; }
+    void set_scheme(int scheme) { scheme_ = scheme; }
 
 private:
     struct Pulse {
@@ -64,6 +72,7 @@ private:
     int nPoro = 0, nComponents = 1, nFluid = 1, nSurfRelWater = 0, nSurfRelOil = 0;
     int nVectorMult = 0, nVelocity = 0, nDiffusion = 0, nDirectionField = 3, nSequence = 0;
     int nBoundary[6] = {1, 1, 1, 1, 1, 1};
+    int bcMode[3] = {0, 0, 0}; // NMR_BC_* per axis from the NB flags
     int bufferSizeX = 0, bufferSizeY = 0, bufferSizeZ = 0;
     double dx = 0, dy = 0, dz = 0, bsx = 0, bsy = 0, bsz = 0, sequenceTime1 = 0, sequenceTime2 = 0;
     double constGradX = 0, constGradY = 0, constGradZ = 0, noiseAmpl = 0, velocityMultiplicator = 1;
@@ -80,6 +89,7 @@ private:
     std::vector<uint8_t> actCell;
     std::vector<double> vel[3], magn[3];
     bool initialised_ = false;
+    int scheme_ = -1;
     std::string root_ = ".";
     nmr_ctx* ctx_ = nullptr;
 
@@ -93,7 +103,7 @@ private:
     void Coefficients();
     void FillInMagnetization();
     void StepTime();
-    void create_context();
+    void create_context(int scheme);
     std::vector<Pulse> SequenceGpu(double currentTime);
     std::vector<Pulse> SequenceCpu(double currentTime);
     void apply(const std::vector<Pulse>& pulses);

@@ -283,9 +283,31 @@ def _chain(dx, dy, dz, f):
     return out
 
 
-def run_dropin(cfg, mask, vel_raw, gpu=True):
-    """The reference's solve_gpu (gpu=True, cu:1187-1227) or solve (cpp:35-66)
-    loop in FP64 for one fluid: returns (rows (n, 4), dTime, nStep).
+def nb_modes(nb):
+    """Ghost fill per axis from NB1..NB6 by the rules of
+    BoundaryConditionsForVectorField_gpu (cu:1328-1396): x zero derivative
+    when NB1 = NB3 = 1, periodic when both are 3, zero value when either is 2
+    (applied last); y (NB2, NB4) and z (NB5, NB6) zero derivative or periodic
+    when both agree.  None where no rule refreshes the ghosts."""
+    x = None
+    if nb[0] == 1 and nb[2] == 1:
+        x = BC_NEUMANN
+    if nb[0] == 3 and nb[2] == 3:
+        x = BC_PERIODIC
+    if nb[0] == 2 or nb[2] == 2:
+        x = BC_DIRICHLET
+
+    def pair(lo, hi):
+        return BC_NEUMANN if lo == hi == 1 else (BC_PERIODIC if lo == hi == 3 else None)
+
+    return (x, pair(nb[1], nb[3]), pair(nb[4], nb[5]))
+
+
+def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None):
+    """The reference's solve_gpu (gpu=True, cu:1187-1227, MacCormack update)
+    or solve (cpp:35-66, ExplicitSimple update) loop in FP64 for one fluid:
+    returns (rows (n, 4), dTime, nStep).  scheme overrides the update.  The
+    NB flags select the ghost fill in both loops.
     cfg: dict of the data-file values (see initializator_amd.refio)."""
     import math
 
@@ -307,8 +329,11 @@ def run_dropin(cfg, mask, vel_raw, gpu=True):
     d = [(1.0 if ndf == i + 1 else 0.0) if ndf != 0 else bs[i] / btot for i in range(3)]
     buf = (cfg["bufferX"], cfg["bufferY"], cfg["bufferZ"])
     peq = 1.0 / ((nx - 2 * buf[0]) * (ny - 2 * buf[1]) * (nz - 2 * buf[2]))
+    if scheme is None:
+        scheme = MACCORMACK if gpu else UPWIND
+    bc = nb_modes([cfg.get(f"NB{i}", 1) for i in range(1, 7)])
     p = params(mask.shape, dx, 1.0, dy=dy, dz=dz, precession=cfg["NVECTOR"] == 1, velocity=cfg["NVELOC"] == 1,
-               diffusion=cfg["NDIFFUS"] == 1, p_eq=peq, b0=b0)
+               diffusion=cfg["NDIFFUS"] == 1, p_eq=peq, b0=b0, scheme=scheme, bc=bc)
     act = layout.pad_mask(mask)
     D, t1, t2, (dmax, tmin) = fill_coefficients(p, act, Dw, T1b, T2b, r1, r2)
     # step limits (cpp:1583-1587, 1849-1864, 1980-1988, 2519-2534)
@@ -344,6 +369,9 @@ def run_dropin(cfg, mask, vel_raw, gpu=True):
     m = [np.where(pad_pore, peq * d[q], 0.0) for q in range(3)]
     nf = fill_gradient(p, grads if cfg["NVECTOR"] == 1 else (0.0, 0.0, 0.0))
     a = [x.copy() for x in m]
+    pc = [np.zeros(padded_shape(p)) for _ in range(6)]
+    for x in m:
+        bc_field(p, x)
     flags = [0] * 5
     gp = cfg.get("gradientPulse", 0.0)
     mg = cfg.get("multipleGradients", (0.0, 0.0, 0.0))
@@ -353,7 +381,10 @@ def run_dropin(cfg, mask, vel_raw, gpu=True):
         t = i * dt
         if not gpu and i % ndp == 0:
             rows.append([t] + [output(p, act, buf, m[q])[0] for q in range(3)])
-        explicit_simple(p, act, D, t1, t2, vel, nf, m, a)
+        if scheme == MACCORMACK:
+            maccormack(p, act, D, t1, t2, vel, nf, m, pc)
+        else:
+            explicit_simple(p, act, D, t1, t2, vel, nf, m, a)
         for fl, par, ax in sequence_events(gpu, nseq, t, dt, T1s, T2s, d, gp, mg, flags):
             add_impulse(p, act, fl, par, ax, m)
         for x in m:

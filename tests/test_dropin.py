@@ -121,27 +121,35 @@ def _write_case(root, shape, mask, vel, grad_table=None, **ov):
 
 
 @pytest.mark.gpu
-def test_fid_closed_form(exe, tmp_path):
+@pytest.mark.parametrize("scheme", ["maccormack", "upwind"])
+def test_fid_closed_form(exe, tmp_path, scheme):
     """FID on an all-pore box with bulk T2 only: the 90 deg pulse about x turns
-    pEq*N = 1 along z into -1 along y, which then decays by (1 - dt/T2)^i
-    (results_gpu.csv row i holds i+1 updates and the t = 0 pulse)."""
+    pEq*N = 1 along z into -1 along y, which then decays per update by
+    a(dt/T2), a(x) = 1 - x + x^2/2 for the MacCormack update solve_gpu() uses
+    by default, 1 - x for the upwind update (results_gpu.csv row i holds i+1
+    updates and the t = 0 pulse that follows the first)."""
     shape = (10, 6, 8)
     cfg = _write_case(tmp_path, shape, None, None, NPOR=0, NSEQUENCE=0, T2_WAT=0.01, Printing_on=50)
-    _run(exe, tmp_path, "1", "0", "0")
+    args = ["1", "0", "0"] + ([] if scheme == "maccormack" else ["--scheme", "upwind"])
+    _run(exe, tmp_path, *args)
     rows = refio.read_results(tmp_path / "results_gpu.csv")
     c = _config(exe, tmp_path, "1", "0", "0", full=True)
     dt = c["dTime"]
+    x = dt / 0.01
+    a = 1 - x + x * x / 2 if scheme == "maccormack" else 1 - x
     i = np.arange(0, c["nStep"] + 1, 50)
     assert len(rows) == len(i)
     np.testing.assert_allclose(rows[:, 0], i * dt, rtol=1e-6)
-    np.testing.assert_allclose(rows[:, 2], -(1 - dt / 0.01) ** i, rtol=5e-5)  # FP32 rounding of a uniform field
+    np.testing.assert_allclose(rows[:, 2], -a ** i, rtol=5e-5)  # FP32 rounding of a uniform field
     assert np.abs(rows[:, 1]).max() < 1e-6 and np.abs(rows[:, 3]).max() < 1e-6
     assert os.path.exists(tmp_path / "results_NMR" / "results_grad_1_0_0.csv")
 
 
 @pytest.mark.gpu
 def test_cpmg_echo_train_closed_form(exe, tmp_path):
-    """CPMG (20 echoes on the GPU path): each 180 deg pulse about x flips My."""
+    """CPMG (20 echoes on the GPU path): each 180 deg pulse about x flips My;
+    between pulses My decays by the MacCormack factor 1 - x + x^2/2 per
+    update, x = dt/T2."""
     shape = (6, 6, 6)
     T1s, T2 = 2e-4, 2e-3
     _write_case(tmp_path, shape, None, None, NPOR=0, NSEQUENCE=1, TIME1=T1s, T2_WAT=T2, Printing_on=7)
@@ -157,30 +165,66 @@ def test_cpmg_echo_train_closed_form(exe, tmp_path):
     assert len(flips) >= 5
     i = np.arange(0, c["nStep"] + 1, 7)
     sign = np.array([(-1) ** sum(1 for m in flips if m <= k) for k in i])
-    np.testing.assert_allclose(rows[:, 2], -sign * (1 - dt / T2) ** i, rtol=5e-5, atol=1e-7)
+    x = dt / T2
+    np.testing.assert_allclose(rows[:, 2], -sign * (1 - x + x * x / 2) ** i, rtol=5e-5, atol=1e-7)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cadence", ["gpu", "cpu"])
-def test_pfg_flow_against_fp64_loop(exe, tmp_path, cadence):
+@pytest.mark.parametrize("cadence,scheme,nb", [
+    ("gpu", None, (1, 1, 1, 1, 1, 1)),
+    ("cpu", None, (1, 1, 1, 1, 1, 1)),
+    ("gpu", "upwind", (1, 1, 1, 1, 1, 1)),
+    ("gpu", None, (3, 3, 3, 3, 1, 1)),
+    ("cpu", None, (2, 3, 1, 3, 3, 3)),
+])
+def test_pfg_flow_against_fp64_loop(exe, tmp_path, cadence, scheme, nb):
     """Porous box with divergence-free flow, surface relaxation and a pulsed-
     gradient stimulated echo (sequence 2): every CSV row of solve_gpu()
-    (results_gpu.csv) or solve() (results_cpu.csv) matches the FP64
-    restatement of the reference's loop to the CSV's 7 digits."""
+    (results_gpu.csv, MacCormack unless --scheme upwind) or solve()
+    (results_cpu.csv, upwind) matches the FP64 restatement of the reference's
+    loop to the CSV's 7 digits, with zero-derivative, periodic or zero-value
+    ghosts from the NB flags."""
     shape = (12, 10, 14)
     mask = generators.porosity(shape, 0.55, seed=6, sigma=1.2)
     vel = generators.velocity(mask, seed=6, sigma=1.5, vmax=5e-3)
     table = np.linspace(-0.03, 0.03, 399)
+    nbk = {f"NB{i + 1}": v for i, v in enumerate(nb)}
     cfg = _write_case(tmp_path, shape, mask, vel, grad_table=table, NSEQUENCE=2, TIME1=2e-4, TIME2=4e-4,
-                      R2_WAT=2e-5, R1_WAT=1e-5, T1_WAT=0.05, Printing_on=5, BX0=0.0, BZ0=1.0)
+                      R2_WAT=2e-5, R1_WAT=1e-5, T1_WAT=0.05, Printing_on=5, BX0=0.0, BZ0=1.0, **nbk)
     cfg["gradientPulse"] = float(table[349])
     args = ["350", "0", "0"] + (["--cpu-cadence"] if cadence == "cpu" else [])
+    if scheme:
+        args += ["--scheme", scheme]
     _run(exe, tmp_path, *args)
     rows = refio.read_results(tmp_path / f"results_{cadence}.csv")
-    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=(cadence == "gpu"))
+    sch = {None: None, "upwind": oracle.UPWIND}[scheme]
+    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=(cadence == "gpu"), scheme=sch)
     assert rows.shape == ref.shape, (rows.shape, ref.shape)
     np.testing.assert_allclose(rows[:, 0], ref[:, 0], rtol=2e-6)
     scale = np.abs(ref[:, 1:]).max()
     err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
-    print(f"[dropin] sequence 2, {cadence} cadence: {len(rows)} rows, max error {err:.2e} of the signal scale")
+    print(f"[dropin] sequence 2, {cadence} cadence, scheme {scheme or 'default'}, NB {nb}: {len(rows)} rows, "
+          f"max error {err:.2e} of the signal scale")
     assert err < 5e-6
+
+
+@pytest.mark.parametrize("nb,modes", [
+    ((1, 1, 1, 1, 1, 1), [0, 0, 0]),
+    ((3, 1, 3, 1, 3, 3), [1, 0, 1]),
+    ((2, 3, 1, 3, 1, 1), [2, 1, 0]),
+    ((3, 1, 2, 1, 1, 1), [2, 0, 0]),
+])
+def test_boundary_flags_map_like_the_gpu_ghost_fill(exe, tmp_path, nb, modes):
+    """NB1/NB3 (x), NB2/NB4 (y), NB5/NB6 (z) select the ghost fill as
+    BoundaryConditionsForVectorField_gpu applies it (cu:1328-1396)."""
+    _write_case(tmp_path, (4, 5, 6), None, None, NPOR=0, **{f"NB{i + 1}": v for i, v in enumerate(nb)})
+    c = _config(exe, tmp_path, "1", "0", "0", full=True)
+    assert c["bc"] == modes == [int(m) for m in oracle.nb_modes(nb)]
+
+
+def test_refuses_boundary_flags_that_leave_ghosts_stale(exe, tmp_path):
+    """x with NB1 = 1, NB3 = 3 matches no rule of the reference's ghost fill:
+    init() says so and exits 1."""
+    _write_case(tmp_path, (4, 5, 6), None, None, NPOR=0, NB1=1, NB3=3)
+    r = _run(exe, tmp_path, "1", "0", "0", check=False)
+    assert r.returncode == 1 and "ghosts undefined" in r.stderr
