@@ -210,6 +210,63 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
 /* GPU ms of the last run's steps and the voxel-steps they advanced. */
 int nmr_hip_batch_last_run_ms(nmr_batch* b, double* ms, int64_t* voxel_steps);
 
+/* ---- learned corrector ----
+ * No reference counterpart.  A 3-layer 3-D convolution net (6 -> 16 -> 16 -> 2
+ * channels, 3x3x3 kernels, zero padding, ReLU; BF16 weights and activations,
+ * FP32 accumulation on the matrix cores) that corrects one explicit step of
+ * factor*dt toward the result of factor plain steps.  Inputs per voxel:
+ * (Mx, My) of the state before the step times the pore mask, the pore mask,
+ * and the cell-centred velocity / vscale.  Output: the correction of
+ * (Mx, My) in units of out_scale, applied at pores.
+ * Parameters (FP32, NMR_CORRECTOR_PARAMS of them) in the order of
+ * conv3d weights: W1[16][6][3][3][3], b1[16], W2[16][16][3][3][3], b2[16],
+ * W3[2][16][3][3][3], b3[2] (kernel index kz, ky, kx; x fastest). */
+#define NMR_CORRECTOR_PARAMS 10402
+typedef struct nmr_corrector nmr_corrector;
+
+typedef struct nmr_train_opts {
+    int32_t factor;         /* plain steps per coarse step */
+    int32_t rounds;         /* sampling rounds: one sample per instance each */
+    int32_t stride;         /* plain steps of the batch before each round */
+    int32_t reserved;
+    double lr;              /* Adam step size */
+} nmr_train_opts;
+
+/* He-normal initial weights from seed (deterministic), zero biases. */
+int nmr_hip_corrector_create(uint64_t seed, nmr_corrector** out);
+int nmr_hip_corrector_destroy(nmr_corrector* net);
+int nmr_hip_corrector_num_params(void);
+int nmr_hip_corrector_get_params(nmr_corrector* net, float* params);
+/* Also resets the Adam moments. */
+int nmr_hip_corrector_set_params(nmr_corrector* net, const float* params);
+/* vscale: velocity unit of the input (m/s, default 1e-3); out_scale: unit of
+ * the output correction (0: set by the first training sample to the RMS of
+ * its targets). */
+int nmr_hip_corrector_set_scales(nmr_corrector* net, double vscale, double out_scale);
+int nmr_hip_corrector_get_scales(nmr_corrector* net, double* vscale, double* out_scale);
+/* The net on host data: features [nz][ny][nx][6] -> out [nz][ny][nx][2]. */
+int nmr_hip_corrector_forward(nmr_corrector* net, int32_t nx, int32_t ny, int32_t nz, const float* features,
+                              float* out);
+/* Gradient of sum(grad_out * out) with respect to the parameters. */
+int nmr_hip_corrector_backward(nmr_corrector* net, int32_t nx, int32_t ny, int32_t nz, const float* features,
+                               const float* grad_out, float* grad_params);
+/* Trains on samples the batch generates (ncomp 2, exact phase): each round
+ * advances the batch `stride` steps, keeps every instance's state, advances
+ * `factor` steps, and takes one Adam step per instance on the mean squared
+ * error over its pores between the net's output and (plain result - coarse
+ * step) / out_scale.  losses (nullable): [rounds][ninst], before each step. */
+int nmr_hip_corrector_train(nmr_corrector* net, nmr_batch* b, const nmr_train_opts* opts, double* losses);
+/* ncoarse coarse steps of factor*dt with the corrector on a context (ncomp
+ * 2, exact phase, upwind, zero-derivative boundaries).  A step is accepted
+ * when nothing is non-finite, the pore energy sum(Mx^2 + My^2) has not
+ * grown by more than a factor (1 + tol), and no correction exceeds tol
+ * times the largest |M|; otherwise it is redone as factor plain steps.
+ * grad_sched (nullable): [ncoarse*factor][3] z gradients per plain step.
+ * signal (nullable): [ncoarse+1][2] trimmed pore sums after each coarse step.
+ * fallbacks (nullable): number of coarse steps redone. */
+int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* net, int64_t ncoarse, int32_t factor, const double* grad_sched,
+                        double tol, double* signal, int64_t* fallbacks);
+
 const char* nmr_hip_strerror(int code);
 const char* nmr_hip_last_error(void);
 int nmr_hip_abi_version(void);

@@ -65,10 +65,23 @@ EXPORTS = (
     "nmr_hip_batch_set_tile",
     "nmr_hip_batch_autotune",
     "nmr_hip_batch_last_run_ms",
+    "nmr_hip_corrector_create",
+    "nmr_hip_corrector_destroy",
+    "nmr_hip_corrector_num_params",
+    "nmr_hip_corrector_get_params",
+    "nmr_hip_corrector_set_params",
+    "nmr_hip_corrector_set_scales",
+    "nmr_hip_corrector_get_scales",
+    "nmr_hip_corrector_forward",
+    "nmr_hip_corrector_backward",
+    "nmr_hip_corrector_train",
+    "nmr_hip_run_learned",
     "nmr_hip_strerror",
     "nmr_hip_last_error",
     "nmr_hip_abi_version",
 )
+
+NMR_CORRECTOR_PARAMS = 10402
 
 
 class NmrProblemC(ctypes.Structure):
@@ -114,6 +127,18 @@ class NmrBatchInstanceC(ctypes.Structure):
         ("grad_amp", c_double),
         ("delta", c_double),
         ("big_delta", c_double),
+    ]
+
+
+class NmrTrainOptsC(ctypes.Structure):
+    """Mirror of ``nmr_train_opts`` in include/nmr_hip.h."""
+
+    _fields_ = [
+        ("factor", c_int32),
+        ("rounds", c_int32),
+        ("stride", c_int32),
+        ("reserved", c_int32),
+        ("lr", c_double),
     ]
 
 
@@ -176,6 +201,19 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_batch_set_tile": (c_int, [vp, c_int32, c_int32, c_int32, c_int32]),
         "nmr_hip_batch_autotune": (c_int, [vp, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
         "nmr_hip_batch_last_run_ms": (c_int, [vp, dp, POINTER(c_int64)]),
+        "nmr_hip_corrector_create": (c_int, [ctypes.c_uint64, POINTER(vp)]),
+        "nmr_hip_corrector_destroy": (c_int, [vp]),
+        "nmr_hip_corrector_num_params": (c_int, []),
+        "nmr_hip_corrector_get_params": (c_int, [vp, POINTER(ctypes.c_float)]),
+        "nmr_hip_corrector_set_params": (c_int, [vp, POINTER(ctypes.c_float)]),
+        "nmr_hip_corrector_set_scales": (c_int, [vp, c_double, c_double]),
+        "nmr_hip_corrector_get_scales": (c_int, [vp, dp, dp]),
+        "nmr_hip_corrector_forward": (c_int, [vp, c_int32, c_int32, c_int32, POINTER(ctypes.c_float),
+                                              POINTER(ctypes.c_float)]),
+        "nmr_hip_corrector_backward": (c_int, [vp, c_int32, c_int32, c_int32, POINTER(ctypes.c_float),
+                                               POINTER(ctypes.c_float), POINTER(ctypes.c_float)]),
+        "nmr_hip_corrector_train": (c_int, [vp, vp, POINTER(NmrTrainOptsC), dp]),
+        "nmr_hip_run_learned": (c_int, [vp, vp, c_int64, c_int32, dp, c_double, dp, POINTER(c_int64)]),
         "nmr_hip_strerror": (c_char_p, [c_int]),
         "nmr_hip_last_error": (c_char_p, []),
         "nmr_hip_abi_version": (c_int, []),

@@ -7,21 +7,12 @@
 // solve_gpu (cu:1187-1227); every HIP call is checked and turned into a
 // status code instead of gpuAssert's exit() (headers/NmrRelaxationSolve.h:12-27).
 
-#include <hip/hip_runtime.h>
+#include "nmr_internal.h"
 
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
+#include <map>
+#include <mutex>
 
-#include "nmr_hip.h"
-#include "nmr_kernels.h"
-
-namespace {
+namespace nmri {
 
 thread_local std::string g_last_error;
 
@@ -36,52 +27,19 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(e_ == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
-    } while (0)
+} // namespace nmri
+
+namespace {
+
+using nmri::fail;
+using nmri::g_last_error;
+using nmri::round_up4;
 
 constexpr int kChunkSteps = 512; // steps whose partials are held before a reduction
 
-int round_up4(int n) { return (n + 3) & ~3; }
-
 } // namespace
 
-struct nmr_ctx {
-    nmr_problem prob{};
-    int device = -1;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int nc = 1;
-    bool vel = false;
-    bool exact = false;
-    int pitch = 0, pitch1 = 0;
-    size_t n_m = 0;  // floats per magnetization component (nz*ny*pitch)
-    size_t n_vx = 0, n_vy = 0, n_vz = 0;
-    float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    float* pred[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor
-    bool general = false; // MacCormack or a non-zero-derivative boundary: per-voxel kernel
-    uint8_t* mask = nullptr;
-    float* vx = nullptr;
-    float* vy = nullptr;
-    float* vz = nullptr;
-    double* partials = nullptr;
-    size_t partials_cap = 0; // doubles
-    double* dsignal = nullptr;
-    size_t dsignal_cap = 0;  // doubles
-    int cur = 0;
-    bool has_mask = false, has_mag = false;
-    bool inert_dirty = true; // mask or velocity changed since the last check
-    int inert = 0;           // no pore-solid face carries velocity
-    int* dflag = nullptr;
-    int variant = -1, lz = 0; // -1 / 0 = automatic
-    double last_ms = 0.0;
-    int64_t last_sweeps = 0;
-    int64_t device_bytes = 0;
-};
+
 
 namespace {
 
@@ -865,44 +823,6 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 // ---------------------------------------------------------------------------
 // Batched solves: one persistent launch per step over all instances.
 
-#include <map>
-#include <mutex>
-
-struct nmr_batch {
-    nmr_problem prob{};
-    int ninst = 0, nc = 1;
-    bool vel = false, exact = false;
-    std::vector<nmr_batch_instance> spec;
-    struct Inst {
-        int pitch = 0, pitch1 = 0;
-        size_t n_m = 0, n_vx = 0, n_vy = 0, n_vz = 0;
-        float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-        uint8_t* mask = nullptr;
-        float *vx = nullptr, *vy = nullptr, *vz = nullptr;
-        bool uploaded = false;
-        int inert = 0;
-    };
-    std::vector<Inst> inst;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int variant = -1, lz = 0, graph_steps = 64, persistent = 1;
-    int parity = 0;
-    int64_t steps_done = 0;
-    // launch state for the current variant / lz / grid kind
-    int cur_variant = -2, cur_lz = -1, cur_persistent = -1, grid = 0, ntiles = 0;
-    nmr::BatchInst* d_inst = nullptr;
-    int2* d_tiles = nullptr;
-    int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
-    double* d_part = nullptr;
-    size_t part_cap = 0;
-    double* d_signal = nullptr;
-    size_t signal_cap = 0;
-    double* d_fpart = nullptr; // field-sum partials (max nz)
-    int* d_flag = nullptr;
-    double last_ms = 0.0;
-    int64_t last_voxel_steps = 0;
-};
-
 namespace {
 
 std::mutex g_tune_mu;
@@ -1173,13 +1093,16 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
     const int P = I.pitch, P1 = I.pitch1;
     {
         std::vector<uint8_t> h(I.n_m, 0);
+        int64_t np = 0;
         for (int k = 0; k < s.nz; k++)
             for (int j = 0; j < s.ny; j++)
                 for (int x = 0; x < s.nx; x++) {
                     const uint8_t v = mask[(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)];
                     if (v > 1) return fail(NMR_ERR_INVALID, "mask value %d (want 0/1)", (int)v);
                     h[(size_t)k * s.ny * P + (size_t)j * P + x] = v;
+                    np += v;
                 }
+        I.npore = np;
         HIP_TRY(hipMemcpy(I.mask, h.data(), h.size(), hipMemcpyHostToDevice));
     }
     if (b->vel) {
@@ -1451,3 +1374,52 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Entry points for the other host files of the library.
+
+namespace nmri {
+
+nmr::Coef make_coef(const nmr_ctx* c, const double g[3]) { return ::make_coef(c, g); }
+
+int ensure_partials(nmr_ctx* c, size_t doubles) { return ::ensure_partials(c, doubles); }
+
+int sweep_step(nmr_ctx* c, const nmr::Coef& cf, bool exact, int in, int out)
+{
+    const nmr_problem& p = c->prob;
+    Launch L{};
+    auto_launch(c, -1, 0, &L);
+    if (L.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
+    int rc = refresh_inert(c);
+    if (rc) return rc;
+    if ((rc = ::ensure_partials(c, (size_t)L.nwg * c->nc))) return rc;
+    int w[6];
+    signal_window(p, w);
+    nmr::SweepArgs a{};
+    for (int q = 0; q < 3; q++) {
+        a.min[q] = c->m[in][q];
+        a.mout[q] = c->m[out][q];
+    }
+    a.mask = c->mask;
+    a.vx = c->vx;
+    a.vy = c->vy;
+    a.vz = c->vz;
+    a.partials = c->partials;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.pitch = c->pitch;
+    a.pitch1 = c->pitch1;
+    a.ntx = L.ntx;
+    a.nty = L.nty;
+    a.ntz = L.ntz;
+    a.lz = L.lz;
+    a.nblocks = L.nwg;
+    a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
+    a.inert = c->inert;
+    a.c = cf;
+    HIP_TRY(nmr::launch_sweep(c->nc, c->vel, exact, L.variant, a, c->stream));
+    return NMR_OK;
+}
+
+} // namespace nmri

@@ -1,0 +1,112 @@
+// nmr_internal.h -- state shared by the host files of libnmr_hip.so (the C
+// ABI in nmr_abi.hip and the learned corrector in nmr_learn.hip).  Not part
+// of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "nmr_hip.h"
+#include "nmr_kernels.h"
+
+namespace nmri {
+// Record a status detail for nmr_hip_last_error() and return code.
+int fail(int code, const char* fmt, ...);
+inline int round_up4(int n) { return (n + 3) & ~3; }
+} // namespace nmri
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return nmri::fail(e_ == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s (%s:%d)", \
+                              #expr, hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+struct nmr_ctx {
+    nmr_problem prob{};
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nc = 1;
+    bool vel = false;
+    bool exact = false;
+    int pitch = 0, pitch1 = 0;
+    size_t n_m = 0;  // floats per magnetization component (nz*ny*pitch)
+    size_t n_vx = 0, n_vy = 0, n_vz = 0;
+    float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    float* pred[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor
+    bool general = false; // MacCormack or a non-zero-derivative boundary: per-voxel kernel
+    uint8_t* mask = nullptr;
+    float* vx = nullptr;
+    float* vy = nullptr;
+    float* vz = nullptr;
+    double* partials = nullptr;
+    size_t partials_cap = 0; // doubles
+    double* dsignal = nullptr;
+    size_t dsignal_cap = 0;  // doubles
+    int cur = 0;
+    bool has_mask = false, has_mag = false;
+    bool inert_dirty = true; // mask or velocity changed since the last check
+    int inert = 0;           // no pore-solid face carries velocity
+    int* dflag = nullptr;
+    int variant = -1, lz = 0; // -1 / 0 = automatic
+    double last_ms = 0.0;
+    int64_t last_sweeps = 0;
+    int64_t device_bytes = 0;
+};
+
+struct nmr_batch {
+    nmr_problem prob{};
+    int ninst = 0, nc = 1;
+    bool vel = false, exact = false;
+    std::vector<nmr_batch_instance> spec;
+    struct Inst {
+        int pitch = 0, pitch1 = 0;
+        size_t n_m = 0, n_vx = 0, n_vy = 0, n_vz = 0;
+        float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+        uint8_t* mask = nullptr;
+        float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+        bool uploaded = false;
+        int inert = 0;
+        int64_t npore = 0;
+    };
+    std::vector<Inst> inst;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int variant = -1, lz = 0, graph_steps = 64, persistent = 1;
+    int parity = 0;
+    int64_t steps_done = 0;
+    // launch state for the current variant / lz / grid kind
+    int cur_variant = -2, cur_lz = -1, cur_persistent = -1, grid = 0, ntiles = 0;
+    nmr::BatchInst* d_inst = nullptr;
+    int2* d_tiles = nullptr;
+    int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
+    double* d_part = nullptr;
+    size_t part_cap = 0;
+    double* d_signal = nullptr;
+    size_t signal_cap = 0;
+    double* d_fpart = nullptr; // field-sum partials (max nz)
+    int* d_flag = nullptr;
+    double last_ms = 0.0;
+    int64_t last_voxel_steps = 0;
+};
+
+namespace nmri {
+// Coefficients of one step of a context's problem under gradient g (T/m).
+nmr::Coef make_coef(const nmr_ctx* c, const double g[3]);
+// Grow the context's partial-sum buffer to hold `doubles` values.
+int ensure_partials(nmr_ctx* c, size_t doubles);
+// One z-sweep step with coefficients cf from buffer `in` into buffer `out`
+// of the context (the automatic one-step tile), on its stream; the step's
+// signal partials go to the context's partial buffer.
+int sweep_step(nmr_ctx* c, const nmr::Coef& cf, bool exact, int in, int out);
+} // namespace nmri

@@ -124,6 +124,81 @@ struct FieldSumArgs {
     int bx0, bx1, by0, by1, bz0, bz1;
 };
 
+// ---- learned corrector (nmr_learn_kernels.hip) ----
+// Tensors: BF16 channels-last on the grid padded by one zero voxel per side.
+constexpr int kWgradPart = 27 * 256 + 16; // floats per wavefront partial of wgrad
+
+struct ConvArgs {
+    const uint16_t* in;     // [(nz+2)(ny+2)(nx+2)][cin]
+    const uint16_t* wa;     // packed A fragments [chunks][64][8]
+    const float* bias;      // [16] (forward modes)
+    const uint16_t* ref;    // gate tensor [..][16] (input-gradient mode)
+    uint16_t* out;          // [..][16]
+    float* outf;            // [..][2] (linear mode)
+    const int* rows;        // row list, row = (z*ny + y)*nseg + seg; null = all rows
+    int nrows;
+    int nx, ny, nz, nseg;   // nseg = ceil(nx/16)
+};
+
+struct WgradArgs {
+    const uint16_t* in;     // layer input [..][cin]
+    const uint16_t* gout;   // output gradient [..][16]
+    float* part;            // [waves][kWgradPart]
+    int nx, ny, nz;
+};
+
+struct FeatureArgs {
+    const float* mx;
+    const float* my;        // nullable
+    const uint8_t* mask;
+    const float* vx;        // nullable (no velocity)
+    const float* vy;
+    const float* vz;
+    uint16_t* out;          // [..][8]
+    int nx, ny, nz, pitch, pitch1;
+    float vinv;             // 1 / velocity scale
+};
+
+struct LossArgs {
+    const float* pred;      // [..][2]
+    const float* tx;        // target, compact with pitch
+    const float* ty;
+    const uint8_t* mask;
+    uint16_t* gout;         // [..][16]
+    double* lpart;          // [blocks]
+    int nx, ny, nz, pitch;
+    float tinv;             // 1 / output scale
+    float winv;             // 1 / number of pores
+};
+
+struct ApplyArgs {
+    float* mx;
+    float* my;
+    const uint8_t* mask;
+    const float* pred;      // [..][2] or null (statistics only)
+    double* part;           // [blocks][6]
+    int nx, ny, nz, pitch;
+    int bx0, bx1, by0, by1, bz0, bz1;
+    float scale;
+};
+
+hipError_t launch_conv(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s);
+hipError_t launch_wgrad(int cin, const WgradArgs& a, int grid, hipStream_t s);
+hipError_t launch_wreduce(const float* part, int nwaves, int cin_real, int cout_real, float* grad, hipStream_t s);
+hipError_t launch_pack(const float* w, int cin_real, int cout_real, int cin_store, int backward, uint16_t* frag,
+                       hipStream_t s);
+hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
+hipError_t launch_feature_copy(const float* f6, int nx, int ny, int nz, uint16_t* out, hipStream_t s);
+hipError_t launch_gout_copy(const float* g2, int nx, int ny, int nz, uint16_t* out, hipStream_t s);
+int loss_blocks(int nx, int ny, int nz);
+hipError_t launch_loss(const LossArgs& a, hipStream_t s);
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t s);
+hipError_t launch_diff(const float* ax, const float* ay, const float* bx, const float* by, float* ox, float* oy,
+                       long n, hipStream_t s);
+int apply_blocks(int nx, int ny, int nz);
+hipError_t launch_apply(const ApplyArgs& a, hipStream_t s);
+
 int num_variants();
 // steps: time steps per pass (1, or 2 for the temporally blocked sweep)
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);

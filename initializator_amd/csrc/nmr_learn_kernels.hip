@@ -1,0 +1,485 @@
+// nmr_learn_kernels.hip -- device kernels of the learned corrector: a 3-layer
+// 3-D convolution net (6 -> 16 -> 16 -> 2 channels, 3x3x3 kernels, "same"
+// zero padding, ReLU between layers) with BF16 weights and activations and
+// FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16), its backward pass (input and weight
+// gradients), Adam, and the glue between the solver's FP32 fields and the
+// net's tensors.
+//
+// Tensors live on the grid padded by one zero voxel per side, channels last:
+// element (x, y, z, c) of an nx*ny*nz grid with C channels is at
+// (((z+1)*(ny+2) + y+1)*(nx+2) + x+1)*C + c.  The halo stays zero, which is
+// the zero padding of the convolution.
+//
+// Convolution as an implicit GEMM: out[v][co] = sum_{t,ci} W[co][ci][t] *
+// in[v + off(t)][ci].  One wavefront computes a row of 16 x-consecutive
+// voxels for all 16 output channels: the MFMA's A operand is the weight
+// matrix (16 output channels x 32 reduction indices), its B operand the
+// input (32 reduction indices x 16 voxels), and the reduction index k of a
+// 32-chunk covers 32/CIN taps of CIN channels.  The A fragments of all
+// chunks stay in registers for the whole kernel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nmr_kernels.h"
+
+namespace nmr {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float fl4 __attribute__((ext_vector_type(4)));
+
+// Round to nearest even (NaN kept quiet), and back.
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bool bf_pos(uint16_t h) { return h != 0 && (h & 0x8000u) == 0; }
+
+__device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// tap t = kz*9 + ky*3 + kx, offsets -1..1 on each axis
+__device__ __forceinline__ long tap_off(int t, long px, long plane)
+{
+    const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+    return (long)(kz - 1) * plane + (long)(ky - 1) * px + (long)(kx - 1);
+}
+
+// MODE 0: + bias, ReLU, BF16 out (16 ch); 1: + bias, FP32 out (2 ch);
+// 2: gated by ref > 0, BF16 out (16 ch) -- the input gradient through ReLU.
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
+{
+    constexpr int TPC = 32 / CIN;               // taps per reduction chunk
+    constexpr int NCH = (27 + TPC - 1) / TPC;   // chunks
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    bf16x8 A[NCH];
+    const uint4* wa4 = reinterpret_cast<const uint4*>(a.wa);
+#pragma unroll
+    for (int c = 0; c < NCH; c++) A[c] = as_bf8(wa4[c * 64 + lane]);
+    const long px = a.nx + 2, plane = px * (long)(a.ny + 2);
+    const int cin0 = (8 * g) % CIN;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    for (int r = wave; r < a.nrows; r += nwaves) {
+        const int row = a.rows ? a.rows[r] : r;
+        const int seg = row % a.nseg, yz = row / a.nseg;
+        const int y = yz % a.ny, z = yz / a.ny;
+        const int x = seg * 16 + n;
+        const int xv = x < a.nx ? x : a.nx - 1;
+        const long base = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * px + (xv + 1);
+        uint4 B[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int t = c * TPC + (8 * g) / CIN;
+            B[c] = make_uint4(0u, 0u, 0u, 0u);
+            if (t < 27)
+                B[c] = *reinterpret_cast<const uint4*>(a.in + (base + tap_off(t, px, plane)) * CIN + cin0);
+        }
+        fl4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < NCH; c++) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], as_bf8(B[c]), acc, 0, 0, 0);
+        if (x >= a.nx) continue;
+        // lane holds output channels 4g..4g+3 of voxel n
+        if constexpr (MODE == 1) {
+            if (g == 0) {
+                a.outf[base * 2 + 0] = acc[0] + a.bias[0];
+                a.outf[base * 2 + 1] = acc[1] + a.bias[1];
+            }
+        } else {
+            uint16_t h[4];
+            if constexpr (MODE == 0) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) h[j] = f2bf(fmaxf(acc[j] + a.bias[4 * g + j], 0.f));
+            } else {
+                const uint2 rv = *reinterpret_cast<const uint2*>(a.ref + base * 16 + 4 * g);
+                const uint16_t rr[4] = {(uint16_t)(rv.x & 0xffffu), (uint16_t)(rv.x >> 16),
+                                        (uint16_t)(rv.y & 0xffffu), (uint16_t)(rv.y >> 16)};
+#pragma unroll
+                for (int j = 0; j < 4; j++) h[j] = bf_pos(rr[j]) ? f2bf(acc[j]) : (uint16_t)0;
+            }
+            *reinterpret_cast<uint2*>(a.out + base * 16 + 4 * g) =
+                make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+        }
+    }
+}
+
+// Weight gradient dW[t][ci][co] = sum_v in[v + off(t)][ci] * gout[v][co] and
+// bias gradient sum_v gout[v][co], per wavefront over 32-voxel x segments,
+// into part[wave][27*256 + 16].  MFMA: A = input (16 ci x 32 voxels), B =
+// output gradient (32 voxels x 16 co), both gathered per lane.
+template <int CIN>
+__global__ void __launch_bounds__(256) wgrad_kernel(WgradArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    const long px = a.nx + 2, plane = px * (long)(a.ny + 2);
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    fl4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) acc[t] = fl4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    const int nseg32 = (a.nx + 31) / 32;
+    const int nrows = nseg32 * a.ny * a.nz;
+    const bool chan = n < CIN;
+    for (int r = wave; r < nrows; r += nwaves) {
+        const int seg = r % nseg32, yz = r / nseg32;
+        const int y = yz % a.ny, z = yz / a.ny;
+        const long rowb = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * px;
+        // B: gout[x0 + 8g + j][co = n]
+        uint16_t bv[8];
+        long vb[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int x = seg * 32 + 8 * g + j;
+            const bool ok = x < a.nx;
+            vb[j] = rowb + (ok ? x : a.nx - 1) + 1;
+            bv[j] = ok ? a.gout[vb[j] * 16 + n] : (uint16_t)0;
+            bsum += bf2f(bv[j]);
+        }
+        const uint4 B = make_uint4((uint32_t)bv[0] | ((uint32_t)bv[1] << 16), (uint32_t)bv[2] | ((uint32_t)bv[3] << 16),
+                                   (uint32_t)bv[4] | ((uint32_t)bv[5] << 16), (uint32_t)bv[6] | ((uint32_t)bv[7] << 16));
+#pragma unroll
+        for (int t = 0; t < 27; t++) {
+            const long o = tap_off(t, px, plane);
+            uint16_t av[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) av[j] = chan ? a.in[(vb[j] + o) * CIN + n] : (uint16_t)0;
+            const uint4 Av = make_uint4((uint32_t)av[0] | ((uint32_t)av[1] << 16),
+                                        (uint32_t)av[2] | ((uint32_t)av[3] << 16),
+                                        (uint32_t)av[4] | ((uint32_t)av[5] << 16),
+                                        (uint32_t)av[6] | ((uint32_t)av[7] << 16));
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(Av), as_bf8(B), acc[t], 0, 0, 0);
+        }
+    }
+    float* dst = a.part + (size_t)wave * kWgradPart;
+    // D[i = ci = 4g + j][col = co = n]
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) dst[t * 256 + (4 * g + j) * 16 + n] = acc[t][j];
+    // bias: lanes n, n+16, n+32, n+48 hold the same co
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (g == 0) dst[27 * 256 + n] = bsum;
+}
+
+// Sum of the wavefront partials in a fixed order, scattered to the
+// canonical gradient layout W[co][ci][t] (cout_real x cin_real x 27) followed
+// by the bias.
+__global__ void __launch_bounds__(256) wreduce_kernel(const float* __restrict__ part, int nwaves, int cin_real,
+                                                      int cout_real, float* __restrict__ grad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kWgradPart) return;
+    double s = 0.0;
+    for (int w = 0; w < nwaves; w++) s += (double)part[(size_t)w * kWgradPart + e];
+    if (e < 27 * 256) {
+        const int t = e / 256, ci = (e / 16) % 16, co = e % 16;
+        if (ci < cin_real && co < cout_real) grad[((size_t)co * cin_real + ci) * 27 + t] = (float)s;
+    } else {
+        const int co = e - 27 * 256;
+        if (co < cout_real) grad[(size_t)cout_real * cin_real * 27 + co] = (float)s;
+    }
+}
+
+// Packs the canonical FP32 weights of one layer, W[co][ci][t], into the A
+// fragments of conv_kernel: forward (rows = output channels co, reduction
+// (t, ci) with cin_store channels per tap) or input gradient (rows = ci,
+// reduction (t', co) over 16 stored channels, W'[t'][co][ci] = W[26-t'][ci][co]).
+__global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ w, int cin_real, int cout_real,
+                                                   int cin_store, int backward, uint16_t* __restrict__ frag)
+{
+    const int cin = backward ? 16 : cin_store;
+    const int tpc = 32 / cin, nch = (27 + tpc - 1) / tpc;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nch * 64 * 8) return;
+    const int c = e / 512, lane = (e / 8) % 64, j = e % 8;
+    const int g = lane / 16, row = lane % 16;
+    const int k = 8 * g + j;
+    const int t = c * tpc + k / cin, ch = k % cin;
+    float v = 0.f;
+    if (t < 27) {
+        if (!backward) {
+            if (ch < cin_real && row < cout_real) v = w[((size_t)row * cin_real + ch) * 27 + t];
+        } else {
+            if (ch < cout_real && row < cin_real) v = w[((size_t)ch * cin_real + row) * 27 + (26 - t)];
+        }
+    }
+    frag[e] = f2bf(v);
+}
+
+// Input tensor of the net: (Mx, My) * pore, pore, cell-centred velocity /
+// vscale, two zero channels, BF16, 8 channels per voxel.
+__global__ void __launch_bounds__(256) feature_kernel(FeatureArgs a)
+{
+    const long n = (long)a.nx * a.ny * a.nz;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int x = (int)(v % a.nx), y = (int)((v / a.nx) % a.ny), z = (int)(v / ((long)a.nx * a.ny));
+    const size_t c = ((size_t)z * a.ny + y) * a.pitch + x;
+    const float pore = a.mask[c] ? 1.f : 0.f;
+    float f[8] = {a.mx[c] * pore, a.my ? a.my[c] * pore : 0.f, pore, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (a.vx) {
+        const size_t cx = ((size_t)z * a.ny + y) * a.pitch1 + x;
+        const size_t cy = ((size_t)z * (a.ny + 1) + y) * a.pitch + x;
+        const size_t plane = (size_t)a.ny * a.pitch;
+        f[3] = 0.5f * (a.vx[cx] + a.vx[cx + 1]) * a.vinv;
+        f[4] = 0.5f * (a.vy[cy] + a.vy[cy + a.pitch]) * a.vinv;
+        f[5] = 0.5f * (a.vz[c] + a.vz[c + plane]) * a.vinv;
+    }
+    const long pv = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * (a.nx + 2) + (x + 1);
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = (uint32_t)f2bf(f[2 * q]) | ((uint32_t)f2bf(f[2 * q + 1]) << 16);
+    *reinterpret_cast<uint4*>(a.out + pv * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Same tensor from host-given FP32 features [nz][ny][nx][6] (test entry).
+__global__ void __launch_bounds__(256) feature_copy_kernel(const float* __restrict__ f6, int nx, int ny, int nz,
+                                                           uint16_t* __restrict__ out)
+{
+    const long n = (long)nx * ny * nz;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long)nx * ny));
+    const long pv = ((long)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + (x + 1);
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float lo = 2 * q < 6 ? f6[v * 6 + 2 * q] : 0.f, hi = 2 * q + 1 < 6 ? f6[v * 6 + 2 * q + 1] : 0.f;
+        w[q] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+    }
+    *reinterpret_cast<uint4*>(out + pv * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Output gradient of the last layer as a BF16 16-channel tensor from
+// host-given FP32 values [nz][ny][nx][2] (test entry).
+__global__ void __launch_bounds__(256) gout_copy_kernel(const float* __restrict__ g2, int nx, int ny, int nz,
+                                                        uint16_t* __restrict__ out)
+{
+    const long n = (long)nx * ny * nz;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long)nx * ny));
+    const long pv = ((long)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + (x + 1);
+    uint4* o = reinterpret_cast<uint4*>(out + pv * 16);
+    o[0] = make_uint4((uint32_t)f2bf(g2[v * 2]) | ((uint32_t)f2bf(g2[v * 2 + 1]) << 16), 0u, 0u, 0u);
+    o[1] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Squared-error loss over pores of (pred - target/scale) and its gradient
+// 2 (pred - t) / npore as the BF16 16-channel output gradient; per-block
+// partial loss sums.
+__global__ void __launch_bounds__(256) loss_kernel(LossArgs a)
+{
+    __shared__ double sh[256];
+    const long n = (long)a.nx * a.ny * a.nz;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (v < n) {
+        const int x = (int)(v % a.nx), y = (int)((v / a.nx) % a.ny), z = (int)(v / ((long)a.nx * a.ny));
+        const size_t c = ((size_t)z * a.ny + y) * a.pitch + x;
+        const long pv = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * (a.nx + 2) + (x + 1);
+        float g0 = 0.f, g1 = 0.f;
+        if (a.mask[c]) {
+            const float d0 = a.pred[pv * 2] - a.tx[c] * a.tinv;
+            const float d1 = a.pred[pv * 2 + 1] - a.ty[c] * a.tinv;
+            l = (double)d0 * d0 + (double)d1 * d1;
+            g0 = 2.f * d0 * a.winv;
+            g1 = 2.f * d1 * a.winv;
+        }
+        uint4* o = reinterpret_cast<uint4*>(a.gout + pv * 16);
+        o[0] = make_uint4((uint32_t)f2bf(g0) | ((uint32_t)f2bf(g1) << 16), 0u, 0u, 0u);
+        o[1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    sh[threadIdx.x] = l;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.lpart[blockIdx.x] = sh[0];
+}
+
+// Adam on the FP32 master parameters (bias-corrected moments).
+__global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                                                   const float* __restrict__ g, int n, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// target = after - coarse over both components (compact layout).
+__global__ void __launch_bounds__(256) diff_kernel(const float* __restrict__ ax, const float* __restrict__ ay,
+                                                   const float* __restrict__ bx, const float* __restrict__ by,
+                                                   float* __restrict__ ox, float* __restrict__ oy, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ox[i] = ax[i] - bx[i];
+    oy[i] = ay[i] - by[i];
+}
+
+// Adds scale * pred to the pores of (mx, my) and reduces, per block, the
+// state's pore energy sum(Mx^2 + My^2), the largest |correction|, the
+// largest |M| of the state before the correction, a non-finite count, and
+// the signal sums over the trim window: part[block][6].
+__global__ void __launch_bounds__(256) apply_kernel(ApplyArgs a)
+{
+    __shared__ double sh[6][256];
+    const long n = (long)a.nx * a.ny * a.nz;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, sx = 0.0, sy = 0.0, bad = 0.0, dmax = 0.0, mmax = 0.0;
+    if (v < n) {
+        const int x = (int)(v % a.nx), y = (int)((v / a.nx) % a.ny), z = (int)(v / ((long)a.nx * a.ny));
+        const size_t c = ((size_t)z * a.ny + y) * a.pitch + x;
+        if (a.mask[c]) {
+            float mx = a.mx[c], my = a.my[c];
+            mmax = fmaxf(fabsf(mx), fabsf(my));
+            if (a.pred) {
+                const long pv = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * (a.nx + 2) + (x + 1);
+                const float d0 = a.scale * a.pred[pv * 2], d1 = a.scale * a.pred[pv * 2 + 1];
+                mx += d0;
+                my += d1;
+                a.mx[c] = mx;
+                a.my[c] = my;
+                dmax = fmaxf(fabsf(d0), fabsf(d1));
+                if (!(fabsf(d0) <= 3.0e38f) || !(fabsf(d1) <= 3.0e38f)) bad = 1.0;
+            }
+            if (!(fabsf(mx) <= 3.0e38f) || !(fabsf(my) <= 3.0e38f)) bad = 1.0;
+            e = (double)mx * mx + (double)my * my;
+            if (x >= a.bx0 && x < a.bx1 && y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1) {
+                sx = mx;
+                sy = my;
+            }
+        }
+    }
+    sh[0][threadIdx.x] = e;
+    sh[1][threadIdx.x] = dmax;
+    sh[2][threadIdx.x] = mmax;
+    sh[3][threadIdx.x] = bad;
+    sh[4][threadIdx.x] = sx;
+    sh[5][threadIdx.x] = sy;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
+            sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
+            sh[4][threadIdx.x] += sh[4][threadIdx.x + h];
+            sh[5][threadIdx.x] += sh[5][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) a.part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+} // namespace
+
+// ---- launchers ----
+
+hipError_t launch_conv(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s)
+{
+    const dim3 g(grid), b(256);
+    if (cin == 8 && mode == 0) hipLaunchKernelGGL((conv_kernel<8, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_kernel<16, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_kernel<16, 1>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_kernel<16, 2>), g, b, 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_wgrad(int cin, const WgradArgs& a, int grid, hipStream_t s)
+{
+    if (cin == 8) hipLaunchKernelGGL((wgrad_kernel<8>), dim3(grid), dim3(256), 0, s, a);
+    else if (cin == 16) hipLaunchKernelGGL((wgrad_kernel<16>), dim3(grid), dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_wreduce(const float* part, int nwaves, int cin_real, int cout_real, float* grad, hipStream_t s)
+{
+    hipLaunchKernelGGL(wreduce_kernel, dim3((kWgradPart + 255) / 256), dim3(256), 0, s, part, nwaves, cin_real,
+                       cout_real, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* w, int cin_real, int cout_real, int cin_store, int backward, uint16_t* frag,
+                       hipStream_t s)
+{
+    const int cin = backward ? 16 : cin_store;
+    const int tpc = 32 / cin, nch = (27 + tpc - 1) / tpc;
+    const int n = nch * 64 * 8;
+    hipLaunchKernelGGL(pack_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w, cin_real, cout_real, cin_store,
+                       backward, frag);
+    return hipGetLastError();
+}
+
+static int blocks_for(long n) { return (int)((n + 255) / 256); }
+
+hipError_t launch_features(const FeatureArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(feature_kernel, dim3(blocks_for((long)a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_feature_copy(const float* f6, int nx, int ny, int nz, uint16_t* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(feature_copy_kernel, dim3(blocks_for((long)nx * ny * nz)), dim3(256), 0, s, f6, nx, ny, nz,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gout_copy(const float* g2, int nx, int ny, int nz, uint16_t* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(gout_copy_kernel, dim3(blocks_for((long)nx * ny * nz)), dim3(256), 0, s, g2, nx, ny, nz,
+                       out);
+    return hipGetLastError();
+}
+
+int loss_blocks(int nx, int ny, int nz) { return blocks_for((long)nx * ny * nz); }
+
+hipError_t launch_loss(const LossArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(loss_kernel, dim3(loss_blocks(a.nx, a.ny, a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t s)
+{
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(256), 0, s, p, m, v, g, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff(const float* ax, const float* ay, const float* bx, const float* by, float* ox, float* oy,
+                       long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(diff_kernel, dim3(blocks_for(n)), dim3(256), 0, s, ax, ay, bx, by, ox, oy, n);
+    return hipGetLastError();
+}
+
+int apply_blocks(int nx, int ny, int nz) { return blocks_for((long)nx * ny * nz); }
+
+hipError_t launch_apply(const ApplyArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(apply_kernel, dim3(apply_blocks(a.nx, a.ny, a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace nmr

@@ -274,3 +274,102 @@ class BatchSolver:
         check(self.lib.nmr_hip_batch_last_run_ms(self.h, ctypes.byref(ms), ctypes.byref(n)),
               "nmr_hip_batch_last_run_ms")
         return ms.value, n.value
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Corrector:
+    """The learned corrector (nmr_hip_corrector_*): a 3-layer BF16 3-D
+    convolution net that corrects one coarse step of factor*dt toward
+    factor plain steps; trained on samples of a BatchSolver, applied by
+    run_learned on a HipSolver with a residual check and plain fallback."""
+
+    NPARAMS = _lib.NMR_CORRECTOR_PARAMS
+
+    def __init__(self, seed: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.nmr_hip_corrector_create(int(seed), ctypes.byref(h)), "nmr_hip_corrector_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            check(self.lib.nmr_hip_corrector_destroy(self.h), "nmr_hip_corrector_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.NPARAMS, dtype=np.float32)
+        check(self.lib.nmr_hip_corrector_get_params(self.h, _fptr(out)), "nmr_hip_corrector_get_params")
+        return out
+
+    @params.setter
+    def params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.shape != (self.NPARAMS,):
+            raise ValueError(f"expected {self.NPARAMS} parameters")
+        check(self.lib.nmr_hip_corrector_set_params(self.h, _fptr(a)), "nmr_hip_corrector_set_params")
+
+    def set_scales(self, vscale: float, out_scale: float) -> None:
+        check(self.lib.nmr_hip_corrector_set_scales(self.h, float(vscale), float(out_scale)),
+              "nmr_hip_corrector_set_scales")
+
+    def scales(self):
+        v, o = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.nmr_hip_corrector_get_scales(self.h, ctypes.byref(v), ctypes.byref(o)),
+              "nmr_hip_corrector_get_scales")
+        return v.value, o.value
+
+    def forward(self, features: np.ndarray) -> np.ndarray:
+        """features (nz, ny, nx, 6) -> output (nz, ny, nx, 2)."""
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        nz, ny, nx, c = f.shape
+        assert c == 6
+        out = np.empty((nz, ny, nx, 2), dtype=np.float32)
+        check(self.lib.nmr_hip_corrector_forward(self.h, nx, ny, nz, _fptr(f), _fptr(out)),
+              "nmr_hip_corrector_forward")
+        return out
+
+    def backward(self, features: np.ndarray, grad_out: np.ndarray) -> np.ndarray:
+        """Gradient of sum(grad_out * forward(features)) w.r.t. the parameters."""
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        g = np.ascontiguousarray(grad_out, dtype=np.float32)
+        nz, ny, nx, _ = f.shape
+        out = np.empty(self.NPARAMS, dtype=np.float32)
+        check(self.lib.nmr_hip_corrector_backward(self.h, nx, ny, nz, _fptr(f), _fptr(g), _fptr(out)),
+              "nmr_hip_corrector_backward")
+        return out
+
+    def train(self, batch: "BatchSolver", factor: int = 8, rounds: int = 10, stride: int = 0, lr: float = 1e-3):
+        """Returns the (rounds, ninst) losses."""
+        o = _lib.NmrTrainOptsC()
+        o.factor, o.rounds, o.stride, o.lr = int(factor), int(rounds), int(stride), float(lr)
+        losses = np.empty((rounds, batch.ninst))
+        check(self.lib.nmr_hip_corrector_train(self.h, batch.h, ctypes.byref(o), _dptr(losses)),
+              "nmr_hip_corrector_train")
+        return losses
+
+    def run_learned(self, solver: "HipSolver", ncoarse: int, factor: int = 8, grad_sched=None, tol: float = 1e-3):
+        """Returns (signal (ncoarse+1, 2), fallbacks)."""
+        g = None if grad_sched is None else np.ascontiguousarray(grad_sched, dtype=np.float64)
+        if g is not None and g.shape != (ncoarse * factor, 3):
+            raise ValueError("grad_sched must have shape (ncoarse*factor, 3)")
+        sig = np.empty((ncoarse + 1, 2))
+        nf = ctypes.c_int64()
+        check(self.lib.nmr_hip_run_learned(solver.ctx, self.h, int(ncoarse), int(factor), _dptr(g), float(tol),
+                                           _dptr(sig), ctypes.byref(nf)), "nmr_hip_run_learned")
+        return sig, nf.value

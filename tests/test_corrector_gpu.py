@@ -1,0 +1,201 @@
+"""The learned corrector on the GPU: the BF16 MFMA convolution kernels
+(forward, input gradient, weight gradient) against the FP64 restatement
+with BF16 rounding emulated (oracle/corrector.py) and against the torch
+fixture; training on batched-solver samples (loss decreases, bitwise
+repeatable); the learned solve's residual check and plain fallback; and
+what the trained net gains over the coarse step without it."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import corrector as oc
+from initializator_amd import generators
+from initializator_amd.solver import BatchSolver, Corrector, HipSolver
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def fix():
+    return dict(np.load(os.path.join(HERE, "golden", "corrector_fixture.npz")))
+
+
+@pytest.mark.parametrize("shape,seed", [((7, 9, 10), 0), ((13, 11, 37), 1), ((3, 4, 5), 2), ((6, 2, 70), 3)])
+def test_forward_against_emulating_oracle(shape, seed):
+    rng = np.random.default_rng(seed)
+    params = oc.random_params(seed)
+    feats = rng.normal(0.0, 1.0, shape + (6,))
+    with Corrector() as net:
+        net.params = params.astype(np.float32)
+        got = net.forward(feats)
+    want = oc.forward(params.astype(np.float32), feats.astype(np.float32), emulate=True)
+    e = rel_l2(got, want)
+    print(f"[corrector] forward {shape}: relL2 vs emulating oracle {e:.2e}", flush=True)
+    assert e < 1e-3
+
+
+def test_forward_and_backward_against_torch_fixture(fix):
+    with Corrector() as net:
+        net.params = fix["params"].astype(np.float32)
+        out = net.forward(fix["features"])
+        g = net.backward(fix["features"], fix["grad_out"])
+    e_out, e_g = rel_l2(out, fix["out"]), rel_l2(g, fix["grad_params"])
+    p32 = fix["params"].astype(np.float32)
+    g_emul = oc.backward(p32, fix["features"].astype(np.float32), fix["grad_out"].astype(np.float32))
+    e_emul = rel_l2(g, g_emul)
+    print(f"[corrector] vs torch FP64: output relL2 {e_out:.2e}, gradient relL2 {e_g:.2e}; "
+          f"gradient vs BF16-emulating oracle {e_emul:.2e}", flush=True)
+    # BF16 storage of weights, activations and propagated gradients moves the
+    # result off exact FP64 by the amount the emulating oracle shows (~5e-2
+    # for this fixture's gradient); against that oracle the kernels agree closely
+    assert e_out < 2e-2 and e_g < 0.1 and e_emul < 1e-3
+
+
+@pytest.mark.parametrize("shape,seed", [((7, 9, 10), 4), ((9, 6, 41), 5)])
+def test_backward_against_emulating_oracle(shape, seed):
+    rng = np.random.default_rng(seed)
+    params = oc.random_params(seed).astype(np.float32)
+    feats = rng.normal(0.0, 1.0, shape + (6,)).astype(np.float32)
+    gout = rng.normal(0.0, 1.0, shape + (2,)).astype(np.float32)
+    with Corrector() as net:
+        net.params = params
+        got = net.backward(feats, gout)
+    want = oc.backward(params, feats, gout, emulate=True)
+    for name, lo, hi in [("W1", 0, 2592), ("b1", 2592, 2608), ("W2", 2608, 9520), ("b2", 9520, 9536),
+                         ("W3", 9536, 10400), ("b3", 10400, 10402)]:
+        e = rel_l2(got[lo:hi], want[lo:hi])
+        print(f"[corrector] backward {shape} {name}: relL2 {e:.2e}", flush=True)
+        assert e < 5e-3, name
+
+
+DX, D = 1e-6, 2.3e-9
+DT = generators.stable_dt(DX, D)
+
+
+def _instances(n, seed0=100):
+    out = []
+    for i in range(n):
+        r = np.random.default_rng(seed0 + i)
+        shp = tuple(int(v) for v in r.integers(18, 34, 3))
+        phi = float(r.uniform(0.2, 0.4))
+        mask = generators.porosity(shp, phi, seed0 + i, sigma=2.0)
+        vel = generators.velocity(mask, seed0 + i, sigma=2.0, vmax=1e-3, d=(DX, DX, DX))
+        out.append(dict(shape=shp, mask=mask, vel=vel, rho1=float(r.uniform(1e-6, 50e-6)),
+                        grad_amp=float(r.uniform(0.0, 0.3)), delta=2e-3, big_delta=4e-3))
+    for s in out:
+        s["rho2"] = s["rho1"]
+    return out
+
+
+def _batch(specs):
+    b = BatchSolver(specs, 2, DX, DT, D, 0.5, r1_bulk=1 / 2.5e10, velocity=True, precession=True, phase_exact=True)
+    for i, s in enumerate(specs):
+        b.upload(i, s["mask"], [s["mask"].astype(np.float64), np.zeros(s["shape"])], s["vel"])
+    return b
+
+
+def _train(seed=7, rounds=24, ninst=6):
+    specs = _instances(ninst)
+    with _batch(specs) as b:
+        net = Corrector(seed)
+        losses = net.train(b, factor=8, rounds=rounds, stride=8, lr=2e-3)
+    return net, losses
+
+
+def test_training_reduces_loss_and_repeats_bitwise():
+    net, losses = _train()
+    first, last = losses[:4].mean(), losses[-4:].mean()
+    print(f"[corrector] training loss {first:.3e} -> {last:.3e}", flush=True)
+    assert np.all(np.isfinite(losses)) and last < 0.5 * first
+    net2, losses2 = _train()
+    assert np.array_equal(net.params, net2.params)
+    assert np.array_equal(losses, losses2)
+    net.close()
+    net2.close()
+
+
+def _solver(case_seed=900, shape=(30, 28, 32)):
+    mask = generators.porosity(shape, 0.3, case_seed, sigma=2.0)
+    vel = generators.velocity(mask, case_seed, sigma=2.0, vmax=1e-3, d=(DX, DX, DX))
+    s = HipSolver(shape, 2, DX, DT, D, 0.5, r1_bulk=1 / 2.5e10, rho1=20e-6, rho2=20e-6, velocity=True,
+                  precession=True, phase_exact=True)
+    s.upload_mask(mask)
+    s.upload_velocity(*vel)
+    s.upload_magnetization(mask.astype(np.float64), np.zeros(shape))
+    return s, mask, vel
+
+
+def _plain_signal(nsteps, sched, factor):
+    s, _, _ = _solver()
+    sig = s.run(nsteps, sched)
+    f = s.download_magnetization()
+    s.close()
+    return sig[::factor], f
+
+
+def test_fallback_reproduces_the_plain_solve():
+    """A corrector whose output is NaN fails every residual check; every
+    coarse step is redone as plain steps and the result is the plain
+    solve's (fields bit for bit)."""
+    ncoarse, factor = 6, 8
+    sched = generators.pgse_schedule(ncoarse * factor, DT, 0.2, 2e-4, 6e-4)
+    want_sig, want_f = _plain_signal(ncoarse * factor, sched, factor)
+    with Corrector(1) as net:
+        p = net.params
+        p[-1] = np.nan
+        net.params = p
+        net.set_scales(1e-3, 1e-3)
+        s, _, _ = _solver()
+        sig, nf = net.run_learned(s, ncoarse, factor, sched, tol=1e-3)
+        f = s.download_magnetization()
+        s.close()
+    assert nf == ncoarse
+    # fields bit for bit; the signal sums are the same values summed in another order
+    for a, b in zip(f, want_f):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_allclose(sig, want_sig, rtol=1e-12, atol=1e-12 * np.abs(want_sig).max())
+
+
+def test_zero_tolerance_always_falls_back():
+    ncoarse, factor = 3, 8
+    with Corrector(2) as net:
+        net.set_scales(1e-3, 1e-3)
+        s, _, _ = _solver()
+        _, nf = net.run_learned(s, ncoarse, factor, None, tol=1e-30)
+        s.close()
+    assert nf == ncoarse
+
+
+def test_trained_corrector_gains_over_the_coarse_step_alone():
+    """On a held-out porous case: the trained net keeps the learned solve's
+    signal close to the plain solve; the coarse step alone (a net whose
+    output is zero, every step accepted) drifts far more or blows up."""
+    net, _ = _train(rounds=32)
+    ncoarse, factor = 24, 8
+    sched = generators.pgse_schedule(ncoarse * factor, DT, 0.2, 1e-3, 3e-3)
+    want, _ = _plain_signal(ncoarse * factor, sched, factor)
+    s, _, _ = _solver()
+    sig, nf = net.run_learned(s, ncoarse, factor, sched, tol=1e-2)
+    s.close()
+    e_learn = rel_l2(sig, want)
+    zero = Corrector(3)
+    p = zero.params
+    p[9536:] = 0.0
+    zero.params = p
+    zero.set_scales(*net.scales())
+    s, _, _ = _solver()
+    sig0, nf0 = zero.run_learned(s, ncoarse, factor, sched, tol=1e30)
+    s.close()
+    e_coarse = rel_l2(sig0, want) if np.all(np.isfinite(sig0)) else np.inf
+    print(f"[corrector] held-out 30x28x32, {ncoarse} coarse steps of 8: learned relL2 {e_learn:.2e} "
+          f"({nf} fallbacks), coarse step alone relL2 {e_coarse:.2e} ({nf0} fallbacks)", flush=True)
+    assert e_learn < 1e-2
+    assert e_coarse > 3 * e_learn
+    net.close()
+    zero.close()
