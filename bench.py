@@ -194,6 +194,102 @@ def batch_main(args) -> int:
     return 0
 
 
+def learned_main(args) -> int:
+    """Learned corrector at 256^3: train on the batched workload's instances,
+    then the time to produce the signal curve with coarse steps of 8 dt
+    (corrector, residual check, plain fallback) against the plain solve of
+    the same problem, and the curves' relative L2 difference."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from initializator_amd import generators
+    from initializator_amd.solver import BatchSolver, Corrector, HipSolver
+
+    factor = 8
+    workers = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(workers) as ex:  # before any GPU call in this process
+        specs = list(ex.map(_batch_instance, range(args.instances)))
+    case = generators.case_flow(n=args.n, phi=args.phi, nsteps=args.steps)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] generated {len(specs)} training instances and the {args.n}^3 case in {gen_s:.1f} s",
+          file=sys.stderr, flush=True)
+    dx, D = 1e-6, 2.3e-9
+    dt = generators.stable_dt(dx, D)
+    # training on the batched generator's trajectories
+    b = BatchSolver(specs, 2, dx, dt, D, 1.0 / 2.0, r1_bulk=1.0 / 2.5e10, velocity=True, precession=True,
+                    phase_exact=True)
+    for i, sp in enumerate(specs):
+        b.upload(i, sp["mask"], [sp["mask"].astype(np.float64), np.zeros(sp["shape"])], sp["vel"])
+    net = Corrector(args.seed)
+    t0 = time.perf_counter()
+    losses = net.train(b, factor=factor, rounds=args.train_rounds, stride=args.train_stride, lr=args.lr)
+    train_s = time.perf_counter() - t0
+    b.close()
+    print(f"[bench] trained on {losses.size} samples in {train_s:.1f} s: loss {losses[:2].mean():.3e} -> "
+          f"{losses[-2:].mean():.3e}", file=sys.stderr, flush=True)
+
+    ncoarse = args.steps // factor
+    nsteps = ncoarse * factor
+    sched = case.grad_sched[:nsteps]
+
+    def solver():
+        s = HipSolver.from_case(case, phase_exact=True)
+        s.upload_magnetization(case.mask.astype(np.float64), np.zeros(case.shape))
+        return s
+
+    # warm both paths once on a short run, then time each on a fresh state
+    s = solver()
+    s.run(2 * factor, sched[: 2 * factor])
+    net.run_learned(s, 2, factor, sched[: 2 * factor], tol=args.tol)
+    s.close()
+    s = solver()
+    t0 = time.perf_counter()
+    plain = s.run(nsteps, sched)
+    plain_s = time.perf_counter() - t0
+    s.close()
+    s = solver()
+    t0 = time.perf_counter()
+    sig, why = net.run_learned(s, ncoarse, factor, sched, tol=args.tol, reasons=True)
+    nfall = why[0]
+    learned_s = time.perf_counter() - t0
+    s.close()
+    ref = plain[::factor]
+    err = float(np.linalg.norm(sig - ref) / np.linalg.norm(ref))
+    net.close()
+    out = {
+        "metric": "learned vs plain time-to-signal @1e-3",
+        "value": plain_s / learned_s,
+        "unit": "x (plain wall-clock / learned wall-clock)",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 2 * factor,
+        "ms_per_step": learned_s * 1e3 / nsteps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (net) / f32 (solver)",
+        "data": "synthetic (training: batched generator instances; evaluation: seeded 256^3 porosity, "
+                "divergence-free velocity, PGSE)",
+        "config": {
+            "workload": f"learned corrector, coarse steps of {factor} dt, {args.n}^3, phi={args.phi}, flow + PGSE "
+                        "G=0.3 T/m, complex (Mx,My), exact phase",
+            "plain_s": plain_s,
+            "learned_s": learned_s,
+            "signal_rel_l2": err,
+            "within_1e-3": err <= 1e-3,
+            "fallbacks": nfall,
+            "fallbacks_nonfinite_energy_size": list(why[1:]),
+            "coarse_steps": ncoarse,
+            "tol": args.tol,
+            "train_samples": int(losses.size),
+            "train_s": train_s,
+            "train_loss_first_last": [float(losses[:2].mean()), float(losses[-2:].mean())],
+        },
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,8 +301,14 @@ def main() -> int:
     ap.add_argument("--lz", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "batch"), default="plain",
-                    help="plain: one 256^3 grid (default); batch: many uneven instances per launch")
+    ap.add_argument("--workload", choices=("plain", "batch", "learned"), default="plain",
+                    help="plain: one 256^3 grid (default); batch: many uneven instances per launch; "
+                         "learned: corrector training and learned-vs-plain time to signal")
+    ap.add_argument("--train-rounds", type=int, default=40, help="learned workload: training rounds")
+    ap.add_argument("--train-stride", type=int, default=120, help="learned workload: plain steps between rounds")
+    ap.add_argument("--lr", type=float, default=1e-3, help="learned workload: Adam step size")
+    ap.add_argument("--tol", type=float, default=1e-3, help="learned workload: residual tolerance")
+    ap.add_argument("--seed", type=int, default=0, help="learned workload: weight seed")
     ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--graph-steps", type=int, default=64, help="batch workload: steps per captured graph")
     ap.add_argument("--full-grid", action="store_true",
@@ -224,6 +326,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return batch_main(args)
+    if args.workload == "learned":
+        return learned_main(args)
 
     from initializator_amd import generators
     from initializator_amd.solver import HipSolver

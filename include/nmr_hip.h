@@ -260,10 +260,11 @@ int nmr_hip_corrector_train(nmr_corrector* net, nmr_batch* b, const nmr_train_op
  * 2, exact phase, upwind, zero-derivative boundaries).  A step is accepted
  * when nothing is non-finite, the pore energy sum(Mx^2 + My^2) has not
  * grown by more than a factor (1 + tol), and no correction exceeds tol
- * times the largest |M|; otherwise it is redone as factor plain steps.
- * grad_sched (nullable): [ncoarse*factor][3] z gradients per plain step.
- * signal (nullable): [ncoarse+1][2] trimmed pore sums after each coarse step.
- * fallbacks (nullable): number of coarse steps redone. */
+ * times the largest |M| of the previous state; otherwise it is redone as
+ * factor plain steps.  grad_sched (nullable): [ncoarse*factor][3] z gradients per plain
+ * step.  signal (nullable): [ncoarse+1][2] trimmed pore sums after each
+ * coarse step.  fallbacks (nullable): [4] = coarse steps redone, and of
+ * those the ones that failed on a non-finite value, on energy, on size. */
 int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* net, int64_t ncoarse, int32_t factor, const double* grad_sched,
                         double tol, double* signal, int64_t* fallbacks);
 

@@ -477,6 +477,7 @@ int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout)
     HIP_TRY(hipMemcpyAsync(ctx->mask, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->has_mask = true;
+    ctx->mask_version += 1;
     ctx->inert_dirty = true;
     return NMR_OK;
 }

@@ -55,6 +55,7 @@ struct nmr_ctx {
     size_t dsignal_cap = 0;  // doubles
     int cur = 0;
     bool has_mask = false, has_mag = false;
+    int64_t mask_version = 0; // bumped by every mask upload
     bool inert_dirty = true; // mask or velocity changed since the last check
     int inert = 0;           // no pore-solid face carries velocity
     int* dflag = nullptr;

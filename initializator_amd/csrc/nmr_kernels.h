@@ -198,6 +198,8 @@ hipError_t launch_diff(const float* ax, const float* ay, const float* bx, const 
                        long n, hipStream_t s);
 int apply_blocks(int nx, int ny, int nz);
 hipError_t launch_apply(const ApplyArgs& a, hipStream_t s);
+hipError_t launch_stats_reduce(const double* part, int nblocks, double* out, hipStream_t s);
+hipError_t launch_row_flags(const uint8_t* mask, int nx, int ny, int nz, int pitch, uint8_t* flag, hipStream_t s);
 
 int num_variants();
 // steps: time steps per pass (1, or 2 for the temporally blocked sweep)

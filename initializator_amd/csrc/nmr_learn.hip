@@ -17,8 +17,9 @@
 // Inference: each coarse step is checked before it is accepted -- no
 // non-finite value, pore energy sum(Mx^2 + My^2) not above (1 + tol) times
 // the previous state's (the exact-rotation update only dissipates), and no
-// correction larger than tol times the largest |M| -- and otherwise replaced
-// by factor plain steps from the kept state.
+// correction larger than tol times the largest |M| of the previous state --
+// and otherwise replaced by factor plain steps from the kept state.  The net
+// computes only the rows its pore outputs depend on.
 #include <random>
 
 #include "nmr_internal.h"
@@ -31,7 +32,7 @@ constexpr int kParams = 10402;
 // canonical layout: W1 [16][6][27], b1 [16], W2 [16][16][27], b2 [16], W3 [2][16][27], b3 [2]
 constexpr int kW1 = 0, kB1 = 2592, kW2 = 2608, kB2 = 9520, kW3 = 9536, kB3 = 10400;
 constexpr int kFragF1 = 7 * 512, kFragF = 14 * 512; // uint16 per fragment set (cin 8 / 16)
-constexpr int kWgradGrid = 512;                     // workgroups of the weight-gradient kernel
+constexpr int kWgradGrid = 128;                     // workgroups of the weight-gradient kernel
 
 struct Tensors {
     int nx = 0, ny = 0, nz = 0;
@@ -54,7 +55,13 @@ struct nmr_corrector {
     size_t lpart_cap = 0;
     double* apart = nullptr; // apply partials
     size_t apart_cap = 0;
+    double* dstat = nullptr; // reduced statistics [6]
     bool train_ws = false;
+    // rows of the three layers that the pores of one context need
+    const nmr_ctx* rows_ctx = nullptr;
+    int64_t rows_version = -1;
+    int* rows[3] = {nullptr, nullptr, nullptr};
+    int nrows[3] = {0, 0, 0};
 };
 
 namespace {
@@ -116,8 +123,10 @@ int repack(nmr_corrector* c)
 
 int conv_grid(int nrows) { return std::max(1, std::min(2048, (nrows + 3) / 4)); }
 
-// Forward pass on the tensors (x already filled): a1, a2, p.
-int forward(nmr_corrector* c)
+// Forward pass on the tensors (x already filled): a1, a2, p.  With row
+// lists (sparse = true), layer l computes only the rows c->rows[l-1]; the
+// lists are nested so that every row a later layer reads is computed.
+int forward(nmr_corrector* c, bool sparse = false)
 {
     const Tensors& t = c->t;
     nmr::ConvArgs a{};
@@ -125,24 +134,82 @@ int forward(nmr_corrector* c)
     a.ny = t.ny;
     a.nz = t.nz;
     a.nseg = (t.nx + 15) / 16;
-    a.nrows = a.nseg * t.ny * t.nz;
-    const int grid = conv_grid(a.nrows);
+    const int all = a.nseg * t.ny * t.nz;
+    auto rows = [&](int l) {
+        a.rows = sparse ? c->rows[l] : nullptr;
+        a.nrows = sparse ? c->nrows[l] : all;
+        return conv_grid(a.nrows);
+    };
     a.in = t.x;
     a.wa = c->f1;
     a.bias = c->w + kB1;
     a.out = t.a1;
-    HIP_TRY(nmr::launch_conv(8, 0, a, grid, c->stream));
+    int grid = rows(0);
+    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(8, 0, a, grid, c->stream));
     a.in = t.a1;
     a.wa = c->f2;
     a.bias = c->w + kB2;
     a.out = t.a2;
-    HIP_TRY(nmr::launch_conv(16, 0, a, grid, c->stream));
+    grid = rows(1);
+    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 0, a, grid, c->stream));
     a.in = t.a2;
     a.wa = c->f3;
     a.bias = c->w + kB3;
     a.out = nullptr;
     a.outf = t.p;
-    HIP_TRY(nmr::launch_conv(16, 1, a, grid, c->stream));
+    grid = rows(2);
+    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 1, a, grid, c->stream));
+    return NMR_OK;
+}
+
+// Row lists of a context's mask: layer 3 computes the 16-voxel rows that
+// hold a pore, layer 2 those rows dilated by one row in y, z and one
+// segment in x, layer 1 that set dilated once more.
+int build_rows(nmr_corrector* c, nmr_ctx* ctx)
+{
+    if (c->rows_ctx == ctx && c->rows_version == ctx->mask_version) return NMR_OK;
+    const nmr_problem& p = ctx->prob;
+    const int nseg = (p.nx + 15) / 16;
+    const size_t n = (size_t)nseg * p.ny * p.nz;
+    uint8_t* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, std::max<size_t>(n, 256)));
+    std::vector<uint8_t> f(n);
+    hipError_t e = nmr::launch_row_flags(ctx->mask, p.nx, p.ny, p.nz, ctx->pitch, d, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(f.data(), d, n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(NMR_ERR_DEVICE, "row flags: %s", hipGetErrorString(e));
+    auto dilate = [&](const std::vector<uint8_t>& in) {
+        std::vector<uint8_t> a(in), b(n);
+        const size_t st[3] = {1, (size_t)nseg, (size_t)nseg * p.ny};
+        const int ext[3] = {nseg, p.ny, p.nz};
+        for (int ax = 0; ax < 3; ax++) {
+            for (size_t i = 0; i < n; i++) {
+                const int coord = (int)((i / st[ax]) % ext[ax]);
+                uint8_t v = a[i];
+                if (coord > 0) v |= a[i - st[ax]];
+                if (coord + 1 < ext[ax]) v |= a[i + st[ax]];
+                b[i] = v;
+            }
+            a.swap(b);
+        }
+        return a;
+    };
+    std::vector<uint8_t> l3 = f, l2 = dilate(l3), l1 = dilate(l2);
+    const std::vector<uint8_t>* sets[3] = {&l1, &l2, &l3};
+    for (int l = 0; l < 3; l++) {
+        std::vector<int> ids;
+        for (size_t i = 0; i < n; i++)
+            if ((*sets[l])[i]) ids.push_back((int)i);
+        if (c->rows[l]) HIP_TRY(hipFree(c->rows[l]));
+        c->rows[l] = nullptr;
+        int rc = dalloc(&c->rows[l], ids.size());
+        if (rc) return rc;
+        if (!ids.empty()) HIP_TRY(hipMemcpy(c->rows[l], ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+        c->nrows[l] = (int)ids.size();
+    }
+    c->rows_ctx = ctx;
+    c->rows_version = ctx->mask_version;
     return NMR_OK;
 }
 
@@ -278,7 +345,7 @@ int nmr_hip_corrector_create(uint64_t seed, nmr_corrector** out)
     if ((rc = dalloc(&c->w, kParams)) || (rc = dalloc(&c->m, kParams)) || (rc = dalloc(&c->v, kParams)) ||
         (rc = dalloc(&c->grad, kParams)) || (rc = dalloc(&c->f1, kFragF1)) || (rc = dalloc(&c->f2, kFragF)) ||
         (rc = dalloc(&c->f3, kFragF)) || (rc = dalloc(&c->d2, kFragF)) || (rc = dalloc(&c->d3, kFragF)) ||
-        (rc = dalloc(&c->wpart, (size_t)kWgradGrid * 4 * nmr::kWgradPart)))
+        (rc = dalloc(&c->wpart, (size_t)kWgradGrid * 4 * nmr::kWgradPart)) || (rc = dalloc(&c->dstat, 8)))
         return bail(rc);
     // He-normal weights (last layer scaled by 0.1), zero biases
     std::mt19937_64 gen(seed);
@@ -304,7 +371,8 @@ int nmr_hip_corrector_destroy(nmr_corrector* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     tfree(c->t);
     for (void* q : {(void*)c->w, (void*)c->m, (void*)c->v, (void*)c->grad, (void*)c->f1, (void*)c->f2, (void*)c->f3,
-                    (void*)c->d2, (void*)c->d3, (void*)c->wpart, (void*)c->lpart, (void*)c->apart})
+                    (void*)c->d2, (void*)c->d3, (void*)c->wpart, (void*)c->lpart, (void*)c->apart, (void*)c->dstat,
+                    (void*)c->rows[0], (void*)c->rows[1], (void*)c->rows[2]})
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -580,7 +648,7 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
     if (rc) return rc;
     const int nab = nmr::apply_blocks(p.nx, p.ny, p.nz);
     if ((rc = ensure_dev_doubles(&c->apart, &c->apart_cap, (size_t)nab * 6))) return rc;
-    std::vector<double> hp((size_t)nab * 6);
+    if ((rc = build_rows(c, ctx))) return rc;
     nmr::ApplyArgs ap{};
     ap.mask = ctx->mask;
     ap.part = c->apart;
@@ -598,17 +666,9 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
         ap.my = ctx->m[buf][1];
         ap.pred = pred;
         HIP_TRY(nmr::launch_apply(ap, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(hp.data(), c->apart, hp.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(nmr::launch_stats_reduce(c->apart, nab, c->dstat, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(st, c->dstat, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (int k = 0; k < 6; k++) st[k] = 0.0;
-        for (int bk = 0; bk < nab; bk++) {
-            st[0] += hp[bk * 6 + 0];
-            st[1] = std::max(st[1], hp[bk * 6 + 1]);
-            st[2] = std::max(st[2], hp[bk * 6 + 2]);
-            st[3] += hp[bk * 6 + 3];
-            st[4] += hp[bk * 6 + 4];
-            st[5] += hp[bk * 6 + 5];
-        }
         return NMR_OK;
     };
     // the net's stream waits on the context's stream through explicit syncs
@@ -619,7 +679,7 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
         signal[0] = cur[4];
         signal[1] = cur[5];
     }
-    int64_t nfall = 0;
+    int64_t nfall = 0, why[3] = {0, 0, 0};
     std::vector<double> sig8((size_t)(factor + 1) * 2);
     const float vinv = (float)(1.0 / c->vscale);
     for (int64_t k = 0; k < ncoarse; k++) {
@@ -653,17 +713,18 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
         // the net runs on the context's stream
         hipStream_t keep_stream = c->stream;
         c->stream = ctx->stream;
-        rc = forward(c);
+        rc = forward(c, true);
         c->stream = keep_stream;
         if (rc) return rc;
         double nxt[6];
         if ((rc = stats(out, c->t.p, nxt))) return rc;
-        const bool ok = nxt[3] == 0.0 && nxt[0] <= cur[0] * (1.0 + tol) && nxt[1] <= tol * cur[2];
-        if (ok) {
+        const bool finite = nxt[3] == 0.0, energy = nxt[0] <= cur[0] * (1.0 + tol), size = nxt[1] <= tol * cur[2];
+        if (finite && energy && size) {
             ctx->cur = out;
             std::copy(nxt, nxt + 6, cur);
         } else {
             nfall += 1;
+            why[!finite ? 0 : (!size ? 2 : 1)] += 1;
             // plain steps from the kept state (buffer `in` is untouched)
             std::vector<double> gs;
             if (grad_sched) gs.assign(grad_sched + 3 * k * factor, grad_sched + 3 * (k + 1) * factor);
@@ -675,7 +736,12 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
             signal[2 * (k + 1) + 1] = cur[5];
         }
     }
-    if (fallbacks) *fallbacks = nfall;
+    if (fallbacks) {
+        fallbacks[0] = nfall;
+        fallbacks[1] = why[0];
+        fallbacks[2] = why[1];
+        fallbacks[3] = why[2];
+    }
     return NMR_OK;
 }
 

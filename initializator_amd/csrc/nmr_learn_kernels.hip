@@ -389,6 +389,53 @@ __global__ void __launch_bounds__(256) apply_kernel(ApplyArgs a)
     if (threadIdx.x < 6) a.part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// Final reduction of apply_kernel's per-block statistics into out[6] (sums
+// for entries 0, 3, 4, 5, maxima for 1, 2), one workgroup, fixed order.
+__global__ void __launch_bounds__(256) stats_reduce_kernel(const double* __restrict__ part, int nblocks,
+                                                           double* __restrict__ out)
+{
+    __shared__ double sh[6][256];
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        const double* q = part + (size_t)b * 6;
+        v[0] += q[0];
+        v[1] = fmax(v[1], q[1]);
+        v[2] = fmax(v[2], q[2]);
+        v[3] += q[3];
+        v[4] += q[4];
+        v[5] += q[5];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
+            sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
+            sh[4][threadIdx.x] += sh[4][threadIdx.x + h];
+            sh[5][threadIdx.x] += sh[5][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// flag[row] = 1 when the 16-voxel x segment `row` holds a pore.
+__global__ void __launch_bounds__(256) row_flag_kernel(const uint8_t* __restrict__ mask, int nx, int ny, int nz,
+                                                       int pitch, int nseg, uint8_t* __restrict__ flag)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long)nseg * ny * nz) return;
+    const int seg = (int)(r % nseg);
+    const long yz = r / nseg;
+    const uint8_t* m = mask + (size_t)yz * pitch;
+    uint8_t f = 0;
+    for (int x = seg * 16; x < min(seg * 16 + 16, nx); x++) f |= m[x];
+    flag[r] = f ? 1 : 0;
+}
+
 } // namespace
 
 // ---- launchers ----
@@ -479,6 +526,25 @@ int apply_blocks(int nx, int ny, int nz) { return blocks_for((long)nx * ny * nz)
 hipError_t launch_apply(const ApplyArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(apply_kernel, dim3(apply_blocks(a.nx, a.ny, a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace nmr
+
+namespace nmr {
+
+hipError_t launch_stats_reduce(const double* part, int nblocks, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(1), dim3(256), 0, s, part, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_flags(const uint8_t* mask, int nx, int ny, int nz, int pitch, uint8_t* flag, hipStream_t s)
+{
+    const int nseg = (nx + 15) / 16;
+    const long n = (long)nseg * ny * nz;
+    hipLaunchKernelGGL(row_flag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mask, nx, ny, nz, pitch,
+                       nseg, flag);
     return hipGetLastError();
 }
 

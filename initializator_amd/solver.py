@@ -363,13 +363,15 @@ class Corrector:
               "nmr_hip_corrector_train")
         return losses
 
-    def run_learned(self, solver: "HipSolver", ncoarse: int, factor: int = 8, grad_sched=None, tol: float = 1e-3):
-        """Returns (signal (ncoarse+1, 2), fallbacks)."""
+    def run_learned(self, solver: "HipSolver", ncoarse: int, factor: int = 8, grad_sched=None, tol: float = 1e-3,
+                    reasons: bool = False):
+        """Returns (signal (ncoarse+1, 2), fallbacks) -- with reasons=True,
+        fallbacks is (total, non-finite, energy, size)."""
         g = None if grad_sched is None else np.ascontiguousarray(grad_sched, dtype=np.float64)
         if g is not None and g.shape != (ncoarse * factor, 3):
             raise ValueError("grad_sched must have shape (ncoarse*factor, 3)")
         sig = np.empty((ncoarse + 1, 2))
-        nf = ctypes.c_int64()
+        nf = (ctypes.c_int64 * 4)()
         check(self.lib.nmr_hip_run_learned(solver.ctx, self.h, int(ncoarse), int(factor), _dptr(g), float(tol),
-                                           _dptr(sig), ctypes.byref(nf)), "nmr_hip_run_learned")
-        return sig, nf.value
+                                           _dptr(sig), nf), "nmr_hip_run_learned")
+        return sig, (tuple(nf) if reasons else nf[0])

@@ -152,24 +152,29 @@ def test_fallback_reproduces_the_plain_solve():
         net.params = p
         net.set_scales(1e-3, 1e-3)
         s, _, _ = _solver()
-        sig, nf = net.run_learned(s, ncoarse, factor, sched, tol=1e-3)
+        sig, nf = net.run_learned(s, ncoarse, factor, sched, tol=1e-3, reasons=True)
         f = s.download_magnetization()
         s.close()
-    assert nf == ncoarse
+    assert nf == (ncoarse, ncoarse, 0, 0)
     # fields bit for bit; the signal sums are the same values summed in another order
     for a, b in zip(f, want_f):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_allclose(sig, want_sig, rtol=1e-12, atol=1e-12 * np.abs(want_sig).max())
 
 
-def test_zero_tolerance_always_falls_back():
+def test_oversized_corrections_fall_back():
+    """A correction larger than tol times the field (output bias 1000 in
+    units of out_scale 1) fails the size check on every step."""
     ncoarse, factor = 3, 8
     with Corrector(2) as net:
-        net.set_scales(1e-3, 1e-3)
+        p = net.params
+        p[10400:] = 1000.0
+        net.params = p
+        net.set_scales(1e-3, 1.0)
         s, _, _ = _solver()
-        _, nf = net.run_learned(s, ncoarse, factor, None, tol=1e-30)
+        _, why = net.run_learned(s, ncoarse, factor, None, tol=1e-3, reasons=True)
         s.close()
-    assert nf == ncoarse
+    assert why == (ncoarse, 0, 0, ncoarse)
 
 
 def test_trained_corrector_gains_over_the_coarse_step_alone():
