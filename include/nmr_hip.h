@@ -123,6 +123,20 @@ int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* m
 int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double* mz,
                                    int32_t layout);
 
+/* Per-voxel coefficients for mixtures and face-wise surface relaxivity
+ * (FillInDiffusinAndTime, src/NmrRelaxationSolve.cpp:1756-1893): pore
+ * diffusivity D (m^2/s) and relaxation rates r1 = 1/T1, r2 = 1/T2 (1/s,
+ * surface terms included) of every voxel (r1 may be NULL for ncomp < 3);
+ * solid voxels are ignored.  They replace diffusivity, r1/r2_bulk and
+ * rho1/rho2 of the problem from then on; diffusion between two pores uses
+ * the harmonic mean 2 Da Db / (Da + Db) (cpp:3248-3259).  Linear phase only. */
+int nmr_hip_upload_coefficients(nmr_ctx* ctx, const double* D, const double* r1, const double* r2, int32_t layout);
+/* Phase weights of the two-phase outputs (OutputX/Y/Z with NCOMP=2,
+ * cpp:3324-3350; w = c * [c1 > 0.8 or c2 > 0.8]).  With weights uploaded
+ * every signal row holds 2*ncomp sums: sum(w1*M_q) for q < ncomp, then
+ * sum(w2*M_q), over pores of the trimmed interior. */
+int nmr_hip_upload_phase_weights(nmr_ctx* ctx, const double* w1, const double* w2, int32_t layout);
+
 /* Advance nsteps time steps.  grad_sched (nullable): [nsteps][3] gradient
  * amplitudes (T/m) in force during each step; NULL uses prob->grad.
  * signal (nullable): [(nsteps+1)][ncomp] pore sums over the trimmed
@@ -145,7 +159,7 @@ int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay
 int nmr_hip_device_count(int32_t* count);
 int nmr_hip_set_device(int32_t device);
 
-/* Pore sums of the current state: out[ncomp]. */
+/* Pore sums of the current state: out[ncomp] (out[2*ncomp] with phase weights). */
 int nmr_hip_signal(nmr_ctx* ctx, double* out);
 
 /* GPU time (ms) between HIP events recorded on the context's stream around

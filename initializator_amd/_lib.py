@@ -47,6 +47,8 @@ EXPORTS = (
     "nmr_hip_upload_velocity",
     "nmr_hip_upload_magnetization",
     "nmr_hip_download_magnetization",
+    "nmr_hip_upload_coefficients",
+    "nmr_hip_upload_phase_weights",
     "nmr_hip_run",
     "nmr_hip_signal",
     "nmr_hip_pulse",
@@ -180,6 +182,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_upload_velocity": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_upload_magnetization": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_download_magnetization": (c_int, [vp, dp, dp, dp, c_int32]),
+        "nmr_hip_upload_coefficients": (c_int, [vp, dp, dp, dp, c_int32]),
+        "nmr_hip_upload_phase_weights": (c_int, [vp, dp, dp, c_int32]),
         "nmr_hip_run": (c_int, [vp, c_int64, dp, dp]),
         "nmr_hip_signal": (c_int, [vp, dp]),
         "nmr_hip_pulse": (c_int, [vp, c_int32, c_double, c_double, c_double, c_double]),

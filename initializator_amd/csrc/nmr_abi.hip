@@ -67,6 +67,8 @@ void free_all(nmr_ctx* c)
             if (c->m[b][q]) (void)hipFree(c->m[b][q]);
     for (int q = 0; q < 3; q++)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
+    for (float* q : {c->dn, c->r1v, c->r2v, c->w[0], c->w[1]})
+        if (q) (void)hipFree(q);
     if (c->mask) (void)hipFree(c->mask);
     if (c->vx) (void)hipFree(c->vx);
     if (c->vy) (void)hipFree(c->vy);
@@ -133,7 +135,7 @@ nmr::Coef make_coef(const nmr_ctx* c, const double g[3])
     const nmr_problem& p = c->prob;
     nmr::Coef k{};
     const double dt = p.dt;
-    const double D = (p.flags & NMR_F_DIFFUSION) ? p.diffusivity : 0.0;
+    const double D = (p.flags & NMR_F_DIFFUSION) ? (c->hetero ? c->d_ref : p.diffusivity) : 0.0;
     k.cdx = (float)(dt * D / (p.dx * p.dx));
     k.cdy = (float)(dt * D / (p.dy * p.dy));
     k.cdz = (float)(dt * D / (p.dz * p.dz));
@@ -208,11 +210,18 @@ int ensure_signal(nmr_ctx* c, size_t doubles)
 int enqueue_field_sum(nmr_ctx* c, size_t row)
 {
     const nmr_problem& p = c->prob;
-    int rc = ensure_partials(c, (size_t)p.nz * c->nc);
+    if ((row + 1) * (size_t)c->nsig() > c->dsignal_cap)
+        return fail(NMR_ERR_STATE, "signal buffer holds %zu values, row %zu needs %zu", c->dsignal_cap, row,
+                    (row + 1) * (size_t)c->nsig());
+    int rc = ensure_partials(c, (size_t)p.nz * c->nsig());
     if (rc) return rc;
     nmr::FieldSumArgs a{};
     for (int q = 0; q < 3; q++) a.f[q] = c->m[c->cur][q];
     a.mask = c->mask;
+    if (c->weighted) {
+        a.w[0] = c->w[0];
+        a.w[1] = c->w[1];
+    }
     a.part = c->partials;
     a.nx = p.nx;
     a.ny = p.ny;
@@ -223,7 +232,7 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
     a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
     HIP_TRY(nmr::launch_field_sum(c->nc, a, p.nz, c->stream));
     // one "step" whose nwg = nz partials
-    HIP_TRY(nmr::launch_reduce_partials(c->partials, 1, p.nz, c->nc, c->dsignal + row * c->nc, c->stream));
+    HIP_TRY(nmr::launch_reduce_partials(c->partials, 1, p.nz, c->nsig(), c->dsignal + row * c->nsig(), c->stream));
     return NMR_OK;
 }
 
@@ -232,11 +241,11 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
 int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
 {
     const nmr_problem& p = ctx->prob;
-    const int nc = ctx->nc;
+    const int nc = ctx->nc, ns = ctx->nsig();
     const int64_t rows = nsteps + 1;
-    int rc = ensure_signal(ctx, (size_t)rows * nc);
+    int rc = ensure_signal(ctx, (size_t)rows * ns);
     if (rc) return rc;
-    if ((rc = ensure_partials(ctx, (size_t)p.nz * nc))) return rc;
+    if ((rc = ensure_partials(ctx, (size_t)p.nz * ns))) return rc;
     nmr::GenArgs a{};
     a.mask = ctx->mask;
     a.vx = ctx->vx;
@@ -249,6 +258,9 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     a.pitch1 = ctx->pitch1;
     for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
     a.vel = ctx->vel ? 1 : 0;
+    a.dn = ctx->dn;
+    a.r1v = ctx->r1v;
+    a.r2v = ctx->r2v;
     const bool mak = p.scheme == NMR_SCHEME_MACCORMACK;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if ((rc = enqueue_field_sum(ctx, 0))) return rc;
@@ -272,7 +284,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     if (signal)
-        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * nc, hipMemcpyDeviceToHost,
+        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * ns, hipMemcpyDeviceToHost,
                                ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
@@ -566,6 +578,77 @@ int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* m
     return NMR_OK;
 }
 
+// Reads one per-voxel field of the problem grid (compact or padded).
+static int read_voxels(const nmr_ctx* ctx, const double* src, int32_t layout, std::vector<double>& out)
+{
+    const nmr_problem& p = ctx->prob;
+    out.assign((size_t)p.nx * p.ny * p.nz, 0.0);
+    for (int k = 0; k < p.nz; k++)
+        for (int j = 0; j < p.ny; j++)
+            for (int i = 0; i < p.nx; i++) {
+                const size_t o = (size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k);
+                const double v = layout == NMR_LAYOUT_COMPACT ? src[o] : src[pidx(p, i + 1, j + 1, k + 1)];
+                if (!std::isfinite(v)) return fail(NMR_ERR_INVALID, "non-finite value at (%d, %d, %d)", i, j, k);
+                out[o] = v;
+            }
+    return NMR_OK;
+}
+
+static int put_voxels(nmr_ctx* ctx, float** dst, const std::vector<double>& v, double scale)
+{
+    const nmr_problem& p = ctx->prob;
+    if (!*dst) {
+        int rc = dev_alloc(ctx, dst, ctx->n_m);
+        if (rc) return rc;
+    }
+    std::vector<float> h(ctx->n_m, 0.f);
+    for (int k = 0; k < p.nz; k++)
+        for (int j = 0; j < p.ny; j++)
+            for (int i = 0; i < p.nx; i++)
+                h[(size_t)k * p.ny * ctx->pitch + (size_t)j * ctx->pitch + i] =
+                    (float)(v[(size_t)i + (size_t)p.nx * ((size_t)j + (size_t)p.ny * k)] * scale);
+    HIP_TRY(hipMemcpyAsync(*dst, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return NMR_OK;
+}
+
+int nmr_hip_upload_coefficients(nmr_ctx* ctx, const double* D, const double* r1, const double* r2, int32_t layout)
+{
+    if (!ctx || !D || !r2 || (ctx->nc == 3 && !r1)) return fail(NMR_ERR_NULL, "nmr_hip_upload_coefficients: null argument");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    if (ctx->exact) return fail(NMR_ERR_UNSUPPORTED, "per-voxel coefficients use the linear phase");
+    std::vector<double> d, a, b;
+    if ((rc = read_voxels(ctx, D, layout, d)) || (rc = read_voxels(ctx, r2, layout, b))) return rc;
+    if (r1 && (rc = read_voxels(ctx, r1, layout, a))) return rc;
+    double dref = 0.0;
+    for (double v : d) {
+        if (v < 0.0) return fail(NMR_ERR_INVALID, "negative diffusivity");
+        dref = std::max(dref, v);
+    }
+    if (!(dref > 0.0)) dref = 1.0; // no diffusion anywhere
+    const double dt = ctx->prob.dt;
+    if ((rc = put_voxels(ctx, &ctx->dn, d, 1.0 / dref)) || (rc = put_voxels(ctx, &ctx->r2v, b, dt))) return rc;
+    if (r1 && (rc = put_voxels(ctx, &ctx->r1v, a, dt))) return rc;
+    ctx->d_ref = dref;
+    ctx->hetero = true;
+    ctx->general = true;
+    return NMR_OK;
+}
+
+int nmr_hip_upload_phase_weights(nmr_ctx* ctx, const double* w1, const double* w2, int32_t layout)
+{
+    if (!ctx || !w1 || !w2) return fail(NMR_ERR_NULL, "nmr_hip_upload_phase_weights: null argument");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    std::vector<double> a, b;
+    if ((rc = read_voxels(ctx, w1, layout, a)) || (rc = read_voxels(ctx, w2, layout, b))) return rc;
+    if ((rc = put_voxels(ctx, &ctx->w[0], a, 1.0)) || (rc = put_voxels(ctx, &ctx->w[1], b, 1.0))) return rc;
+    ctx->weighted = true;
+    ctx->general = true;
+    return NMR_OK;
+}
+
 int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double* mz, int32_t layout)
 {
     if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_download_magnetization: null context");
@@ -624,10 +707,10 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out)
 {
     if (!ctx || !out) return fail(NMR_ERR_NULL, "nmr_hip_signal: null argument");
     if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "signal before mask and magnetization upload");
-    int rc = ensure_signal(ctx, (size_t)ctx->nc);
+    int rc = ensure_signal(ctx, (size_t)ctx->nsig());
     if (rc) return rc;
     if ((rc = enqueue_field_sum(ctx, 0))) return rc;
-    HIP_TRY(hipMemcpyAsync(out, ctx->dsignal, sizeof(double) * ctx->nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, ctx->dsignal, sizeof(double) * ctx->nsig(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return NMR_OK;
 }

@@ -44,7 +44,13 @@ struct nmr_ctx {
     size_t n_vx = 0, n_vy = 0, n_vz = 0;
     float* m[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     float* pred[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor
-    bool general = false; // MacCormack or a non-zero-derivative boundary: per-voxel kernel
+    bool general = false; // MacCormack, non-zero-derivative boundaries or per-voxel data: per-voxel kernel
+    // per-voxel coefficients (diffusivity / d_ref, dt/T1, dt/T2) and phase weights
+    float *dn = nullptr, *r1v = nullptr, *r2v = nullptr;
+    float* w[2] = {nullptr, nullptr};
+    double d_ref = 0.0;
+    bool hetero = false, weighted = false;
+    int nsig() const { return weighted ? 2 * nc : nc; }
     uint8_t* mask = nullptr;
     float* vx = nullptr;
     float* vy = nullptr;

@@ -104,6 +104,9 @@ struct GenArgs {
     int nx, ny, nz, pitch, pitch1;
     int bc[3];              // BC_* per axis
     int vel;                // face velocities present
+    const float* dn;        // per-voxel diffusivity / reference (null: uniform, from the mask)
+    const float* r1v;       // per-voxel dt/T1 with surface terms (null: from the mask)
+    const float* r2v;       // per-voxel dt/T2 with surface terms
     Coef c;
 };
 
@@ -119,7 +122,8 @@ struct PulseArgs {
 struct FieldSumArgs {
     const float* f[3];
     const uint8_t* mask;
-    double* part;           // [nz][nc]
+    const float* w[2];      // phase weights (null: plain sums)
+    double* part;           // [nz][nc] or, with weights, [nz][2*nc]
     int nx, ny, nz, pitch;
     int bx0, bx1, by0, by1, bz0, bz1;
 };

@@ -250,6 +250,16 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
 
     // face weights on (f_neighbour - f_centre)
     float w[6] = {cf.cdx * kn[0], cf.cdx * kn[1], cf.cdy * kn[2], cf.cdy * kn[3], cf.cdz * kn[4], cf.cdz * kn[5]};
+    if (a.dn) { // per-voxel diffusivity: harmonic mean between pores; ghosts copy the centre
+        const float dc = a.dn[c];
+        const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            const float dnb = kind[s] == 0 || kind[s] == 3 ? (kind[s] == 3 ? dc : a.dn[ni[s]]) : dc;
+            const float hm = (dc + dnb) > 0.f ? 2.f * dc * dnb / (dc + dnb) : 0.f;
+            w[s] = cd[s / 2] * hm * kn[s];
+        }
+    }
     if (a.vel) {
         const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
         const size_t P1 = (size_t)a.pitch1;
@@ -266,18 +276,18 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
             w[3] = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
             w[4] = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
             w[5] = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
-        } else if constexpr (ST == GEN_PREDICT) { // + u hx (f_{i-1} - f_i)
-            w[0] = kn[0] * fmaf(hxh, su, cf.cdx);
-            w[2] = kn[2] * fmaf(hyh, sv, cf.cdy);
-            w[4] = kn[4] * fmaf(hzh, sw, cf.cdz);
+        } else if constexpr (ST == GEN_PREDICT) { // + u hx (f_{i-1} - f_i); kn is 0 or 1
+            w[0] = fmaf(kn[0] * hxh, su, w[0]);
+            w[2] = fmaf(kn[2] * hyh, sv, w[2]);
+            w[4] = fmaf(kn[4] * hzh, sw, w[4]);
         } else { // - u hx (f_{i+1} - f_i)
-            w[1] = kn[1] * fmaf(-hxh, su, cf.cdx);
-            w[3] = kn[3] * fmaf(-hyh, sv, cf.cdy);
-            w[5] = kn[5] * fmaf(-hzh, sw, cf.cdz);
+            w[1] = fmaf(-kn[1] * hxh, su, w[1]);
+            w[3] = fmaf(-kn[3] * hyh, sv, w[3]);
+            w[5] = fmaf(-kn[5] * hzh, sw, w[5]);
         }
     }
     const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
-    const float rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+    const float rate2 = a.r2v ? a.r2v[c] : fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
     const float* const* src = (ST == GEN_CORRECT) ? a.p : a.m;
     float m0[NC], d[NC];
 #pragma unroll
@@ -301,7 +311,7 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
     }
     const float thy = cf.bgy * ((float)y + 0.5f), thz = cf.bgz * ((float)z + 0.5f);
     if constexpr (NC == 3) {
-        const float rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
+        const float rate1 = a.r1v ? a.r1v[c] : fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
         const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
         const float nn[3] = {cf.n0, cf.n1, cf.n2};
 #pragma unroll
@@ -636,7 +646,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
 
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
-            if (!rin[r]) continue;
+            // all-solid chunks keep their value: no update (wavefronts whose
+            // chunks are all solid skip the arithmetic)
+            if (!rin[r] || kc[r] == 0) continue;
             const int lr = 1 + ry + r * RY;
             const int y = iy[r];
             const bool trim_y = y >= a.by0 && y < a.by1;
@@ -659,11 +671,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
                                          kn[r], vx4[r], vxe[r], vy4[r], vy4n[r], wc[r], wn[r], ix, thy, thz, sn,
                                          cm1, out);
             if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
-            if (kc[r] != 0) { // all-solid chunks keep their value in both buffers
-                const uint32_t of = (uint32_t)k * plane_b + rowf[r];
+            const uint32_t of = (uint32_t)k * plane_b + rowf[r];
 #pragma unroll
-                for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
-            }
+            for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
         }
 
         // rotate the register planes
@@ -1149,33 +1159,44 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __re
 
 // Pore sum of a field over the trimmed interior: one block per z-plane
 // writes part[k*NC + q]; nmr_reduce_partials-style second pass sums planes.
-template <int NC>
+template <int NC, bool W>
 __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
 {
+    constexpr int NS = W ? 2 * NC : NC;
     __shared__ double sh[256];
     const int k = blockIdx.x;
-    double acc[NC];
+    double acc[NS];
 #pragma unroll
-    for (int q = 0; q < NC; q++) acc[q] = 0.0;
+    for (int q = 0; q < NS; q++) acc[q] = 0.0;
     if (k >= a.bz0 && k < a.bz1) {
         const size_t base = (size_t)k * (size_t)a.ny * (size_t)a.pitch;
         for (int y = a.by0; y < a.by1; y++) {
             const size_t row = base + (size_t)y * a.pitch;
             for (int x = a.bx0 + (int)threadIdx.x; x < a.bx1; x += 256) {
                 if (a.mask[row + x] == 0) continue;
+                if constexpr (W) {
+                    const double w1 = a.w[0][row + x], w2 = a.w[1][row + x];
 #pragma unroll
-                for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
+                    for (int q = 0; q < NC; q++) {
+                        const double v = a.f[q][row + x];
+                        acc[q] += v * w1;
+                        acc[NC + q] += v * w2;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
+                }
             }
         }
     }
-    for (int q = 0; q < NC; q++) {
+    for (int q = 0; q < NS; q++) {
         sh[threadIdx.x] = acc[q];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
             if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.part[(size_t)k * NC + q] = sh[0];
+        if (threadIdx.x == 0) a.part[(size_t)k * NS + q] = sh[0];
         __syncthreads();
     }
 }
@@ -1371,12 +1392,17 @@ hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s)
 
 hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s)
 {
+    const bool w = a.w[0] != nullptr;
+#define NMR_FS(NC_)                                                                              \
+    if (w) hipLaunchKernelGGL((field_sum_kernel<NC_, true>), dim3(nz), dim3(256), 0, s, a);       \
+    else hipLaunchKernelGGL((field_sum_kernel<NC_, false>), dim3(nz), dim3(256), 0, s, a);
     switch (nc) {
-    case 1: hipLaunchKernelGGL(field_sum_kernel<1>, dim3(nz), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(field_sum_kernel<2>, dim3(nz), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(field_sum_kernel<3>, dim3(nz), dim3(256), 0, s, a); break;
+    case 1: NMR_FS(1) break;
+    case 2: NMR_FS(2) break;
+    case 3: NMR_FS(3) break;
     default: return hipErrorInvalidValue;
     }
+#undef NMR_FS
     return hipGetLastError();
 }
 

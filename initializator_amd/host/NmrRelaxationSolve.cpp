@@ -199,9 +199,8 @@ static void boundary_modes(const int nb[6], int out[3])
 // noise field, boundary flags that define every ghost).
 void NmrRelaxationSolve::Validate() const
 {
-    if (nComponents != 1) unsupported("NCOMP=2 (two-phase mixing) is not implemented by the GPU solver");
-    if (nFluid != 1 && nFluid != 2) unsupported("NFLUID must be 1 or 2");
-    if (nSurfRelWater != 0 || nSurfRelOil != 0) unsupported("face-wise surface relaxivity files are not implemented");
+    if (nComponents != 1 && nComponents != 2) unsupported("NCOMP must be 1 or 2");
+    if (nComponents == 1 && nFluid != 1 && nFluid != 2) unsupported("NFLUID must be 1 or 2");
     if (noiseAmpl != 0.0 && nDirectionField != 0) unsupported("the random noise field (AMPL_NOISE) is not implemented");
     int modes[3];
     boundary_modes(nBoundary, modes);
@@ -294,29 +293,117 @@ void NmrRelaxationSolve::ReadVelocity()
     if ((dz <= dy) && (dz <= dz)) dTimeVel = cfl * dz / (vmax + kSmall);
 }
 
-// The step-size limits of FillInDiffusinAndTime (cpp:1756-1864) and
-// FillInMainField (cpp:1980-1988); the per-voxel D/T1/T2 themselves are
-// derived on the GPU from the mask.
+// Concentrations (cpp:930-1008): one fluid fills concentration[0] (water)
+// or [1] (oil) with 1 on the interior; NCOMP=2 reads data/n1_init.dat
+// (header, then i outer, j, k inner) into [0] and 1 - value into [1].
+void NmrRelaxationSolve::ReadConcentration()
+{
+    for (int q = 0; q < 2; q++) conc[q].assign(actCell.size(), 0.0);
+    if (nComponents == 1) {
+        const int q = nFluid == 1 ? 0 : 1;
+        for (int k = 1; k <= kk; k++)
+            for (int j = 1; j <= jk; j++)
+                for (int i = 1; i <= ik; i++) conc[q][index(i, j, k)] = 1.0;
+        return;
+    }
+    std::ifstream f(path("data/n1_init.dat"));
+    if (!f) unsupported("no file data/n1_init.dat (NCOMP=2)");
+    std::string header;
+    std::getline(f, header);
+    for (int i = 1; i <= ik; i++)
+        for (int j = 1; j <= jk; j++)
+            for (int k = 1; k <= kk; k++) {
+                float b = 0.f; // the reference reads into a float
+                f >> b;
+                if (!f) unsupported("data/n1_init.dat: too few values");
+                conc[0][index(i, j, k)] = b;
+                conc[1][index(i, j, k)] = 1.0 - b;
+            }
+}
+
+// Face relaxivities (cpp:1010-1433): with nSurfRelWater / nSurfRelOil = 1
+// the files data/SurfRel{1,2}{Water,Oil}{X,Y,Z}.dat (header, i outer, k
+// inner, staggered like the velocity files); otherwise the uniform
+// R1/R2 of that fluid on every face.
+void NmrRelaxationSolve::ReadSurfRel()
+{
+    const char* fluid[2] = {"Water", "Oil"};
+    const int on[2] = {nSurfRelWater, nSurfRelOil};
+    const double uni[2][2] = {{r1Water, r2Water}, {r1Oil, r2Oil}};
+    for (int fl = 0; fl < 2; fl++)
+        for (int rel = 0; rel < 2; rel++)
+            for (int ax = 0; ax < 3; ax++) {
+                std::vector<double>& a = surf[fl * 6 + rel * 3 + ax];
+                if (on[fl] != 1) {
+                    a.assign(actCell.size(), uni[fl][rel]);
+                    continue;
+                }
+                a.assign(actCell.size(), 0.0);
+                const std::string name = std::string("data/SurfRel") + (rel == 0 ? "1" : "2") + fluid[fl] + "XYZ"[ax] +
+                                         ".dat";
+                std::ifstream f(path(name.c_str()));
+                if (!f) unsupported("no file " + name);
+                std::string header;
+                std::getline(f, header);
+                const int ni = ik + (ax == 0), nj = jk + (ax == 1), nk = kk + (ax == 2);
+                for (int i = 1; i <= ni; i++)
+                    for (int j = 1; j <= nj; j++)
+                        for (int k = 1; k <= nk; k++) {
+                            double b = 0;
+                            f >> b;
+                            if (!f) unsupported(name + ": too few values");
+                            a[index(i, j, k)] = b;
+                        }
+            }
+}
+
+// Phase fractions of cell c (cpp:1780-1791).
+void NmrRelaxationSolve::Phase(size_t c, double* c1, double* c2) const
+{
+    if (nComponents == 2) {
+        const double x0 = conc[0][c], x1 = conc[1][c];
+        const double z1 = std::sqrt((x0 - an1) * (x0 - an1) + (x1 - an2) * (x1 - an2));
+        const double z2 = std::sqrt((x0 - bn1) * (x0 - bn1) + (x1 - bn2) * (x1 - bn2));
+        *c1 = z2 / (z1 + z2);
+        *c2 = z1 / (z1 + z2);
+    } else {
+        *c1 = conc[0][c];
+        *c2 = conc[1][c];
+    }
+}
+
+// FillInDiffusinAndTime (cpp:1756-1864) and the step limits of
+// FillInMainField (cpp:1980-1988): per-voxel D, 1/T1, 1/T2 (kept for the GPU
+// when the coefficients vary), their extremes for StepTime.
 void NmrRelaxationSolve::Coefficients()
 {
-    const double c1 = nFluid == 1 ? 1.0 : 0.0, c2 = nFluid == 2 ? 1.0 : 0.0;
     double dMax = 0.0, tMin = 100.0;
+    dvox.assign(actCell.size(), 0.0);
+    r1vox.assign(actCell.size(), 0.0);
+    r2vox.assign(actCell.size(), 0.0);
+    const double h3[3] = {dx, dy, dz};
     for (int k = 1; k <= kk; k++)
         for (int j = 1; j <= jk; j++)
             for (int i = 1; i <= ik; i++) {
-                if (actCell[index(i, j, k)] != '1') continue;
+                const size_t c = index(i, j, k);
+                if (actCell[c] != '1') continue;
+                double c1, c2;
+                Phase(c, &c1, &c2);
                 double D = (c1 * dWater + c2 * dOil) * std::pow(1.0 - 2.0 * c1 * c2 / (c1 * c1 + c2 * c2), 10.0);
                 if (nDiffusion == 0) D = kSmall;
                 double ct1 = c1 / t1Water + c2 / t1Oil, ct2 = c1 / t2Water + c2 / t2Oil;
-                const double r1 = c1 * r1Water + c2 * r1Oil, r2 = c1 * r2Water + c2 * r2Oil;
                 const size_t nb[6] = {index(i - 1, j, k), index(i + 1, j, k), index(i, j - 1, k),
                                       index(i, j + 1, k), index(i, j, k - 1), index(i, j, k + 1)};
-                const double h[6] = {dx, dx, dy, dy, dz, dz};
+                const size_t fc[6] = {c, nb[1], c, nb[3], c, nb[5]};
                 for (int s = 0; s < 6; s++)
                     if (actCell[nb[s]] == '0') {
-                        ct1 += r1 / h[s];
-                        ct2 += r2 / h[s];
+                        const int ax = s / 2;
+                        ct1 += (c1 * surf[ax][fc[s]] + c2 * surf[6 + ax][fc[s]]) / h3[ax];
+                        ct2 += (c1 * surf[3 + ax][fc[s]] + c2 * surf[9 + ax][fc[s]]) / h3[ax];
                     }
+                dvox[c] = nDiffusion == 1 ? D : 0.0;
+                r1vox[c] = ct1;
+                r2vox[c] = ct2;
                 const double t1 = 1.0 / ct1, t2 = 1.0 / ct2;
                 if (dMax <= D) dMax = D;
                 if (tMin >= t1) tMin = t1;
@@ -336,6 +423,7 @@ void NmrRelaxationSolve::Coefficients()
     if (constGradZ != 0) gradAmpl = std::abs(constGradZ * dz * kk);
     dTimeNoise = cfl1 / (kG * noiseAmpl);
     dTimeGrad = cfl1 / (kG * gradAmpl);
+    perVoxel = nComponents == 2 || nSurfRelWater == 1 || nSurfRelOil == 1;
 }
 
 // cpp:2472-2517: pEq along the field direction in pores (ghosts included).
@@ -380,6 +468,8 @@ void NmrRelaxationSolve::init()
     boundary_modes(nBoundary, bcMode);
     ReadActiveCells();
     ReadVelocity();
+    ReadConcentration();
+    ReadSurfRel();
     Coefficients();
     FillInMagnetization();
     StepTime();
@@ -459,6 +549,23 @@ void NmrRelaxationSolve::create_context(int scheme)
               "nmr_hip_upload_velocity");
     check(nmr_hip_upload_magnetization(ctx_, magn[0].data(), magn[1].data(), magn[2].data(), NMR_LAYOUT_PADDED),
           "nmr_hip_upload_magnetization");
+    if (perVoxel)
+        check(nmr_hip_upload_coefficients(ctx_, dvox.data(), r1vox.data(), r2vox.data(), NMR_LAYOUT_PADDED),
+              "nmr_hip_upload_coefficients");
+    if (nComponents == 2) { // two-phase outputs: weights c1, c2 on cells with c1 or c2 above 0.8
+        std::vector<double> w1(actCell.size(), 0.0), w2(actCell.size(), 0.0);
+        for (size_t c = 0; c < actCell.size(); c++) {
+            if (actCell[c] != '1') continue;
+            double c1, c2;
+            Phase(c, &c1, &c2);
+            if (c1 > 0.8 || c2 > 0.8) {
+                w1[c] = c1;
+                w2[c] = c2;
+            }
+        }
+        check(nmr_hip_upload_phase_weights(ctx_, w1.data(), w2.data(), NMR_LAYOUT_PADDED),
+              "nmr_hip_upload_phase_weights");
+    }
 }
 
 void NmrRelaxationSolve::apply(const std::vector<Pulse>& pulses)
@@ -535,10 +642,23 @@ std::vector<NmrRelaxationSolve::Pulse> NmrRelaxationSolve::SequenceCpu(double t)
 
 namespace {
 
-void csv_row(std::ofstream& f, double t, const double* s)
+// One CSV row: Time; SumX; SumY; SumZ, or with two phases
+// Time; SumX1; SumY1; SumZ1; SumX2; SumY2; SumZ2 (cu:1212-1218).
+void csv_row(std::ofstream& f, double t, const double* s, int n)
 {
-    f << std::scientific << t << "; " << std::scientific << s[0] << "; " << std::scientific << s[1] << "; "
-      << std::scientific << s[2] << std::endl;
+    f << std::scientific << t;
+    for (int q = 0; q < n; q++) f << "; " << std::scientific << s[q];
+    f << std::endl;
+}
+
+void print_row(double t, const double* s, int n)
+{
+    if (n == 3) {
+        std::cout << " Time = " << t << " SumX=" << s[0] << " SumY=" << s[1] << " SumZ=" << s[2] << std::endl;
+    } else {
+        std::cout << " Time = " << t << " SumX1=" << s[0] << " SumY1=" << s[1] << " SumZ1=" << s[2] << "\t"
+                  << " SumX2=" << s[3] << " SumY2=" << s[4] << " SumZ2=" << s[5] << std::endl;
+    }
 }
 
 } // namespace
@@ -553,7 +673,8 @@ void NmrRelaxationSolve::solve_gpu()
     std::ofstream res(path("results_gpu.csv"));
     res << "Time; SumX; SumY; SumZ" << std::endl;
     std::vector<double> sig;
-    double s[3] = {0, 0, 0};
+    const int ns = nComponents == 2 ? 6 : 3; // values per signal row
+    double s[6] = {0, 0, 0, 0, 0, 0};
     double currentTime = 0.0;
     int64_t pending = 0; // updates not yet run
     for (int64_t i = 0; i < nStep + 1; i++) {
@@ -562,21 +683,18 @@ void NmrRelaxationSolve::solve_gpu()
         std::vector<Pulse> pulses = SequenceGpu(currentTime);
         const bool out = (i % ndPrint) == 0;
         if (pulses.empty() && !out && i < nStep) continue;
-        sig.resize((size_t)(pending + 1) * 3);
+        sig.resize((size_t)(pending + 1) * ns);
         check(nmr_hip_run(ctx_, pending, nullptr, sig.data()), "nmr_hip_run");
         pending = 0;
         if (!pulses.empty()) apply(pulses);
         if (out) {
             if (pulses.empty()) {
-                s[0] = sig[sig.size() - 3];
-                s[1] = sig[sig.size() - 2];
-                s[2] = sig[sig.size() - 1];
+                for (int q = 0; q < ns; q++) s[q] = sig[sig.size() - ns + q];
             } else {
                 check(nmr_hip_signal(ctx_, s), "nmr_hip_signal");
             }
-            std::cout << " Time = " << currentTime << " SumX=" << s[0] << " SumY=" << s[1] << " SumZ=" << s[2]
-                      << std::endl;
-            csv_row(res, currentTime, s);
+            print_row(currentTime, s, ns);
+            csv_row(res, currentTime, s, ns);
         }
     }
     res.close();
@@ -585,8 +703,8 @@ void NmrRelaxationSolve::solve_gpu()
     std::snprintf(name, sizeof(name), "results_NMR/results_grad_%d_%d_%d.csv", PulseNum[0], PulseNum[1], PulseNum[2]);
     std::ofstream grad(path(name));
     check(nmr_hip_signal(ctx_, s), "nmr_hip_signal");
-    std::cout << " SumX=" << s[0] << " SumY=" << s[1] << " SumZ=" << s[2] << std::endl;
-    if (grad) csv_row(grad, currentTime, s);
+    print_row(currentTime, s, ns);
+    if (grad) csv_row(grad, currentTime, s, ns);
     nmr_hip_destroy(ctx_);
     ctx_ = nullptr;
 }
@@ -599,15 +717,15 @@ void NmrRelaxationSolve::solve()
     flag1 = flag2 = flag3 = flag4 = flag5 = 0;
     std::ofstream res(path("results_cpu.csv"));
     res << "Time; SumX; SumY; SumZ" << std::endl;
-    double s[3];
+    const int ns = nComponents == 2 ? 6 : 3;
+    double s[6];
     int64_t i = 0;
     while (i < nStep + 1) {
         const double currentTime = (double)i * dTime;
         if ((i % ndPrint) == 0) {
             check(nmr_hip_signal(ctx_, s), "nmr_hip_signal");
-            std::cout << " Time = " << currentTime << " SumX=" << s[0] << " SumY=" << s[1] << " SumZ=" << s[2]
-                      << std::endl;
-            csv_row(res, currentTime, s);
+            print_row(currentTime, s, ns);
+            csv_row(res, currentTime, s, ns);
         }
         // updates i .. j-1 in one call: stop after a step that is followed by
         // a pulse, or before a step that starts with an output row

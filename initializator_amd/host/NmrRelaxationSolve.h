@@ -16,11 +16,12 @@
 //
 // Both loops run the update on the MI355X through the C ABI of nmr_hip.h;
 // there is no CPU solver behind them.  The whole domain lives on one GPU, so
-// MPI_size must be 1.  Configurations the GPU path does not implement
-// (two-phase NCOMP=2, face-wise relaxivity files, random noise field, NB
-// flag combinations whose ghosts the reference never refreshes) are
-// reported and end the program with exit(1), where the reference would run
-// on with uninitialised data.  The NB flags select zero-derivative,
+// MPI_size must be 1.  Two-phase mixtures (NCOMP=2, data/n1_init.dat) and
+// face-wise relaxivity files run with per-voxel coefficients and, for
+// NCOMP=2, the 7-column two-phase output.  Configurations the GPU path does
+// not implement (random noise field, NB flag combinations whose ghosts the
+// reference never refreshes) are reported and end the program with
+// exit(1), where the reference would run on with uninitialised data.  The NB flags select zero-derivative,
 // periodic or zero-value ghosts in both loops (the reference's host loop
 // ignores them).
 #ifndef NMR_RELAXATION_SOLVE_H
@@ -88,6 +89,10 @@ private:
     // padded host arrays, index i + (ik+2)*(j + (jk+2)*k)
     std::vector<uint8_t> actCell;
     std::vector<double> vel[3], magn[3];
+    std::vector<double> conc[2];   // concentration[0], [1] (cpp:930-1008)
+    std::vector<double> surf[12];  // face relaxivities 1W xyz, 2W xyz, 1O xyz, 2O xyz (cpp:1010-1433)
+    std::vector<double> dvox, r1vox, r2vox; // FillInDiffusinAndTime: D, 1/T1, 1/T2 per voxel
+    bool perVoxel = false;         // mixture or face-wise relaxivity: per-voxel coefficients on the GPU
     bool initialised_ = false;
     int scheme_ = -1;
     std::string root_ = ".";
@@ -100,6 +105,9 @@ private:
     void Validate() const;
     void ReadActiveCells();
     void ReadVelocity();
+    void ReadConcentration();
+    void ReadSurfRel();
+    void Phase(size_t c, double* c1, double* c2) const;
     void Coefficients();
     void FillInMagnetization();
     void StepTime();

@@ -49,9 +49,13 @@ def _fmt(v) -> str:
     return repr(float(v))
 
 
-def write_inputs(root: str, shape, mask=None, vx=None, vy=None, vz=None, grad_table=None, **overrides) -> None:
+def write_inputs(root: str, shape, mask=None, vx=None, vy=None, vz=None, grad_table=None, n1=None, surf=None,
+                 **overrides) -> None:
     """Write root/data/*.dat for a compact case.  overrides name a key by the
-    start of its description (e.g. NSEQUENCE=2, R2_WAT=1e-5, Printing_on=10)."""
+    start of its description (e.g. NSEQUENCE=2, R2_WAT=1e-5, Printing_on=10).
+    n1: concentration (nz, ny, nx) for data/n1_init.dat (NCOMP=2); surf:
+    {"SurfRel1WaterX": faces, ...} face relaxivity files, staggered like the
+    velocities (X: (nz, ny, nx+1), Y: (nz, ny+1, nx), Z: (nz+1, ny, nx))."""
     nz, ny, nx = shape
     d = os.path.join(root, "data")
     os.makedirs(d, exist_ok=True)
@@ -79,17 +83,24 @@ def write_inputs(root: str, shape, mask=None, vx=None, vy=None, vz=None, grad_ta
             f.write("poro\n")
             for k in range(nz):
                 f.write("\n".join(" ".join("1" if c else "0" for c in row) for row in mask[k]) + "\n")
+    # i outer, j, k inner; X-staggered arrays have nx+1 faces, Y ny+1, Z nz+1
+    staggered = []
     if vx is not None:
-        # i outer, j, k inner; Vx has nx+1 faces, Vy ny+1, Vz nz+1
-        for name, v in (("Vx.dat", vx), ("Vy.dat", vy), ("Vz.dat", vz)):
-            arr = np.transpose(np.asarray(v, float), (2, 1, 0))  # (i, j, k)
-            with open(os.path.join(d, name), "w") as f:
-                f.write(name + "\n")
-                np.savetxt(f, arr.reshape(-1), fmt="%.17g")
+        staggered += [("Vx.dat", vx), ("Vy.dat", vy), ("Vz.dat", vz)]
+    if n1 is not None:
+        staggered.append(("n1_init.dat", n1))
+    for key, v in (surf or {}).items():
+        staggered.append((key + ".dat", v))
+    for name, v in staggered:
+        arr = np.transpose(np.asarray(v, float), (2, 1, 0))  # (i, j, k)
+        with open(os.path.join(d, name), "w") as f:
+            f.write(name + "\n")
+            np.savetxt(f, arr.reshape(-1), fmt="%.17g")
 
 
 def read_results(path: str) -> np.ndarray:
-    """Rows of a results_*.csv as an (n, 4) array: time, SumX, SumY, SumZ."""
+    """Rows of a results_*.csv: (n, 4) time, SumX, SumY, SumZ, or (n, 7) with
+    the two-phase sums SumX1..SumZ1, SumX2..SumZ2."""
     rows = []
     with open(path) as f:
         next(f)

@@ -108,6 +108,19 @@ class HipSolver:
         check(self.lib.nmr_hip_upload_magnetization(self.ctx, *[_dptr(a) for a in arrs], layout),
               "nmr_hip_upload_magnetization")
 
+    def upload_coefficients(self, D, r2, r1=None, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        """Per-voxel diffusivity and relaxation rates 1/T1, 1/T2 (surface terms included)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (D, r1, r2)]
+        check(self.lib.nmr_hip_upload_coefficients(self.ctx, *[_dptr(a) for a in arrs], layout),
+              "nmr_hip_upload_coefficients")
+
+    def upload_phase_weights(self, w1, w2, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        """Two-phase output weights; signal rows then hold 2*ncomp sums."""
+        a, b = (np.ascontiguousarray(x, dtype=np.float64) for x in (w1, w2))
+        check(self.lib.nmr_hip_upload_phase_weights(self.ctx, _dptr(a), _dptr(b), layout),
+              "nmr_hip_upload_phase_weights")
+        self.nsig = 2 * self.ncomp
+
     def download_magnetization(self, layout: int = _lib.NMR_LAYOUT_COMPACT):
         nz, ny, nx = self.shape
         shp = (nz, ny, nx) if layout == _lib.NMR_LAYOUT_COMPACT else (nz + 2, ny + 2, nx + 2)
@@ -127,7 +140,7 @@ class HipSolver:
             g = np.ascontiguousarray(grad_sched, dtype=np.float64)
             if g.shape != (nsteps, 3):
                 raise ValueError(f"grad_sched must have shape ({nsteps}, 3)")
-        sig = np.empty((nsteps + 1, self.ncomp)) if want_signal else None
+        sig = np.empty((nsteps + 1, getattr(self, "nsig", self.ncomp))) if want_signal else None
         check(self.lib.nmr_hip_run(self.ctx, int(nsteps), _dptr(g), _dptr(sig)), "nmr_hip_run")
         return sig
 
@@ -137,7 +150,7 @@ class HipSolver:
                                      float(axis[2])), "nmr_hip_pulse")
 
     def signal(self) -> np.ndarray:
-        out = np.empty(self.ncomp)
+        out = np.empty(getattr(self, "nsig", self.ncomp))
         check(self.lib.nmr_hip_signal(self.ctx, _dptr(out)), "nmr_hip_signal")
         return out
 

@@ -44,6 +44,35 @@ class OrParams(ctypes.Structure):
     ]
 
 
+class OrFluids(ctypes.Structure):
+    """Water (index 0) and oil (1) properties and the two-phase projection."""
+
+    _fields_ = [
+        ("d", c_double * 2),
+        ("t1", c_double * 2),
+        ("t2", c_double * 2),
+        ("r1", c_double * 2),
+        ("r2", c_double * 2),
+        ("ncomp", c_int32),
+        ("reserved", c_int32),
+        ("an1", c_double),
+        ("an2", c_double),
+        ("bn1", c_double),
+        ("bn2", c_double),
+    ]
+
+
+def fluids(d=(2.3e-9, 2.1e-10), t1=(2.5e10, 2.5e10), t2=(2.0, 2.0), r1=(0.0, 0.0), r2=(0.0, 0.0), ncomp=2,
+           an=(1.0, 0.0), bn=(0.0, 1.0)) -> OrFluids:
+    f = OrFluids()
+    for i in range(2):
+        f.d[i], f.t1[i], f.t2[i], f.r1[i], f.r2[i] = d[i], t1[i], t2[i], r1[i], r2[i]
+    f.ncomp = int(ncomp)
+    f.an1, f.an2 = an
+    f.bn1, f.bn2 = bn
+    return f
+
+
 UPWIND, MACCORMACK = 0, 1
 BC_NEUMANN, BC_PERIODIC, BC_DIRICHLET = 0, 1, 2
 
@@ -76,6 +105,10 @@ def lib():
         L.or_run.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, c_int64, dp, dp, c_int, c_int, c_int,
                              dp, dp, dp, dp]
         L.or_maccormack.argtypes = [P, cp] + [dp] * 18
+        F = POINTER(OrFluids)
+        L.or_phase.argtypes = [F, c_double, c_double, dp, dp]
+        L.or_fill_coefficients_2f.argtypes = [P, cp, F, dp, dp, POINTER(dp), dp, dp, dp, dp]
+        L.or_output_2f.argtypes = [P, cp, c_int, c_int, c_int, F, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -135,6 +168,32 @@ def fill_coefficients(p, act, d_water, t1_water, t2_water, r1_water, r2_water, f
     lib().or_fill_coefficients(ctypes.byref(p), _c(act), d_water, t1_water, t2_water, r1_water, r2_water,
                                *[_d(x) for x in f], _d(D), _d(t1), _d(t2), _d(out2))
     return D, t1, t2, (out2[0], out2[1])
+
+
+def phase(fl, x0, x1):
+    a, b = c_double(), c_double()
+    lib().or_phase(ctypes.byref(fl), float(x0), float(x1), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def fill_coefficients_2f(p, act, fl, conc=None, faces=None):
+    """Two-fluid D, T1, T2 (padded) and (Dmax, Tmin).  conc: (conc0, conc1)
+    padded or None (water only); faces: 12 padded arrays or None entries
+    (1Water x,y,z, 2Water x,y,z, 1Oil x,y,z, 2Oil x,y,z)."""
+    shp = padded_shape(p)
+    D, t1, t2, out2 = np.zeros(shp), np.zeros(shp), np.zeros(shp), np.zeros(2)
+    fa = (POINTER(c_double) * 12)(*[_d(x) if x is not None else None for x in (faces or [None] * 12)])
+    c0, c1 = (None, None) if conc is None else conc
+    lib().or_fill_coefficients_2f(ctypes.byref(p), _c(act), ctypes.byref(fl), _d(c0), _d(c1), fa, _d(D), _d(t1),
+                                  _d(t2), _d(out2))
+    return D, t1, t2, (out2[0], out2[1])
+
+
+def output_2f(p, act, buf, fl, conc, f):
+    a, b = c_double(), c_double()
+    lib().or_output_2f(ctypes.byref(p), _c(act), buf[0], buf[1], buf[2], ctypes.byref(fl), _d(conc[0]),
+                       _d(conc[1]), _d(f), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
 
 
 def fill_gradient(p, g):
@@ -303,7 +362,7 @@ def nb_modes(nb):
     return (x, pair(nb[1], nb[3]), pair(nb[4], nb[5]))
 
 
-def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None):
+def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None, n1=None, surf=None):
     """The reference's solve_gpu (gpu=True, cu:1187-1227, MacCormack update)
     or solve (cpp:35-66, ExplicitSimple update) loop in FP64 for one fluid:
     returns (rows (n, 4), dTime, nStep).  scheme overrides the update.  The
@@ -335,7 +394,41 @@ def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None):
     p = params(mask.shape, dx, 1.0, dy=dy, dz=dz, precession=cfg["NVECTOR"] == 1, velocity=cfg["NVELOC"] == 1,
                diffusion=cfg["NDIFFUS"] == 1, p_eq=peq, b0=b0, scheme=scheme, bc=bc)
     act = layout.pad_mask(mask)
-    D, t1, t2, (dmax, tmin) = fill_coefficients(p, act, Dw, T1b, T2b, r1, r2)
+    two = cfg.get("NCOMP", 1) == 2
+    conc = None
+    if two or surf is not None:
+        # mixture / face-wise relaxivity: FillInDiffusinAndTime with both fluids
+        fprops = fluids(d=(cfg["D_WAT"], cfg["D_OIL"]), t1=(cfg["T1_WAT"], cfg["T1_OIL"]), t2=(cfg["T2_WAT"], cfg["T2_OIL"]),
+                    r1=(cfg["R1_WAT"], cfg["R1_OIL"]), r2=(cfg["R2_WAT"], cfg["R2_OIL"]), ncomp=2 if two else 1,
+                    an=(cfg["AN(1)"], cfg["AN(2)"]), bn=(cfg["BN(1)"], cfg["BN(2)"]))
+        if two:
+            n1f = np.asarray(n1, dtype=np.float32).astype(np.float64)  # read into a float (cpp:976-1006)
+            conc = (np.ascontiguousarray(layout.pad_field(n1f)), np.ascontiguousarray(layout.pad_field(1.0 - n1f)))
+            for cc in conc:  # ghosts hold 0 (cpp:935-944)
+                cc[0, :, :] = cc[-1, :, :] = 0.0
+                cc[:, 0, :] = cc[:, -1, :] = 0.0
+                cc[:, :, 0] = cc[:, :, -1] = 0.0
+        else:
+            one = np.zeros(padded_shape(p))
+            one[1:-1, 1:-1, 1:-1] = 1.0
+            conc = (one, np.zeros(padded_shape(p))) if fluid == 1 else (np.zeros(padded_shape(p)), one)
+        faces = [None] * 12
+        for rel in (0, 1):
+            for fi, fname in enumerate(("Water", "Oil")):
+                keys = [f"SurfRel{rel + 1}{fname}{a}" for a in "XYZ"]
+                if surf is not None and all(k in surf for k in keys):
+                    arrs = layout.pad_velocity(*[surf[k] for k in keys], mask.shape)
+                    for ax in range(3):
+                        faces[fi * 6 + rel * 3 + ax] = np.ascontiguousarray(arrs[ax])
+        D, t1, t2, (dmax, tmin) = fill_coefficients_2f(p, act, fprops, conc, faces)
+    else:
+        D, t1, t2, (dmax, tmin) = fill_coefficients(p, act, Dw, T1b, T2b, r1, r2)
+
+    def row(t):
+        if two:
+            s1, s2 = zip(*[output_2f(p, act, buf, fprops, conc, m[q]) for q in range(3)])
+            return [t] + list(s1) + list(s2)
+        return [t] + [output(p, act, buf, m[q])[0] for q in range(3)]
     # step limits (cpp:1583-1587, 1849-1864, 1980-1988, 2519-2534)
     cfl, cfl1, g = cfg["CFL"], cfg["CFL1"], G_REF
     if cfg["NVELOC"] == 1:
@@ -380,7 +473,7 @@ def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None):
     for i in range(nstep + 1):
         t = i * dt
         if not gpu and i % ndp == 0:
-            rows.append([t] + [output(p, act, buf, m[q])[0] for q in range(3)])
+            rows.append(row(t))
         if scheme == MACCORMACK:
             maccormack(p, act, D, t1, t2, vel, nf, m, pc)
         else:
@@ -390,5 +483,5 @@ def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None):
         for x in m:
             bc_field(p, x)
         if gpu and i % ndp == 0:
-            rows.append([t] + [output(p, act, buf, m[q])[0] for q in range(3)])
+            rows.append(row(t))
     return np.array(rows), dt, nstep

@@ -31,6 +31,10 @@
  *                        zero-value diffusion sine eigenmode
  *   or_bc_mask           pinned: brute copy check
  *   or_fill_coefficients pinned: hand-computed rates on a 3^3 pore in 5^3
+ *   or_fill_coefficients_2f, or_phase, or_output_2f  pinned: hand-computed
+ *                        two-phase fractions, penalty, rates with distinct
+ *                        face relaxivities per side and fluid; gate and
+ *                        weights of the two-phase sums by brute force
  *   or_fill_gradient     pinned: closed form G*d*(idx-1/2)
  *   or_output            pinned: brute-force numpy sum with trimming
  *   or_step_time         pinned: formula arithmetic
@@ -133,19 +137,44 @@ void or_bc_field(const or_params* p, double* f)
         }
 }
 
-/* Per-voxel diffusivity and relaxation times for one fluid (NCOMP=1,
- * NFLUID=1 -> c1=1, c2=0), NmrRelaxationSolve.cpp:1756-1893, followed by
- * the diffusion ghost fill of NmrRelaxationSolve.cpp:1895-1942.
- * Surface relaxivities: uniform (nSurfRelWater=0, cpp:1217-1231) unless
- * face arrays are given (padded layout, face f of cell i on the minus side
- * is stored at i, the plus side at i+1, as cpp:1798-1841 reads them).
+/* Fluid properties of FillInDiffusinAndTime (water = index 0, oil = 1) and
+ * the two-phase projection constants AN(1), AN(2), BN(1), BN(2). */
+typedef struct or_fluids {
+    double d[2], t1[2], t2[2], r1[2], r2[2];
+    int32_t ncomp;           /* 1: (c1, c2) = (conc0, conc1); 2: projection */
+    int32_t reserved;
+    double an1, an2, bn1, bn2;
+} or_fluids;
+
+/* Phase fractions of a cell from its two concentrations
+ * (NmrRelaxationSolve.cpp:1780-1791): with NCOMP=2 the distances z1, z2 of
+ * (x0, x1) to the points (AN1, AN2) and (BN1, BN2) give c1 = z2/(z1+z2),
+ * c2 = z1/(z1+z2); with NCOMP=1 the concentrations are the fractions. */
+void or_phase(const or_fluids* f, double x0, double x1, double* c1, double* c2)
+{
+    if (f->ncomp == 2) {
+        const double z1 = sqrt((x0 - f->an1) * (x0 - f->an1) + (x1 - f->an2) * (x1 - f->an2));
+        const double z2 = sqrt((x0 - f->bn1) * (x0 - f->bn1) + (x1 - f->bn2) * (x1 - f->bn2));
+        *c1 = z2 / (z1 + z2);
+        *c2 = z1 / (z1 + z2);
+    } else {
+        *c1 = x0;
+        *c2 = x1;
+    }
+}
+
+/* Per-voxel diffusivity and relaxation times (NmrRelaxationSolve.cpp:1756-1893),
+ * followed by the diffusion ghost fill (cpp:1895-1942).  Concentrations
+ * conc0/conc1 (padded; NULL: one fluid, water) give the phase fractions;
+ * faces[12] are the face relaxivity arrays 1Water x,y,z, 2Water x,y,z,
+ * 1Oil x,y,z, 2Oil x,y,z (padded; face of cell i on the minus side stored at
+ * i, the plus side at i+1, as cpp:1798-1841 reads them), a NULL entry
+ * meaning the uniform R*_WAT / R*_OIL value (cpp:1217-1231).
  * Ghost entries of D copy the interior; T1/T2 ghosts are left untouched
  * (the reference never reads them).  out2[0]=max D, out2[1]=min T. */
-void or_fill_coefficients(const or_params* p, const char* act, double d_water,
-                          double t1_water, double t2_water, double r1_water, double r2_water,
-                          const double* sr1x, const double* sr1y, const double* sr1z,
-                          const double* sr2x, const double* sr2y, const double* sr2z,
-                          double* D, double* t1, double* t2, double* out2)
+void or_fill_coefficients_2f(const or_params* p, const char* act, const or_fluids* fl, const double* conc0,
+                             const double* conc1, const double* const* faces, double* D, double* t1, double* t2,
+                             double* out2)
 {
     double dmax = 0.0, tmin = 100.0;
     for (int k = 1; k < p->kk + 1; k++)
@@ -159,26 +188,28 @@ void or_fill_coefficients(const or_params* p, const char* act, double d_water,
                     continue;
                 }
                 if (act[c] != '1') continue;
-                const double c1 = 1.0, c2 = 0.0;
-                double cd = c1 * d_water;
+                double c1 = 1.0, c2 = 0.0;
+                if (conc0) or_phase(fl, conc0[c], conc1[c], &c1, &c2);
+                const double cd = c1 * fl->d[0] + c2 * fl->d[1];
                 D[c] = cd * pow(1.0 - 2.0 * c1 * c2 / (c1 * c1 + c2 * c2), 10.0);
                 if (!p->diffusion) D[c] = OR_SMALL;
-                double ct1 = c1 / t1_water, ct2 = c1 / t2_water;
-                /* face relaxivity on each solid side */
-                struct { long nb; long face; const double* a1; const double* a2; double h; } side[6] = {
-                    {IX(p, i - 1, j, k), c, sr1x, sr2x, p->dx},
-                    {IX(p, i + 1, j, k), IX(p, i + 1, j, k), sr1x, sr2x, p->dx},
-                    {IX(p, i, j - 1, k), c, sr1y, sr2y, p->dy},
-                    {IX(p, i, j + 1, k), IX(p, i, j + 1, k), sr1y, sr2y, p->dy},
-                    {IX(p, i, j, k - 1), c, sr1z, sr2z, p->dz},
-                    {IX(p, i, j, k + 1), IX(p, i, j, k + 1), sr1z, sr2z, p->dz},
-                };
+                double ct1 = c1 / fl->t1[0] + c2 / fl->t1[1], ct2 = c1 / fl->t2[0] + c2 / fl->t2[1];
+                /* face relaxivity on each solid side: (axis, neighbour, face index) */
+                const long nb[6] = {IX(p, i - 1, j, k), IX(p, i + 1, j, k), IX(p, i, j - 1, k),
+                                    IX(p, i, j + 1, k), IX(p, i, j, k - 1), IX(p, i, j, k + 1)};
+                const long fc[6] = {c, nb[1], c, nb[3], c, nb[5]};
+                const double h[3] = {p->dx, p->dy, p->dz};
                 for (int s = 0; s < 6; s++) {
-                    if (act[side[s].nb] != '0') continue;
-                    double q1 = side[s].a1 ? side[s].a1[side[s].face] : r1_water;
-                    double q2 = side[s].a2 ? side[s].a2[side[s].face] : r2_water;
-                    ct1 += (c1 * q1) / side[s].h;
-                    ct2 += (c1 * q2) / side[s].h;
+                    if (act[nb[s]] != '0') continue;
+                    const int ax = s / 2;
+                    const double* w1 = faces ? faces[ax] : NULL;
+                    const double* w2 = faces ? faces[3 + ax] : NULL;
+                    const double* o1 = faces ? faces[6 + ax] : NULL;
+                    const double* o2 = faces ? faces[9 + ax] : NULL;
+                    const double q1w = w1 ? w1[fc[s]] : fl->r1[0], q2w = w2 ? w2[fc[s]] : fl->r2[0];
+                    const double q1o = o1 ? o1[fc[s]] : fl->r1[1], q2o = o2 ? o2[fc[s]] : fl->r2[1];
+                    ct1 += (c1 * q1w + c2 * q1o) / h[ax];
+                    ct2 += (c1 * q2w + c2 * q2o) / h[ax];
                 }
                 t1[c] = 1.0 / ct1;
                 t2[c] = 1.0 / ct2;
@@ -195,6 +226,50 @@ void or_fill_coefficients(const or_params* p, const char* act, double d_water,
         out2[0] = dmax;
         out2[1] = tmin;
     }
+}
+
+/* One fluid (water, NCOMP=1, NFLUID=1: c1 = 1, c2 = 0) with water face
+ * relaxivity arrays sr* (NULL: uniform r1_water / r2_water). */
+void or_fill_coefficients(const or_params* p, const char* act, double d_water,
+                          double t1_water, double t2_water, double r1_water, double r2_water,
+                          const double* sr1x, const double* sr1y, const double* sr1z,
+                          const double* sr2x, const double* sr2y, const double* sr2z,
+                          double* D, double* t1, double* t2, double* out2)
+{
+    or_fluids fl;
+    memset(&fl, 0, sizeof(fl));
+    fl.d[0] = d_water;
+    fl.t1[0] = t1_water;
+    fl.t2[0] = t2_water;
+    fl.r1[0] = r1_water;
+    fl.r2[0] = r2_water;
+    fl.t1[1] = fl.t2[1] = 1.0; /* c2 = 0: oil terms vanish */
+    fl.ncomp = 1;
+    const double* faces[12] = {sr1x, sr1y, sr1z, sr2x, sr2y, sr2z, NULL, NULL, NULL, NULL, NULL, NULL};
+    or_fill_coefficients_2f(p, act, &fl, NULL, NULL, faces, D, t1, t2, out2);
+}
+
+/* Two-phase signal (NmrRelaxationSolve.cpp:3324-3350): over pore cells of
+ * the trimmed interior whose phase fraction c1 or c2 exceeds 0.8, the sums
+ * of f*c1 and f*c2. */
+void or_output_2f(const or_params* p, const char* act, int bx, int by, int bz, const or_fluids* fl,
+                  const double* conc0, const double* conc1, const double* f, double* s1, double* s2)
+{
+    double a = 0.0, b = 0.0;
+    for (int k = 1 + bz; k < p->kk + 1 - bz; k++)
+        for (int j = 1 + by; j < p->jk + 1 - by; j++)
+            for (int i = 1 + bx; i < p->ik + 1 - bx; i++) {
+                const long c = IX(p, i, j, k);
+                if (act[c] != '1') continue;
+                double c1, c2;
+                or_phase(fl, conc0[c], conc1[c], &c1, &c2);
+                if (c1 > 0.8 || c2 > 0.8) {
+                    a += f[c] * c1;
+                    b += f[c] * c2;
+                }
+            }
+    *s1 = a;
+    *s2 = b;
 }
 
 /* Constant-gradient field Bg = (Gx*dx*(i-1/2), Gy*dy*(j-1/2), Gz*dz*(k-1/2))

@@ -61,14 +61,76 @@ def test_parses_the_reference_data_files(exe, tmp_path):
     assert c["multipleGradients"] == [grad[199], grad[0], grad[398]]
 
 
-def test_refuses_two_phase_before_touching_a_device(exe, tmp_path):
-    """NCOMP=2 is not implemented on the GPU: init() says so and exits 1 (the
-    reference would run on with missing inputs)."""
+def test_shipped_two_phase_config_needs_its_concentration_file(exe, tmp_path):
+    """The shipped configuration is two-phase (NCOMP=2) but ships without
+    data/n1_init.dat: init() says so and exits 1 before touching a device
+    (the reference would run on with uninitialised concentrations)."""
     os.makedirs(tmp_path / "data")
     for f in ("Nmr_data.dat", "Nmr_grad.dat", "Nmr_start.dat"):
         shutil.copy(os.path.join(REFDATA, f), tmp_path / "data" / f)
     r = _run(exe, tmp_path, "1", "1", "1", check=False)
-    assert r.returncode == 1 and "NCOMP=2" in r.stderr
+    assert r.returncode == 1 and "no file data/Poro.dat" in r.stderr  # NPOR=1: the mask is not shipped either
+    with open(tmp_path / "data" / "Poro.dat", "w") as f:
+        f.write("poro\n" + "1 " * 1000 + "\n")
+    r = _run(exe, tmp_path, "1", "1", "1", check=False)
+    assert r.returncode == 1 and "no file data/Vx.dat" in r.stderr  # NVELOC=1
+    for name in ("Vx.dat", "Vy.dat", "Vz.dat"):
+        with open(tmp_path / "data" / name, "w") as f:
+            f.write(name + "\n" + "0 " * 1100 + "\n")
+    r = _run(exe, tmp_path, "1", "1", "1", check=False)
+    assert r.returncode == 1 and "no file data/n1_init.dat" in r.stderr
+
+
+def _two_phase_case(tmp_path, shape=(6, 5, 7), seed=3, **ov):
+    mask = generators.porosity(shape, 0.6, seed=seed, sigma=1.0)
+    rng = np.random.default_rng(seed)
+    n1 = np.clip(rng.random(shape) * 1.2 - 0.1, 0.0, 1.0)
+    nz, ny, nx = shape
+    surf = {"SurfRel1WaterX": rng.uniform(1e-6, 3e-5, (nz, ny, nx + 1)),
+            "SurfRel1WaterY": rng.uniform(1e-6, 3e-5, (nz, ny + 1, nx)),
+            "SurfRel1WaterZ": rng.uniform(1e-6, 3e-5, (nz + 1, ny, nx)),
+            "SurfRel2WaterX": rng.uniform(1e-6, 3e-5, (nz, ny, nx + 1)),
+            "SurfRel2WaterY": rng.uniform(1e-6, 3e-5, (nz, ny + 1, nx)),
+            "SurfRel2WaterZ": rng.uniform(1e-6, 3e-5, (nz + 1, ny, nx))}
+    base = dict(NCOMP=2, nSurfRelWater=1, R1_OIL=4e-6, R2_OIL=8e-6, T1_OIL=0.3, T2_OIL=0.05, T1_WAT=0.6,
+                T2_WAT=0.2, D_OIL=2.1e-10, D_WAT=2.3e-9)
+    base.update(ov)
+    cfg = _write_case(tmp_path, shape, mask, None, n1=n1, surf=surf, **base)
+    return cfg, mask, n1, surf
+
+
+def _two_phase_oracle_coefficients(cfg, mask, n1, surf):
+    from initializator_amd import layout
+
+    p = oracle.params(mask.shape, cfg["DX"], 1.0, dy=cfg["DY"], dz=cfg["DZ"])
+    act = layout.pad_mask(mask)
+    fl = oracle.fluids(d=(cfg["D_WAT"], cfg["D_OIL"]), t1=(cfg["T1_WAT"], cfg["T1_OIL"]),
+                       t2=(cfg["T2_WAT"], cfg["T2_OIL"]), r1=(cfg["R1_WAT"], cfg["R1_OIL"]),
+                       r2=(cfg["R2_WAT"], cfg["R2_OIL"]), ncomp=2, an=(cfg["AN(1)"], cfg["AN(2)"]),
+                       bn=(cfg["BN(1)"], cfg["BN(2)"]))
+    n1f = n1.astype(np.float32).astype(np.float64)  # the reference reads the file into a float
+    conc = (layout.pad_field(n1f), layout.pad_field(1.0 - n1f))
+    faces = []
+    for rel in ("1", "2"):
+        for ax, stag in (("X", "x"), ("Y", "y"), ("Z", "z")):
+            key = f"SurfRel{rel}Water{ax}"
+            faces.append(np.ascontiguousarray(layout.pad_velocity(*[surf[f"SurfRel{rel}Water{a}"] for a in "XYZ"],
+                                                                  mask.shape)["xyz".index(stag)]))
+    faces += [None] * 6
+    return p, act, fl, conc, faces
+
+
+def test_two_phase_inputs_set_the_step_like_the_reference(exe, tmp_path):
+    """n1_init.dat and face relaxivity files are read (i outer, k inner) and
+    mixed as FillInDiffusinAndTime does: the step limits the drop-in derives
+    equal those of the FP64 restatement's per-voxel D and T."""
+    cfg, mask, n1, surf = _two_phase_case(tmp_path, NSEQUENCE=2, TIME1=2e-4, TIME2=5e-4)
+    c = _config(exe, tmp_path, "5", "0", "0", full=True)
+    p, act, fl, conc, faces = _two_phase_oracle_coefficients(cfg, mask, n1, surf)
+    _, _, _, (dmax, tmin) = oracle.fill_coefficients_2f(p, act, fl, conc, faces)
+    assert c["nComponents"] == 2
+    assert c["dTimeDiff"] == pytest.approx(2.0 * cfg["CFL"] * cfg["DZ"] ** 2 / dmax, rel=1e-14)
+    assert c["dTimeRel"] == pytest.approx(cfg["CFL1"] * tmin, rel=1e-14)
 
 
 def test_written_inputs_round_trip(exe, tmp_path):
@@ -107,7 +169,7 @@ def _limits_only(cfg, mask, vel):
     return None, dt, int(end / dt) + 1
 
 
-def _write_case(root, shape, mask, vel, grad_table=None, **ov):
+def _write_case(root, shape, mask, vel, grad_table=None, n1=None, surf=None, **ov):
     cfg = dict(refio.DATA_KEYS)
     cfg = {k.split(" ")[0] if not k.startswith("Printing") else "Printing on": v for k, v in cfg.items()}
     cfg.update({k.replace("_", " ") if k.startswith("Printing") else k: v for k, v in ov.items()})
@@ -116,7 +178,7 @@ def _write_case(root, shape, mask, vel, grad_table=None, **ov):
     if vel is not None:
         cfg["NVELOC"] = 1
     refio.write_inputs(str(root), shape, mask, *(vel if vel is not None else (None, None, None)),
-                       grad_table=grad_table, **ov)
+                       grad_table=grad_table, n1=n1, surf=surf, **ov)
     return cfg
 
 
@@ -228,3 +290,35 @@ def test_refuses_boundary_flags_that_leave_ghosts_stale(exe, tmp_path):
     _write_case(tmp_path, (4, 5, 6), None, None, NPOR=0, NB1=1, NB3=3)
     r = _run(exe, tmp_path, "1", "0", "0", check=False)
     assert r.returncode == 1 and "ghosts undefined" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cadence", ["gpu", "cpu"])
+def test_two_phase_flow_against_fp64_loop(exe, tmp_path, cadence):
+    """Two-phase mixture (NCOMP=2, n1_init.dat) with face-wise water
+    relaxivities and flow, sequence 2: the 7-column results_*.csv rows
+    (SumX1..SumZ1, SumX2..SumZ2) match the FP64 restatement of the
+    reference's loop."""
+    shape = (8, 9, 11)
+    mask = generators.porosity(shape, 0.6, seed=9, sigma=1.2)
+    vel = generators.velocity(mask, seed=9, sigma=1.5, vmax=4e-3)
+    rng = np.random.default_rng(9)
+    n1 = np.clip(generators._smooth_noise(shape, 9, 5, 1.5) * 4.0 + 0.5, 0.0, 1.0)
+    nz, ny, nx = shape
+    surf = {f"SurfRel{r}Water{a}": rng.uniform(1e-6, 3e-5, {"X": (nz, ny, nx + 1), "Y": (nz, ny + 1, nx),
+                                                              "Z": (nz + 1, ny, nx)}[a])
+            for r in "12" for a in "XYZ"}
+    table = np.linspace(-0.03, 0.03, 399)
+    cfg = _write_case(tmp_path, shape, mask, vel, grad_table=table, n1=n1, surf=surf, NCOMP=2, nSurfRelWater=1,
+                      NSEQUENCE=2, TIME1=2e-4, TIME2=4e-4, R1_OIL=4e-6, R2_OIL=8e-6, T1_OIL=0.3, T2_OIL=0.05,
+                      T1_WAT=0.6, T2_WAT=0.2, D_OIL=4e-10, Printing_on=5, BX0=0.0, BZ0=1.0)
+    cfg["gradientPulse"] = float(table[349])
+    args = ["350", "0", "0"] + (["--cpu-cadence"] if cadence == "cpu" else [])
+    _run(exe, tmp_path, *args)
+    rows = refio.read_results(tmp_path / f"results_{cadence}.csv")
+    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=(cadence == "gpu"), n1=n1, surf=surf)
+    assert rows.shape == ref.shape and rows.shape[1] == 7, (rows.shape, ref.shape)
+    scale = np.abs(ref[:, 1:]).max()
+    err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
+    print(f"[dropin] two-phase sequence 2, {cadence} cadence: {len(rows)} rows, max error {err:.2e} of the scale")
+    assert err < 5e-6

@@ -132,3 +132,101 @@ def test_padded_download_ghosts_follow_boundaries():
     np.testing.assert_array_equal(mx[1:-1, 1:-1, -1], f[0][:, :, 0])
     assert np.all(mx[1:-1, 0, 1:-1] == 0) and np.all(mx[1:-1, -1, 1:-1] == 0)  # zero y
     np.testing.assert_array_equal(mx[0, 1:-1, 1:-1], f[0][0])            # zero derivative z
+
+
+def _mixture(case, seed=4):
+    """Smooth two-phase concentration, face-wise relaxivities, and the FP64
+    per-voxel D, T1, T2 of FillInDiffusinAndTime for it."""
+    from initializator_amd import layout
+
+    rng = np.random.default_rng(seed)
+    shape = case.shape
+    p = oracle.params(shape, case.dx, case.dt, precession=case.grad_sched is not None, velocity=case.vx is not None)
+    act = layout.pad_mask(case.mask)
+    x = np.clip(generators._smooth_noise(shape, seed, 7, 2.0) * 3.0 + 0.5, 0.0, 1.0).astype(np.float64)
+    conc = (layout.pad_field(x), layout.pad_field(1.0 - x))
+    faces = [np.ascontiguousarray(rng.uniform(2e-6, 4e-5, oracle.padded_shape(p))) for _ in range(12)]
+    fl = oracle.fluids(d=(2.3e-9, 4e-10), t1=(0.8, 0.3), t2=(0.4, 0.05), ncomp=2)
+    D, t1, t2, _ = oracle.fill_coefficients_2f(p, act, fl, conc, faces)
+    return p, act, D, t1, t2, conc, fl, x
+
+
+@pytest.mark.parametrize("scheme", [UPW, MAK])
+def test_per_voxel_coefficients_two_phase_against_oracle(scheme):
+    """Two-phase mixture with face-wise relaxivities: the GPU takes the
+    per-voxel D, 1/T1, 1/T2 and the phase weights; fields, the plain pore
+    sums and the two-phase weighted sums follow the FP64 restatement."""
+    from initializator_amd import layout
+
+    case = _flow_case((14, 17, 19), 120, seed=21)
+    p, act, D, t1, t2, conc, fl, x = _mixture(case)
+    p.scheme = scheme
+    vel = [np.ascontiguousarray(v) for v in layout.pad_velocity(case.vx, case.vy, case.vz, case.shape)]
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    m = [np.ascontiguousarray(layout.pad_field(a)) for a in m0]
+    # oracle: step by step to get the two-phase sums of every step
+    pc = [np.zeros(oracle.padded_shape(p)) for _ in range(6)]
+    a = [q.copy() for q in m]
+    ref_w = []
+    for f in m:
+        oracle.bc_field(p, f)
+
+    def wrow():
+        s = [oracle.output_2f(p, act, (0, 0, 0), fl, conc, m[q]) for q in range(2)]
+        return [s[0][0], s[1][0], s[0][1], s[1][1]]
+
+    ref_w.append(wrow())
+    for n in range(case.nsteps):
+        nf = oracle.fill_gradient(p, case.grad_sched[n])
+        if scheme == MAK:
+            oracle.maccormack(p, act, D, t1, t2, vel, nf, m, pc)
+        else:
+            oracle.explicit_simple(p, act, D, t1, t2, vel, nf, m, a)
+        for f in m:
+            oracle.bc_field(p, f)
+        ref_w.append(wrow())
+    ref_w = np.array(ref_w)
+    f_ref = [layout.unpad(q) for q in m[:2]]
+    # GPU
+    c1, c2 = np.vectorize(lambda v: oracle.phase(fl, v, 1.0 - v))(x)
+    gate = (c1 > 0.8) | (c2 > 0.8)
+    s = HipSolver.from_case(case, scheme=scheme)
+    s.upload_magnetization(*m0[:2])
+    pore = case.mask != 0
+    s.upload_coefficients(np.where(pore, layout.unpad(D), 0.0), np.where(pore, 1.0 / layout.unpad(t2), 0.0),
+                          r1=np.where(pore, 1.0 / layout.unpad(t1), 0.0))
+    s.upload_phase_weights(np.where(gate, c1, 0.0), np.where(gate, c2, 0.0))
+    sig = s.run(case.nsteps, case.grad_sched[: case.nsteps])
+    f = s.download_magnetization()
+    s.close()
+    e_sig = rel_l2(sig, ref_w)
+    e_vox = max(float(np.abs(u - v).max()) for u, v in zip(f, f_ref))
+    print(f"[schemes] two-phase per-voxel, scheme {scheme}: weighted signal relL2 = {e_sig:.3e}, "
+          f"max voxel = {e_vox:.3e}", flush=True)
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL
+
+
+def test_per_voxel_coefficients_matching_the_uniform_ones():
+    """Uploading the coefficients the mask implies (uniform D; bulk plus
+    rho/d per solid face) reproduces the uniform-coefficient sweep to FP32
+    rounding."""
+    case = _flow_case((12, 15, 22), 80, seed=8)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape)]
+    s = HipSolver.from_case(case)
+    s.upload_magnetization(*m0)
+    sig_a = s.run(case.nsteps, case.grad_sched[: case.nsteps])
+    fa = s.download_magnetization()
+    s.close()
+    pad = np.pad(case.mask, 1, mode="edge")
+    nsolid = sum((pad[1:-1, 1:-1, 1:-1] != 0) & (np.roll(pad, sh, ax)[1:-1, 1:-1, 1:-1] == 0)
+                 for ax in range(3) for sh in (1, -1))
+    r2 = 1.0 / case.t2_bulk + case.rho * nsolid / case.dx
+    r1 = 1.0 / case.t1_bulk + case.rho * nsolid / case.dx
+    s = HipSolver.from_case(case)
+    s.upload_magnetization(*m0)
+    s.upload_coefficients(np.where(case.mask != 0, case.diffusivity, 0.0), r2, r1=r1)
+    sig_b = s.run(case.nsteps, case.grad_sched[: case.nsteps])
+    fb = s.download_magnetization()
+    s.close()
+    assert rel_l2(sig_b, sig_a) < 1e-6
+    assert max(float(np.abs(u - v).max()) for u, v in zip(fa, fb)) < 1e-6
