@@ -423,6 +423,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     const bool xin = ix < nx;
     const int xlast = nx - 1 - ix;             // last valid element of this chunk (>= 3: all valid)
     const bool xend = xin && xlast <= 3;       // chunk holds voxel nx-1
+    const bool xpart = xin && xlast < 3;       // chunk reaches past nx: elements > xlast take the ghost value
     const bool xbeg = (ix == 0);               // chunk holds voxel 0
     const int c = 4 + 4 * lx;                  // LDS column of element 0
 
@@ -483,10 +484,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             kw[r] = 0;
-            if (rin[r]) {
-                kw[r] = ldw_nt(a.mask, pm + rowm[r]);
-                if (xend) kw[r] = clamp_tail_w(kw[r], xlast); // ghost past nx
-            }
+            if (rin[r]) kw[r] = ldw_nt(a.mask, pm + rowm[r]);
         }
     };
     auto load_m = [&](int kk, const uint32_t (&kw)[RPT], f4 (&m)[RPT][NC]) {
@@ -496,10 +494,6 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             if (rin[r] && (!skip_m || kw[r] != 0)) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) m[r][q] = ld4_nt(a.min[q], pf + rowf[r]);
-                if (xend) { // elements past nx take the ghost value (registers and LDS agree)
-#pragma unroll
-                    for (int q = 0; q < NC; q++) m[r][q] = clamp_tail(m[r][q], xlast);
-                }
             } else {
 #pragma unroll
                 for (int q = 0; q < NC; q++) m[r][q] = f4{0.f, 0.f, 0.f, 0.f};
@@ -508,21 +502,40 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     };
     auto load_hmask = [&](int kk) -> uint32_t {
         if (!hrow) return 0;
-        uint32_t w = *reinterpret_cast<const uint32_t*>(a.mask + (uint32_t)clampz(kk) * plane_m + hrowm);
-        return xend ? clamp_tail_w(w, xlast) : w;
+        return *reinterpret_cast<const uint32_t*>(a.mask + (uint32_t)clampz(kk) * plane_m + hrowm);
     };
     auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
         if (hrow && (!skip_m || hk != 0)) {
             const uint32_t pf = (uint32_t)clampz(kk) * plane_b;
 #pragma unroll
             for (int q = 0; q < NC; q++) h[q] = ld4(a.min[q], pf + hrowf);
-            if (xend) {
-#pragma unroll
-                for (int q = 0; q < NC; q++) h[q] = clamp_tail(h[q], xlast);
-            }
         } else {
 #pragma unroll
             for (int q = 0; q < NC; q++) h[q] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    // Elements past nx take the ghost value (registers and LDS agree).  The
+    // loads above return raw rows; the fix is applied where a plane is
+    // rotated in, one iteration after its loads were issued, so that no
+    // wait for memory follows a load directly.
+    auto fix_mask = [&](uint32_t (&kw)[RPT]) {
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+            if (xpart) kw[r] = clamp_tail_w(kw[r], xlast);
+    };
+    auto fix_m = [&](f4 (&m)[RPT][NC]) {
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+            if (xpart) {
+#pragma unroll
+                for (int q = 0; q < NC; q++) m[r][q] = clamp_tail(m[r][q], xlast);
+            }
+    };
+    auto fix_h = [&](uint32_t& hk, f4 (&h)[NC]) {
+        if (xpart) {
+            hk = clamp_tail_w(hk, xlast);
+#pragma unroll
+            for (int q = 0; q < NC; q++) h[q] = clamp_tail(h[q], xlast);
         }
     };
     auto load_xhalo = [&](int kk, float (&xv)[RPT][NC], uint32_t (&kx)[RPT]) {
@@ -567,12 +580,30 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     load_mask(k0 - 1, kp);
     load_mask(k0, kc);
     load_mask(k0 + 1, kn);
-    load_m(k0 - 1, kp, mp);
-    load_m(k0, kc, mc);
     hkc = load_hmask(k0);
     hkn = load_hmask(k0 + 1);
+    fix_mask(kp);
+    fix_mask(kc);
+    fix_mask(kn);
+    if (xpart) {
+        hkc = clamp_tail_w(hkc, xlast);
+        hkn = clamp_tail_w(hkn, xlast);
+    }
+    load_m(k0 - 1, kp, mp);
+    load_m(k0, kc, mc);
     load_hm(k0, hkc, hc);
     load_xhalo(k0, xc, kxc);
+    fix_m(mp);
+    fix_m(mc);
+    if (xpart) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) hc[q] = clamp_tail(hc[q], xlast);
+    }
+    // Retire the prologue's loads here (vmcnt(0)): otherwise the wait the
+    // compiler places for them lands inside the loop, in front of the LDS
+    // staging, where it would also wait for the loads just issued for the
+    // next plane.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 
     const Coef cf = a.c;
     bool trim_x[4];
@@ -690,6 +721,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             kn[r] = kq[r];
             kxc[r] = kxn[r];
         }
+        fix_m(mc);
+        fix_mask(kn);
+        fix_h(hkq, hn);
 #pragma unroll
         for (int q = 0; q < NC; q++) hc[q] = hn[q];
         hkc = hkn;
