@@ -195,8 +195,8 @@ static void boundary_modes(const int nb[6], int out[3])
     }
 }
 
-// Inputs the GPU path implements (one fluid, uniform relaxivity, no random
-// noise field, boundary flags that define every ghost).
+// Inputs the GPU path implements (one or two fluids, any relaxivity files,
+// no random noise field, boundary flags that define every ghost).
 void NmrRelaxationSolve::Validate() const
 {
     if (nComponents != 1 && nComponents != 2) unsupported("NCOMP must be 1 or 2");

@@ -237,3 +237,68 @@ def test_full_size_256_uniform_field_fixed_point():
     assert np.abs(f[0][pores] - 0.75).max() < 1e-6
     assert np.abs(f[1][pores] - 0.25).max() < 1e-6
     np.testing.assert_allclose(sig[:, 0], sig[0, 0], rtol=1e-6)
+
+
+@pytest.mark.parametrize("phi", [0.20, 0.35])
+def test_flow_pgse_5000_steps(phi):
+    """The 256^3 flow workload's physics (v max 1 mm/s, PGSE G = 0.3 T/m,
+    delta 2 ms, Delta 20 ms, complex (Mx, My)) over its full 5000 steps --
+    both gradient lobes and the echo -- on a 36 x 40 x 44 grid the oracle
+    finishes in seconds."""
+    case = generators.case_flow(seed=91 + int(phi * 100), shape=(36, 40, 44), phi=phi, nsteps=5000)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    sig, f = _gpu_run(case, case.nsteps, m0)
+    e_sig, e_vox = _report(f"36x40x44 phi {phi} flow+PGSE 5000 steps", sig, sig_ref[:, :2], f, f_ref[:2])
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
+
+
+def _mirror(mask, vx, vy, vz, axis):
+    """Reflect a compact problem about its upper face along axis (0 = z,
+    1 = y, 2 = x): cells are mirrored, the face velocity normal to the
+    mirror changes sign (the mirror face itself is a wall face, 0)."""
+    def cells(a):
+        return np.concatenate([a, np.flip(a, axis)], axis)
+
+    def normal(a):
+        n = a.shape[axis] - 1
+        assert not np.take(a, [n], axis).any(), "mirror face must carry no flow"
+        return np.concatenate([a, -np.flip(np.take(a, range(n), axis), axis)], axis)
+
+    v = [vx, vy, vz]
+    normal_of = {2: 0, 1: 1, 0: 2}[axis]  # vx is normal to x, vy to y, vz to z
+    v = [normal(a) if i == normal_of else cells(a) for i, a in enumerate(v)]
+    return cells(mask), v[0], v[1], v[2]
+
+
+def test_full_size_512x512x1024_mirror_symmetry_and_reduction():
+    """512 x 512 x 1024 (the largest grid, 268 M voxels, rho = 5 um/s) with
+    flow: a domain built as mirror images of one 256 x 256 x 512 block in x,
+    y and z has a mirror-symmetric solution; after 20 steps the field must
+    match its own reflection along every axis (a wrong tile, chunk or offset
+    at this size breaks it), and the fused reduction equals an FP64 sum of
+    the downloaded field."""
+    base = (512, 256, 256)
+    seed = 0x4E4D52 + 3000
+    mask = generators.porosity(base, 0.20, seed)
+    vx, vy, vz = generators.velocity(mask, seed, vmax=1e-3)
+    for axis in (2, 1, 0):
+        mask, vx, vy, vz = _mirror(mask, vx, vy, vz, axis)
+    assert mask.shape == (1024, 512, 512)
+    dx, D = 1e-6, 2.3e-9
+    case = generators.Case(mask.shape, mask, vx, vy, vz, dx, generators.stable_dt(dx, D), D, 2.0, 2.5e10,
+                           5e-6, 20, None, 2)
+    s = HipSolver.from_case(case)
+    s.upload_magnetization(case.mask.astype(np.float32), 0.3 * case.mask.astype(np.float32))
+    del vx, vy, vz
+    sig = s.run(case.nsteps)
+    f = s.download_magnetization()
+    s.close()
+    for q, x in enumerate(f):
+        for axis in (0, 1, 2):
+            d = float(np.abs(x - np.flip(x, axis)).max())
+            print(f"[parity] 512x512x1024 M{'xy'[q]} mirror axis {axis}: max asymmetry {d:.3e}", flush=True)
+            assert d <= 1e-6, (q, axis, d)
+    brute = [float(np.sum(x * case.mask, dtype=np.float64)) for x in f]
+    np.testing.assert_allclose(sig[-1], brute, rtol=1e-10)
+    assert sig[-1][0] < sig[0][0]  # surface relaxation removes magnetization
