@@ -213,8 +213,6 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
     if ((row + 1) * (size_t)c->nsig() > c->dsignal_cap)
         return fail(NMR_ERR_STATE, "signal buffer holds %zu values, row %zu needs %zu", c->dsignal_cap, row,
                     (row + 1) * (size_t)c->nsig());
-    int rc = ensure_partials(c, (size_t)p.nz * c->nsig());
-    if (rc) return rc;
     nmr::FieldSumArgs a{};
     for (int q = 0; q < 3; q++) a.f[q] = c->m[c->cur][q];
     a.mask = c->mask;
@@ -230,14 +228,22 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
     int w[6];
     signal_window(p, w);
     a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
-    HIP_TRY(nmr::launch_field_sum(c->nc, a, p.nz, c->stream));
-    // one "step" whose nwg = nz partials
-    HIP_TRY(nmr::launch_reduce_partials(c->partials, 1, p.nz, c->nsig(), c->dsignal + row * c->nsig(), c->stream));
+    const int nfb = nmr::field_sum_blocks(a);
+    int rc = ensure_partials(c, (size_t)nfb * c->nsig());
+    if (rc) return rc;
+    a.part = c->partials;
+    HIP_TRY(nmr::launch_field_sum(c->nc, a, c->stream));
+    // one "step" whose workgroups are the field sum's
+    HIP_TRY(nmr::launch_reduce_partials(c->partials, 1, nfb, c->nsig(), c->dsignal + row * c->nsig(), c->stream));
     return NMR_OK;
 }
 
-// Steps of the per-voxel kernel (MacCormack and/or non-zero-derivative
-// boundaries), each followed by the signal of the new state.
+int refresh_inert(nmr_ctx* c);
+
+// Steps of the general problems (MacCormack, non-zero-derivative boundaries,
+// per-voxel coefficients, two-phase sums): the general z-sweep, or with
+// variant -2 the per-voxel kernel; each step's signal is fused into the sweep
+// (plain sums) or taken by the field sum (phase-weighted sums).
 int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
 {
     const nmr_problem& p = ctx->prob;
@@ -245,42 +251,129 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     const int64_t rows = nsteps + 1;
     int rc = ensure_signal(ctx, (size_t)rows * ns);
     if (rc) return rc;
-    if ((rc = ensure_partials(ctx, (size_t)p.nz * ns))) return rc;
-    nmr::GenArgs a{};
-    a.mask = ctx->mask;
-    a.vx = ctx->vx;
-    a.vy = ctx->vy;
-    a.vz = ctx->vz;
-    a.nx = p.nx;
-    a.ny = p.ny;
-    a.nz = p.nz;
-    a.pitch = ctx->pitch;
-    a.pitch1 = ctx->pitch1;
-    for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
-    a.vel = ctx->vel ? 1 : 0;
-    a.dn = ctx->dn;
-    a.r1v = ctx->r1v;
-    a.r2v = ctx->r2v;
     const bool mak = p.scheme == NMR_SCHEME_MACCORMACK;
-    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
-    for (int64_t s = 0; s < nsteps; s++) {
+    const bool pervoxel = ctx->variant == -2;
+    if (!pervoxel && (rc = refresh_inert(ctx))) return rc;
+    auto coef_of = [&](int64_t step) {
         double g[3] = {p.grad[0], p.grad[1], p.grad[2]};
         if (grad_sched)
-            for (int q = 0; q < 3; q++) g[q] = grad_sched[3 * s + q];
-        a.c = make_coef(ctx, g);
-        for (int q = 0; q < 3; q++) {
-            a.m[q] = ctx->m[ctx->cur][q];
-            a.p[q] = ctx->pred[q];
-            a.out[q] = mak ? ctx->pred[q] : ctx->m[ctx->cur ^ 1][q];
+            for (int q = 0; q < 3; q++) g[q] = grad_sched[3 * step + q];
+        return make_coef(ctx, g);
+    };
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    if (pervoxel) {
+        nmr::GenArgs a{};
+        a.mask = ctx->mask;
+        a.vx = ctx->vx;
+        a.vy = ctx->vy;
+        a.vz = ctx->vz;
+        a.nx = p.nx;
+        a.ny = p.ny;
+        a.nz = p.nz;
+        a.pitch = ctx->pitch;
+        a.pitch1 = ctx->pitch1;
+        for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
+        a.vel = ctx->vel ? 1 : 0;
+        a.dn = ctx->dn;
+        a.r1v = ctx->r1v;
+        a.r2v = ctx->r2v;
+        for (int64_t s = 0; s < nsteps; s++) {
+            a.c = coef_of(s);
+            for (int q = 0; q < 3; q++) {
+                a.m[q] = ctx->m[ctx->cur][q];
+                a.p[q] = ctx->pred[q];
+                a.out[q] = mak ? ctx->pred[q] : ctx->m[ctx->cur ^ 1][q];
+            }
+            HIP_TRY(nmr::launch_general(nc, mak ? nmr::GEN_PREDICT : nmr::GEN_UPWIND, a, ctx->stream));
+            if (mak) {
+                for (int q = 0; q < 3; q++) a.out[q] = ctx->m[ctx->cur ^ 1][q];
+                HIP_TRY(nmr::launch_general(nc, nmr::GEN_CORRECT, a, ctx->stream));
+            }
+            ctx->cur ^= 1;
+            if ((rc = enqueue_field_sum(ctx, (size_t)s + 1))) return rc;
         }
-        HIP_TRY(nmr::launch_general(nc, mak ? nmr::GEN_PREDICT : nmr::GEN_UPWIND, a, ctx->stream));
-        if (mak) {
-            for (int q = 0; q < 3; q++) a.out[q] = ctx->m[ctx->cur ^ 1][q];
-            HIP_TRY(nmr::launch_general(nc, nmr::GEN_CORRECT, a, ctx->stream));
+    } else {
+        // The corrector leaves 0 at solids but skips all-solid chunks: the
+        // first step's output buffer is cleared at solids before the step and
+        // its input buffer after it (the first step still reads the input's
+        // solid values, e.g. through periodic ghosts).
+        const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
+        if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
+        // tile geometry: 256 threads, 4*lanes x (256/lanes) voxels, z-chunks
+        // of lz planes with >= 2048 workgroups where the grid allows
+        const int lanes = nmr::gsweep_lanes(p.nx);
+        const int tx = 4 * lanes, ty = 256 / lanes;
+        const long ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty;
+        int lz = ctx->lz;
+        if (lz <= 0) {
+            const long want = 2048, tiles_xy = ntx * nty;
+            const long ntz0 = std::max<long>(1, (want + tiles_xy - 1) / tiles_xy);
+            lz = (int)std::max<long>(8, (p.nz + ntz0 - 1) / ntz0);
         }
-        ctx->cur ^= 1;
-        if ((rc = enqueue_field_sum(ctx, (size_t)s + 1))) return rc;
+        lz = std::min(std::max(lz, 1), p.nz);
+        const long ntz = (p.nz + lz - 1) / lz;
+        const long nwg = ntx * nty * ntz;
+        if (nwg > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
+        const bool fused = !ctx->weighted;
+        if (fused && (rc = ensure_partials(ctx, (size_t)kChunkSteps * nwg * nc))) return rc;
+        int w[6];
+        signal_window(p, w);
+        nmr::GSweepArgs a{};
+        a.mask = ctx->mask;
+        a.vx = ctx->vx;
+        a.vy = ctx->vy;
+        a.vz = ctx->vz;
+        a.dn = ctx->dn;
+        a.r1v = ctx->r1v;
+        a.r2v = ctx->r2v;
+        a.nx = p.nx;
+        a.ny = p.ny;
+        a.nz = p.nz;
+        a.pitch = ctx->pitch;
+        a.pitch1 = ctx->pitch1;
+        a.ntx = (int)ntx;
+        a.nty = (int)nty;
+        a.ntz = (int)ntz;
+        a.lz = lz;
+        a.nblocks = (int)nwg;
+        a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
+        for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
+        a.skip_m = ctx->inert;
+        const bool coef = ctx->hetero;
+        int64_t done = 0;
+        while (done < nsteps) {
+            const int chunk = (int)std::min<int64_t>(kChunkSteps, nsteps - done);
+            for (int s = 0; s < chunk; s++) {
+                a.c = coef_of(done + s);
+                if (mak) {
+                    for (int q = 0; q < 3; q++) {
+                        a.f[q] = ctx->m[ctx->cur][q];
+                        a.out[q] = ctx->pred[q];
+                    }
+                    a.partials = nullptr;
+                    HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, nmr::GEN_PREDICT, coef, lanes, a, ctx->stream));
+                }
+                for (int q = 0; q < 3; q++) {
+                    a.f[q] = mak ? ctx->pred[q] : ctx->m[ctx->cur][q];
+                    a.mavg[q] = ctx->m[ctx->cur][q];
+                    a.out[q] = ctx->m[ctx->cur ^ 1][q];
+                }
+                a.partials = fused ? ctx->partials + (size_t)s * nwg * nc : nullptr;
+                HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, mak ? nmr::GEN_CORRECT : nmr::GEN_UPWIND, coef, lanes, a,
+                                           ctx->stream));
+                ctx->cur ^= 1;
+                if (zero_first && done + s == 0) {
+                    HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
+                    ctx->solids_zeroed = true;
+                }
+                if (!fused && (rc = enqueue_field_sum(ctx, (size_t)(done + s) + 1))) return rc;
+            }
+            if (fused)
+                HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)nwg, nc,
+                                                    ctx->dsignal + (size_t)(done + 1) * nc, ctx->stream));
+            done += chunk;
+        }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     if (signal)
@@ -489,6 +582,7 @@ int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout)
     HIP_TRY(hipMemcpyAsync(ctx->mask, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->has_mask = true;
+    ctx->solids_zeroed = false;
     ctx->mask_version += 1;
     ctx->inert_dirty = true;
     return NMR_OK;
@@ -575,6 +669,7 @@ int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* m
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     ctx->has_mag = true;
+    ctx->solids_zeroed = false;
     return NMR_OK;
 }
 
@@ -1129,14 +1224,14 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
         return bail(fail(NMR_ERR_DEVICE, "stream/event creation failed"));
     int rc;
-    int maxnz = 1;
+    size_t maxrows = 1; // y*z rows of the largest instance (field-sum partials)
     for (int i = 0; i < ninst; i++) {
         const nmr_batch_instance& s = inst[i];
         auto& I = b->inst[i];
         I.pitch = round_up4(s.nx);
         I.pitch1 = round_up4(s.nx + 1);
         I.n_m = (size_t)s.nz * s.ny * I.pitch;
-        maxnz = std::max(maxnz, s.nz);
+        maxrows = std::max(maxrows, (size_t)s.ny * (size_t)s.nz);
         for (int par = 0; par < 2; par++)
             for (int q = 0; q < b->nc; q++)
                 if ((rc = balloc(&I.m[par][q], I.n_m))) return bail(rc);
@@ -1149,7 +1244,7 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
                 return bail(rc);
         }
     }
-    if ((rc = balloc(&b->d_step, 1)) || (rc = balloc(&b->d_fpart, (size_t)maxnz * 3)) || (rc = balloc(&b->d_flag, 16)))
+    if ((rc = balloc(&b->d_step, 1)) || (rc = balloc(&b->d_fpart, (maxrows / 16 + 2) * 3)) || (rc = balloc(&b->d_flag, 16)))
         return bail(rc);
     *out = b;
     return NMR_OK;
@@ -1396,8 +1491,9 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
         a.bx0 = b->prob.buffer[0]; a.bx1 = s.nx - b->prob.buffer[0];
         a.by0 = b->prob.buffer[1]; a.by1 = s.ny - b->prob.buffer[1];
         a.bz0 = b->prob.buffer[2]; a.bz1 = s.nz - b->prob.buffer[2];
-        HIP_TRY(nmr::launch_field_sum(nc, a, s.nz, b->stream));
-        HIP_TRY(nmr::launch_reduce_partials(b->d_fpart, 1, s.nz, nc, b->d_signal + (size_t)i * rows * nc, b->stream));
+        const int nfb = nmr::field_sum_blocks(a);
+        HIP_TRY(nmr::launch_field_sum(nc, a, b->stream));
+        HIP_TRY(nmr::launch_reduce_partials(b->d_fpart, 1, nfb, nc, b->d_signal + (size_t)i * rows * nc, b->stream));
     }
     // the device step counter holds the global step (the gradient schedule
     // reads it); signal rows count from this run's first step

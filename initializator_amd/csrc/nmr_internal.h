@@ -50,6 +50,7 @@ struct nmr_ctx {
     float* w[2] = {nullptr, nullptr};
     double d_ref = 0.0;
     bool hetero = false, weighted = false;
+    bool solids_zeroed = false; // MacCormack: both state buffers hold 0 at solids
     int nsig() const { return weighted ? 2 * nc : nc; }
     uint8_t* mask = nullptr;
     float* vx = nullptr;

@@ -110,6 +110,31 @@ struct GenArgs {
     Coef c;
 };
 
+// General z-sweep: the sweep's tile march (planes in registers, plane k
+// with its halo in LDS) for what the plain sweep does not cover -- the
+// MacCormack stages, periodic and zero-value ghosts, per-voxel
+// coefficients -- with the per-voxel kernel's arithmetic, so that the two
+// agree bit for bit.
+struct GSweepArgs {
+    const float* f[3];      // field the stage differentiates (the state, or the predictor)
+    const float* mavg[3];   // corrector stage: the state at the start of the step
+    float* out[3];          // result of the stage
+    const uint8_t* mask;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    const float* dn;        // per-voxel diffusivity / reference (coefficient variant)
+    const float* r1v;       // per-voxel dt/T1
+    const float* r2v;       // per-voxel dt/T2
+    double* partials;       // [nblocks][nc] pore sums of the result, or null
+    int nx, ny, nz, pitch, pitch1;
+    int ntx, nty, ntz, lz, nblocks;
+    int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
+    int bc[3];              // BC_* per axis
+    int skip_m;             // all-solid chunks are never read by a pore: not loaded
+    Coef c;
+};
+
 struct PulseArgs {
     float* m[3];
     const uint8_t* mask;
@@ -123,7 +148,7 @@ struct FieldSumArgs {
     const float* f[3];
     const uint8_t* mask;
     const float* w[2];      // phase weights (null: plain sums)
-    double* part;           // [nz][nc] or, with weights, [nz][2*nc]
+    double* part;           // [field_sum_blocks][nc] or, with weights, [..][2*nc]
     int nx, ny, nz, pitch;
     int bx0, bx1, by0, by1, bz0, bz1;
 };
@@ -220,6 +245,13 @@ hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int n
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
 hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
-hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s);
+// lanes per tile row of the general sweep for a grid of width nx (tile 4*lanes wide)
+int gsweep_lanes(int nx);
+hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s);
+// m[q][i] = 0 where mask[i] == 0, i < n, for q < nc
+hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s);
+// workgroups (= partials) of a field sum over the signal window
+int field_sum_blocks(const FieldSumArgs& a);
+hipError_t launch_field_sum(int nc, const FieldSumArgs& a, hipStream_t s);
 
 } // namespace nmr

@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "nmr_kernels.h"
 
 
@@ -1191,47 +1193,554 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __re
     }
 }
 
-// Pore sum of a field over the trimmed interior: one block per z-plane
-// writes part[k*NC + q]; nmr_reduce_partials-style second pass sums planes.
+// Pore sum of a field over the signal window: each workgroup takes 16 rows
+// (y, z) of the window, a wavefront one row at a time with 64 consecutive
+// x per load, and writes part[blockIdx.x*NS + q]; a second pass
+// (reduce_partials_kernel) sums the workgroups in a fixed order.
+constexpr int kFieldSumRows = 16;
 template <int NC, bool W>
 __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
 {
     constexpr int NS = W ? 2 * NC : NC;
-    __shared__ double sh[256];
-    const int k = blockIdx.x;
+    __shared__ double sred[4][NS];
+    const int wy = a.by1 - a.by0, wz = a.bz1 - a.bz0;
+    const long nrows = (wy > 0 && wz > 0) ? (long)wy * wz : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) acc[q] = 0.0;
-    if (k >= a.bz0 && k < a.bz1) {
-        const size_t base = (size_t)k * (size_t)a.ny * (size_t)a.pitch;
-        for (int y = a.by0; y < a.by1; y++) {
-            const size_t row = base + (size_t)y * a.pitch;
-            for (int x = a.bx0 + (int)threadIdx.x; x < a.bx1; x += 256) {
-                if (a.mask[row + x] == 0) continue;
-                if constexpr (W) {
-                    const double w1 = a.w[0][row + x], w2 = a.w[1][row + x];
+    const long r1 = min(nrows, (long)(blockIdx.x + 1) * kFieldSumRows);
+    for (long r = (long)blockIdx.x * kFieldSumRows + wave; r < r1; r += 4) {
+        const int y = a.by0 + (int)(r % wy), z = a.bz0 + (int)(r / wy);
+        const size_t row = ((size_t)z * (size_t)a.ny + (size_t)y) * (size_t)a.pitch;
+        for (int x = a.bx0 + lane; x < a.bx1; x += 64) {
+            if (a.mask[row + x] == 0) continue;
+            if constexpr (W) {
+                const double w1 = a.w[0][row + x], w2 = a.w[1][row + x];
 #pragma unroll
-                    for (int q = 0; q < NC; q++) {
-                        const double v = a.f[q][row + x];
-                        acc[q] += v * w1;
-                        acc[NC + q] += v * w2;
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
+                for (int q = 0; q < NC; q++) {
+                    const double v = a.f[q][row + x];
+                    acc[q] += v * w1;
+                    acc[NC + q] += v * w2;
                 }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
             }
         }
     }
-    for (int q = 0; q < NS; q++) {
-        sh[threadIdx.x] = acc[q];
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-            __syncthreads();
+    block_reduce<NS, 256>(acc, sred, a.part + (size_t)blockIdx.x * NS);
+}
+
+// ---------------------------------------------------------------------------
+// General z-sweep (MacCormack stages, any ghost rule, per-voxel coefficients).
+
+// Update of one 4-voxel chunk with the per-voxel kernel's arithmetic
+// (general_kernel): stage ST of a step, face weights from the mask and, with
+// COEF, harmonic means of per-voxel diffusivities; ghost neighbours already
+// hold their rule's value and the pore mask.  dxl / dxr: diffusivity of the x
+// neighbours of elements 0 and 3; mav: the state at the start of the step
+// (corrector stage).
+template <int NC, bool VEL, int ST, bool COEF>
+__device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC], const f4 (&fzm)[NC],
+                                              const f4 (&fzp)[NC], const f4 (&fym)[NC], const f4 (&fyp)[NC],
+                                              const float (&xl)[NC], const float (&xr)[NC], uint32_t kc,
+                                              uint32_t kxm, uint32_t kxp, uint32_t kym, uint32_t kyp,
+                                              uint32_t kzm, uint32_t kzp, const f4& vx4, float vxe, const f4& vy4,
+                                              const f4& vy4n, const f4& vz0, const f4& vz1, const f4& dc,
+                                              float dxl, float dxr, const f4& dym, const f4& dyp, const f4& dzm,
+                                              const f4& dzp, bool r1p, const f4& r1c, const f4& r2c,
+                                              const f4 (&mav)[NC], int ix, float thy, float thz, f4 (&out)[NC])
+{
+    const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
+    const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const float kn[6] = {ub(kxm, e), ub(kxp, e), ub(kym, e), ub(kyp, e), ub(kzm, e), ub(kzp, e)};
+        float w[6];
+        if constexpr (COEF) {
+            const float d0 = dc[e];
+            const float dnb[6] = {e > 0 ? dc[e - 1] : dxl, e < 3 ? dc[e + 1] : dxr, dym[e], dyp[e], dzm[e], dzp[e]};
+#pragma unroll
+            for (int s = 0; s < 6; s++) {
+                const float hm = (d0 + dnb[s]) > 0.f ? 2.f * d0 * dnb[s] / (d0 + dnb[s]) : 0.f;
+                w[s] = cd[s / 2] * hm * kn[s];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 6; s++) w[s] = cd[s / 2] * kn[s];
         }
-        if (threadIdx.x == 0) a.part[(size_t)k * NS + q] = sh[0];
+        if constexpr (VEL) {
+            const float um = vx4[e], up = (e < 3) ? vx4[e + 1] : vxe;
+            const float vm = vy4[e], vp = vy4n[e];
+            const float wm = vz0[e], wp = vz1[e];
+            const float su = um + up, sv = vm + vp, sw = wm + wp;
+            if constexpr (ST == GEN_UPWIND) {
+                w[0] = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
+                w[1] = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
+                w[2] = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
+                w[3] = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
+                w[4] = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
+                w[5] = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
+            } else if constexpr (ST == GEN_PREDICT) {
+                w[0] = fmaf(kn[0] * hxh, su, w[0]);
+                w[2] = fmaf(kn[2] * hyh, sv, w[2]);
+                w[4] = fmaf(kn[4] * hzh, sw, w[4]);
+            } else {
+                w[1] = fmaf(-kn[1] * hxh, su, w[1]);
+                w[3] = fmaf(-kn[3] * hyh, sv, w[3]);
+                w[5] = fmaf(-kn[5] * hzh, sw, w[5]);
+            }
+        }
+        const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
+        float rate2;
+        if constexpr (COEF) rate2 = r2c[e];
+        else rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+        float m0[NC], d[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const float m = mc[q][e];
+            m0[q] = m;
+            const float f[6] = {(e > 0) ? mc[q][e - 1] : xl[q], (e < 3) ? mc[q][e + 1] : xr[q], fym[q][e], fyp[q][e],
+                                fzm[q][e], fzp[q][e]};
+            float t = w[0] * (f[0] - m);
+            t = fmaf(w[1], f[1] - m, t);
+            t = fmaf(w[2], f[2] - m, t);
+            t = fmaf(w[3], f[3] - m, t);
+            t = fmaf(w[4], f[4] - m, t);
+            d[q] = fmaf(w[5], f[5] - m, t);
+        }
+        if constexpr (NC == 3) {
+            float rate1;
+            if (COEF && r1p) rate1 = r1c[e];
+            else rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
+            const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
+            const float nn[3] = {cf.n0, cf.n1, cf.n2};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float mpar = nn[q] * par;
+                d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
+                d[q] = fmaf(rate2, mpar - m0[q], d[q]);
+            }
+            const float thx = cf.bgx * ((float)(ix + e) + 0.5f);
+            d[0] = fmaf(thz, m0[1], fmaf(-thy, m0[2], d[0]));
+            d[1] = fmaf(-thz, m0[0], fmaf(thx, m0[2], d[1]));
+            d[2] = fmaf(thy, m0[0], fmaf(-thx, m0[1], d[2]));
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
+            if constexpr (NC == 2) {
+                d[0] = fmaf(thz, m0[1], d[0]);
+                d[1] = fmaf(-thz, m0[0], d[1]);
+            }
+        }
+        const bool pore = ((kc >> (8 * e)) & 0xffu) != 0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            if constexpr (ST == GEN_CORRECT) out[q][e] = pore ? 0.5f * (mav[q][e] + (m0[q] + d[q])) : 0.f;
+            else if constexpr (ST == GEN_PREDICT) out[q][e] = pore ? m0[q] + d[q] : 0.f; // the predictor holds 0 at solids
+            else out[q][e] = pore ? m0[q] + d[q] : m0[q];
+        }
+    }
+}
+
+// Elements past nx (index > last) take the ghost value g.
+__device__ __forceinline__ f4 fill_tail(f4 v, int last, float g)
+{
+    if (last < 3) v.w = g;
+    if (last < 2) v.z = g;
+    if (last < 1) v.y = g;
+    return v;
+}
+
+// One tile of the general sweep: 256 threads, LX lanes x RY rows, one row
+// per thread, a z-chunk of lz planes.  Ghost rules per axis: zero
+// derivative (the edge value), periodic (the opposite edge) or zero value;
+// every ghost carries the mask and diffusivity of the voxel it borders (the
+// reference's ghost copies of actCell and diffusion), i.e. it is a pore for
+// every voxel that is updated.  Ghost rows and columns are staged in LDS by
+// the threads that own the tile's halo; ghost planes are loaded in place of
+// the out-of-range planes.
+template <int NC, bool VEL, int ST, bool COEF, int LX>
+__device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, double (&acc)[NC])
+{
+    constexpr int NT = 256;
+    constexpr int RY = NT / LX;
+    constexpr int TX = 4 * LX;
+    constexpr int TY = RY;
+    constexpr int LW = TX + 8;
+    constexpr int LH = TY + 2;
+    constexpr int DH = COEF ? LH : 1, DW = COEF ? LW : 4;
+
+    __shared__ __attribute__((aligned(16))) float sm[2][NC][LH][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t sk[2][LH][LW];
+    __shared__ __attribute__((aligned(16))) float sd[2][DH][DW];
+
+    const int tid = threadIdx.x;
+    const int lx = tid % LX;
+    const int ry = tid / LX;
+    const int tx = t % a.ntx;
+    const int tyz = t / a.ntx;
+    const int ty = tyz % a.nty;
+    const int tz = tyz / a.nty;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int k0 = tz * a.lz;
+    const int k1 = min(k0 + a.lz, a.nz);
+    const int ix = x0 + 4 * lx;
+    const int iy = y0 + ry;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int bcx = a.bc[0], bcy = a.bc[1], bcz = a.bc[2];
+    const bool xin = ix < nx;
+    const int xlast = nx - 1 - ix;
+    const bool xend = xin && xlast <= 3;
+    const bool xpart = xin && xlast < 3;
+    const bool xbeg = ix == 0;
+    const bool rin = xin && iy < ny;
+    const int c = 4 + 4 * lx;
+    const int lr = 1 + ry;
+
+    const uint32_t Pb = (uint32_t)a.pitch * 4u;
+    const uint32_t plane_b = (uint32_t)ny * Pb;
+    const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
+    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
+    const uint32_t planex_b = (uint32_t)ny * P1b;
+    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
+    const int yc = iy < ny ? iy : ny - 1; // rows past ny never load
+    const uint32_t rowf = (uint32_t)yc * Pb + (uint32_t)(xin ? ix : 0) * 4u;
+    const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
+    const uint32_t rowx = (uint32_t)yc * P1b + (uint32_t)(xin ? ix : 0) * 4u;
+
+    // halo / ghost row: the top thread stages row y0-1, the bottom thread
+    // row y0+TY, or the ghost row ny at LDS row 1 + (ny - y0) when the tile
+    // reaches the domain end.  Values come from row hsrc (-1: zeros), mask
+    // and diffusivity from row hmrow.
+    const bool top = ry == 0, bot = ry == RY - 1;
+    int hsrc = -1, hmrow = 0, hlr = 0;
+    if (top) {
+        hlr = 0;
+        if (y0 > 0) hsrc = hmrow = y0 - 1;
+        else {
+            hmrow = 0;
+            hsrc = bcy == BC_NEUMANN ? 0 : (bcy == BC_PERIODIC ? ny - 1 : -1);
+        }
+    } else if (bot) {
+        const int yb = y0 + TY;
+        if (yb < ny) {
+            hsrc = hmrow = yb;
+            hlr = TY + 1;
+        } else {
+            hmrow = ny - 1;
+            hsrc = bcy == BC_NEUMANN ? ny - 1 : (bcy == BC_PERIODIC ? 0 : -1);
+            hlr = 1 + (ny - y0);
+        }
+    }
+    const bool hrow = (top || bot) && xin;
+    const uint32_t hrowf = (uint32_t)(hsrc >= 0 ? hsrc : 0) * Pb + (uint32_t)(xin ? ix : 0) * 4u;
+    const uint32_t hrowm = (uint32_t)hmrow * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
+
+    // x halo column: lane 0 loads column x0-1 (or, for periodic x in the
+    // first tile, the ghost value at nx-1), lane LX-1 column x0+TX when it
+    // is inside the domain.  The lane holding nx-1 loads the periodic ghost
+    // value of column 0.
+    const bool xgl = lx == 0 && x0 == 0;                 // left column is the domain ghost
+    const bool hcl = lx == 0 && (x0 > 0 || bcx == BC_PERIODIC);
+    const bool hcr = lx == LX - 1 && x0 + TX < nx;
+    const int hcx = lx == 0 ? (x0 > 0 ? x0 - 1 : nx - 1) : x0 + TX;
+    const bool gxr = xend && bcx == BC_PERIODIC;         // right ghost value from column 0
+
+    const bool skip_m = (ST != GEN_UPWIND) || a.skip_m != 0;
+    auto clampz = [&](int kk) { return kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk); };
+    auto zsrc = [&](int kk) -> int {
+        if (kk >= 0 && kk < nz) return kk;
+        if (bcz == BC_NEUMANN) return kk < 0 ? 0 : nz - 1;
+        if (bcz == BC_PERIODIC) return kk < 0 ? nz - 1 : 0;
+        return -1;
+    };
+
+    f4 mp[NC], mc[NC], mn[NC], hc[NC], hn[NC];
+    uint32_t kp = 0, kc = 0, kn = 0, kq = 0, hkc = 0, hkn = 0, hkq = 0;
+    float xc[NC], xn[NC], gc[NC], gn[NC];
+    uint32_t kxc = 0, kxn = 0;
+    f4 dp = f4{0.f, 0.f, 0.f, 0.f}, dcv = dp, dnv = dp, hdc = dp, hdn = dp;
+    float xdc = 0.f, xdn = 0.f;
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        mp[q] = mc[q] = mn[q] = hc[q] = hn[q] = f4{0.f, 0.f, 0.f, 0.f};
+        xc[q] = xn[q] = gc[q] = gn[q] = 0.f;
+    }
+
+    auto load_mask = [&](int kk) -> uint32_t {
+        return xin ? ldw_nt(a.mask, (uint32_t)clampz(kk) * plane_m + rowm) : 0u;
+    };
+    auto load_m = [&](int kk, uint32_t kw, f4 (&m)[NC]) {
+        const int zs = zsrc(kk);
+        if (xin && zs >= 0 && (!skip_m || kw != 0)) {
+            const uint32_t of = (uint32_t)zs * plane_b + rowf;
+#pragma unroll
+            for (int q = 0; q < NC; q++) m[q] = ld4_nt(a.f[q], of);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) m[q] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    auto load_d = [&](int kk) -> f4 {
+        if constexpr (COEF) {
+            if (xin) return ld4_nt(a.dn, (uint32_t)clampz(kk) * plane_b + rowf);
+        }
+        return f4{0.f, 0.f, 0.f, 0.f};
+    };
+    auto load_hmask = [&](int kk) -> uint32_t {
+        return hrow ? *reinterpret_cast<const uint32_t*>(a.mask + (uint32_t)clampz(kk) * plane_m + hrowm) : 0u;
+    };
+    auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
+        const int zs = zsrc(kk);
+        if (hrow && hsrc >= 0 && zs >= 0 && (!skip_m || hk != 0)) {
+            const uint32_t of = (uint32_t)zs * plane_b + hrowf;
+#pragma unroll
+            for (int q = 0; q < NC; q++) h[q] = ld4(a.f[q], of);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) h[q] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    auto load_hd = [&](int kk) -> f4 {
+        if constexpr (COEF) {
+            if (hrow) return ld4(a.dn, (uint32_t)clampz(kk) * plane_b + (uint32_t)hmrow * Pb + (uint32_t)ix * 4u);
+        }
+        return f4{0.f, 0.f, 0.f, 0.f};
+    };
+    // x halo (value, mask, diffusivity) of plane kk, and the periodic right ghost
+    auto load_xhalo = [&](int kk, float (&xv)[NC], uint32_t& kx, float& xd, float (&gv)[NC]) {
+        const int zs = zsrc(kk);
+        if ((hcl || hcr) && iy < ny) {
+            if (zs >= 0) {
+                const uint32_t of = (uint32_t)zs * plane_b + (uint32_t)iy * Pb + (uint32_t)hcx * 4u;
+#pragma unroll
+                for (int q = 0; q < NC; q++) xv[q] = ld1(a.f[q], of);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) xv[q] = 0.f;
+            }
+            if (!xgl) {
+                const uint32_t cz = (uint32_t)clampz(kk);
+                kx = a.mask[cz * plane_m + (uint32_t)iy * (uint32_t)a.pitch + (uint32_t)hcx];
+                if constexpr (COEF) xd = ld1(a.dn, cz * plane_b + (uint32_t)iy * Pb + (uint32_t)hcx * 4u);
+            }
+        }
+        if (gxr && iy < ny) {
+            if (zs >= 0) {
+                const uint32_t of = (uint32_t)zs * plane_b + (uint32_t)iy * Pb;
+#pragma unroll
+                for (int q = 0; q < NC; q++) gv[q] = ld1(a.f[q], of);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) gv[q] = 0.f;
+            }
+        }
+    };
+    // ghost value past nx for the tail of a chunk and LDS column c+4
+    auto right_ghost = [&](const f4 (&m)[NC], const float (&gv)[NC], int q) -> float {
+        if (bcx == BC_PERIODIC) return gv[q];
+        if (bcx == BC_DIRICHLET) return 0.f;
+        const int xl = xlast < 3 ? xlast : 3;
+        return xl == 0 ? m[q].x : (xl == 1 ? m[q].y : (xl == 2 ? m[q].z : m[q].w));
+    };
+    // Ghost fix of the centre plane's registers: the element past nx takes
+    // the x ghost value, its mask and diffusivity those of element xlast.
+    // Applied one iteration after the plane's loads were issued, so that no
+    // wait for memory follows a load directly.  Only the centre plane's tail
+    // is ever read by a voxel inside the domain.
+    auto fix_centre = [&](f4 (&m)[NC], uint32_t& kw, f4& dv, const float (&gv)[NC]) {
+        if (xpart) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) m[q] = fill_tail(m[q], xlast, right_ghost(m, gv, q));
+            kw = clamp_tail_w(kw, xlast);
+            if constexpr (COEF) dv = clamp_tail(dv, xlast);
+        }
+    };
+
+    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, wc = vx4, wn = vx4, r1c = vx4, r2c = vx4;
+    float vxe = 0.f;
+    f4 mav[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) mav[q] = f4{0.f, 0.f, 0.f, 0.f};
+    // per-voxel inputs of plane kk (own chunk; only chunks holding a pore)
+    auto load_own = [&](int kk, uint32_t kw) {
+        if (rin && kw != 0) {
+            const uint32_t pz = (uint32_t)kk * plane_b;
+            if constexpr (VEL) {
+                const uint32_t px = (uint32_t)kk * planex_b, py = (uint32_t)kk * planey_b;
+                vx4 = ld4_nt(a.vx, px + rowx);
+                vxe = (xlast >= 3) ? ld1(a.vx, px + rowx + 16u) : 0.f;
+                vy4 = ld4_nt(a.vy, py + rowf);
+                vy4n = ld4(a.vy, py + rowf + Pb);
+                wc = ld4(a.vz, pz + rowf);
+                wn = ld4(a.vz, pz + plane_b + rowf);
+            }
+            if constexpr (COEF) {
+                r2c = ld4_nt(a.r2v, pz + rowf);
+                if constexpr (NC == 3) {
+                    if (a.r1v) r1c = ld4_nt(a.r1v, pz + rowf);
+                }
+            }
+            if constexpr (ST == GEN_CORRECT) {
+#pragma unroll
+                for (int q = 0; q < NC; q++) mav[q] = ld4_nt(a.mavg[q], pz + rowf);
+            }
+        }
+    };
+
+    // prologue: masks of planes k0-1, k0, k0+1; values of k0-1, k0
+    kp = load_mask(k0 - 1);
+    kc = load_mask(k0);
+    kn = load_mask(k0 + 1);
+    hkc = load_hmask(k0);
+    hkn = load_hmask(k0 + 1);
+    load_m(k0 - 1, kp, mp);
+    load_m(k0, kc, mc);
+    load_hm(k0, hkc, hc);
+    if constexpr (COEF) {
+        dp = load_d(k0 - 1);
+        dcv = load_d(k0);
+        hdc = load_hd(k0);
+    }
+    load_xhalo(k0, xc, kxc, xdc, gc);
+    fix_centre(mc, kc, dcv, gc);
+    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): retire the prologue's loads before the loop
+
+    const Coef cf = a.c;
+    bool trim_x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) trim_x[e] = (ix + e >= a.bx0) && (ix + e < a.bx1);
+
+    for (int k = k0; k < k1; k++) {
+        const int buf = (k - k0) & 1;
+        // prefetch: masks of plane k+2, values (and diffusivities) of plane k+1
+        kq = load_mask(k + 2);
+        hkq = load_hmask(k + 2);
+        load_m(k + 1, kn, mn);
+        load_hm(k + 1, hkn, hn);
+        if constexpr (COEF) {
+            dnv = load_d(k + 1);
+            hdn = load_hd(k + 1);
+        }
+        load_xhalo(k + 1, xn, kxn, xdn, gn);
+        load_own(k, kc);
+
+        // stage plane k with its halo and ghosts
+        if (rin) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][lr][c]) = mc[q];
+            *reinterpret_cast<uint32_t*>(&sk[buf][lr][c]) = kc;
+            if constexpr (COEF) *reinterpret_cast<f4*>(&sd[buf][lr][c]) = dcv;
+            if (xend) { // ghost column nx (or padding, when nx falls inside the chunk)
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[buf][q][lr][c + 4] = right_ghost(mc, gc, q);
+                const int xl = xlast < 3 ? xlast : 3;
+                sk[buf][lr][c + 4] = (uint8_t)((kc >> (8 * xl)) & 0xffu);
+                if constexpr (COEF) sd[buf][lr][c + 4] = dcv[xl];
+            }
+            if (xgl) { // ghost column -1
+#pragma unroll
+                for (int q = 0; q < NC; q++)
+                    sm[buf][q][lr][c - 1] = bcx == BC_PERIODIC ? xc[q] : (bcx == BC_DIRICHLET ? 0.f : mc[q].x);
+                sk[buf][lr][c - 1] = (uint8_t)(kc & 0xffu);
+                if constexpr (COEF) sd[buf][lr][c - 1] = dcv.x;
+            } else if (hcl || hcr) {
+                const int col = hcl ? c - 1 : c + 4;
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[buf][q][lr][col] = xc[q];
+                sk[buf][lr][col] = (uint8_t)kxc;
+                if constexpr (COEF) sd[buf][lr][col] = xdc;
+            }
+        }
+        if (hrow) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[buf][q][hlr][c]) = hc[q];
+            *reinterpret_cast<uint32_t*>(&sk[buf][hlr][c]) = hkc;
+            if constexpr (COEF) *reinterpret_cast<f4*>(&sd[buf][hlr][c]) = hdc;
+        }
         __syncthreads();
+
+        if (rin && kc != 0) {
+            const bool trim = k >= a.bz0 && k < a.bz1 && iy >= a.by0 && iy < a.by1;
+            const float thy = cf.bgy * ((float)iy + 0.5f), thz = cf.bgz * ((float)k + 0.5f);
+            f4 ym[NC], yp[NC];
+            float xl[NC], xr[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                ym[q] = *reinterpret_cast<const f4*>(&sm[buf][q][lr - 1][c]);
+                yp[q] = *reinterpret_cast<const f4*>(&sm[buf][q][lr + 1][c]);
+                xl[q] = sm[buf][q][lr][c - 1];
+                xr[q] = sm[buf][q][lr][c + 4];
+            }
+            const uint32_t kym = *reinterpret_cast<const uint32_t*>(&sk[buf][lr - 1][c]);
+            const uint32_t kyp = *reinterpret_cast<const uint32_t*>(&sk[buf][lr + 1][c]);
+            const uint32_t kxm = (kc << 8) | (uint32_t)sk[buf][lr][c - 1];
+            const uint32_t kxp = (kc >> 8) | ((uint32_t)sk[buf][lr][c + 4] << 24);
+            f4 dym = f4{0.f, 0.f, 0.f, 0.f}, dyp = dym;
+            float dxl = 0.f, dxr = 0.f;
+            if constexpr (COEF) {
+                dym = *reinterpret_cast<const f4*>(&sd[buf][lr - 1][c]);
+                dyp = *reinterpret_cast<const f4*>(&sd[buf][lr + 1][c]);
+                dxl = sd[buf][lr][c - 1];
+                dxr = sd[buf][lr][c + 4];
+            }
+            f4 out[NC];
+            gupdate_chunk<NC, VEL, ST, COEF>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, vx4,
+                                             vxe, vy4, vy4n, wc, wn, dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
+                                             r2c, mav,
+                                             ix, thy, thz, out);
+            if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
+            const uint32_t of = (uint32_t)k * plane_b + rowf;
+#pragma unroll
+            for (int q = 0; q < NC; q++) st4_nt(a.out[q], of, out[q]);
+        }
+
+        // rotate the register planes; ghost fixes of the planes loaded above
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            mp[q] = mc[q];
+            mc[q] = mn[q];
+            xc[q] = xn[q];
+            gc[q] = gn[q];
+            hc[q] = hn[q];
+        }
+        kp = kc;
+        kc = kn;
+        kn = kq;
+        kxc = kxn;
+        hkc = hkn;
+        hkn = hkq;
+        if constexpr (COEF) {
+            dp = dcv;
+            dcv = dnv;
+            hdc = hdn;
+            xdc = xdn;
+        }
+        fix_centre(mc, kc, dcv, gc);
+    }
+}
+
+template <int NC, bool VEL, int ST, bool COEF, int LX>
+__global__ void __launch_bounds__(256) gsweep_kernel(GSweepArgs a)
+{
+    __shared__ double sred[4][NC];
+    double acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) acc[q] = 0.0;
+    gsweep_tile<NC, VEL, ST, COEF, LX>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    if (ST != GEN_PREDICT && a.partials) block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
+}
+
+__global__ void __launch_bounds__(256) zero_solids_kernel(float* m0, float* m1, float* m2, int nc,
+                                                          const uint8_t* __restrict__ mask, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        if (mask[i] != 0) continue;
+        m0[i] = 0.f;
+        if (nc > 1) m1[i] = 0.f;
+        if (nc > 2) m2[i] = 0.f;
     }
 }
 
@@ -1424,12 +1933,20 @@ hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s)
+int field_sum_blocks(const FieldSumArgs& a)
+{
+    const long wy = a.by1 - a.by0, wz = a.bz1 - a.bz0;
+    const long nrows = (wy > 0 && wz > 0) ? wy * wz : 0;
+    return (int)std::max<long>(1, (nrows + kFieldSumRows - 1) / kFieldSumRows);
+}
+
+hipError_t launch_field_sum(int nc, const FieldSumArgs& a, hipStream_t s)
 {
     const bool w = a.w[0] != nullptr;
+    const dim3 grid(field_sum_blocks(a)), block(256);
 #define NMR_FS(NC_)                                                                              \
-    if (w) hipLaunchKernelGGL((field_sum_kernel<NC_, true>), dim3(nz), dim3(256), 0, s, a);       \
-    else hipLaunchKernelGGL((field_sum_kernel<NC_, false>), dim3(nz), dim3(256), 0, s, a);
+    if (w) hipLaunchKernelGGL((field_sum_kernel<NC_, true>), grid, block, 0, s, a);              \
+    else hipLaunchKernelGGL((field_sum_kernel<NC_, false>), grid, block, 0, s, a);
     switch (nc) {
     case 1: NMR_FS(1) break;
     case 2: NMR_FS(2) break;
@@ -1437,6 +1954,49 @@ hipError_t launch_field_sum(int nc, const FieldSumArgs& a, int nz, hipStream_t s
     default: return hipErrorInvalidValue;
     }
 #undef NMR_FS
+    return hipGetLastError();
+}
+
+int gsweep_lanes(int nx) { return nx > 128 ? 64 : 32; }
+
+template <int NC, bool VEL, bool COEF>
+static hipError_t launch_gs(int stage, int lanes, const GSweepArgs& a, hipStream_t s)
+{
+    const dim3 grid(a.nblocks), block(256);
+#define NMR_GS(ST_)                                                                                     \
+    if (lanes == 64) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, ST_, COEF, 64>), grid, block, 0, s, a); \
+    else if (lanes == 32) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, ST_, COEF, 32>), grid, block, 0, s, a); \
+    else return hipErrorInvalidValue;
+    switch (stage) {
+    case GEN_UPWIND: NMR_GS(GEN_UPWIND) break;
+    case GEN_PREDICT: NMR_GS(GEN_PREDICT) break;
+    case GEN_CORRECT: NMR_GS(GEN_CORRECT) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef NMR_GS
+    return hipGetLastError();
+}
+
+hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s)
+{
+#define NMR_GSC(NC_)                                                                \
+    if (coef) return vel ? launch_gs<NC_, true, true>(stage, lanes, a, s)           \
+                         : launch_gs<NC_, false, true>(stage, lanes, a, s);         \
+    return vel ? launch_gs<NC_, true, false>(stage, lanes, a, s) : launch_gs<NC_, false, false>(stage, lanes, a, s);
+    switch (nc) {
+    case 1: NMR_GSC(1)
+    case 2: NMR_GSC(2)
+    case 3: NMR_GSC(3)
+    default: return hipErrorInvalidValue;
+    }
+#undef NMR_GSC
+}
+
+hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s)
+{
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(zero_solids_kernel, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s, m[0],
+                       m[1], m[2], nc, mask, n);
     return hipGetLastError();
 }
 
