@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 METRIC = "voxel-steps/s & %HBM roofline @256³; learned vs plain time-to-signal @1e-3"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + Vx,Vy,Vz (12)
+# MacCormack (--scheme maccormack): predictor pass 29 B (as above, writing the
+# predictor) + corrector pass 37 B (reads the predictor and the state, 8 each)
+BYTES_PER_VOXEL_STEP_MAK = 66.0
 # DRAM bytes per sweep launch of the default workload (256^3, phi 0.20,
 # automatic tile): rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per
 # dispatch, scaled by what the same counters read for a plain 1 GiB copy
@@ -35,14 +38,15 @@ BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + Vx,
 DEFAULT_TRAFFIC = 320.6e6 + 50.4e6
 
 
-def cpu_baseline(case, target_s: float):
+def cpu_baseline(case, target_s: float, scheme: int = 0):
     """FP64 oracle steps of the same grid on the host cores (bounded sample)."""
     import numpy as np
 
     import oracle
 
     p, act, D, t1, t2, vel, m = oracle.setup_case(case, m0=[case.mask.astype(np.float64),
-                                                           np.zeros(case.shape), np.zeros(case.shape)])
+                                                           np.zeros(case.shape), np.zeros(case.shape)],
+                                                  scheme=scheme)
     oracle.run(p, act, D, t1, t2, vel, 1, m, grad_sched=case.grad_sched[:1])  # warm (threads, pages)
     t0 = time.perf_counter()
     oracle.run(p, act, D, t1, t2, vel, 2, m, grad_sched=case.grad_sched[1:3])
@@ -57,7 +61,8 @@ def cpu_baseline(case, target_s: float):
         "unit": "voxel-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 oracle (OpenMP restatement of ExplicitSimple), {case.shape[2]}^3, {n} steps, "
+        "sample": f"FP64 oracle (OpenMP restatement of {'ExplicitMakUpgrade' if scheme else 'ExplicitSimple'}), "
+                  f"{case.shape[2]}^3, {n} steps, "
                   f"{dt:.1f} s, {threads} threads",
     }
 
@@ -300,7 +305,10 @@ def main() -> int:
     ap.add_argument("--phi", type=float, default=0.20)
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--lz", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--scheme", choices=("upwind", "maccormack"), default="upwind",
+                    help="plain workload: ExplicitSimple's upwind step (default) or the reference GPU loop's "
+                         "MacCormack step")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("plain", "batch", "learned"), default="plain",
                     help="plain: one 256^3 grid (default); batch: many uneven instances per launch; "
@@ -333,9 +341,13 @@ def main() -> int:
     from initializator_amd import generators
     from initializator_amd.solver import HipSolver
 
+    from initializator_amd import _lib
+
+    mak = args.scheme == "maccormack"
+    bpv = BYTES_PER_VOXEL_STEP_MAK if mak else BYTES_PER_VOXEL_STEP
     total = args.warmup + args.steps
     case = generators.case_flow(n=args.n, phi=args.phi, nsteps=total)
-    s = HipSolver.from_case(case)
+    s = HipSolver.from_case(case, scheme=_lib.NMR_SCHEME_MACCORMACK if mak else _lib.NMR_SCHEME_UPWIND)
     s.set_tile(args.variant, args.lz)
     s.upload_magnetization(case.mask.astype("float64"), np.zeros(case.shape))
     s.run(args.warmup, case.grad_sched[: args.warmup], want_signal=False)
@@ -348,7 +360,7 @@ def main() -> int:
     nvox = case.nvox
     value = nvox * args.steps / wall
     kernel_s = gpu_ms * 1e-3 / args.steps
-    achieved = BYTES_PER_VOXEL_STEP * nvox / kernel_s / 1e9
+    achieved = bpv * nvox / kernel_s / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -363,10 +375,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded porosity, divergence-free velocity, PGSE schedule)",
         "config": {
-            "workload": f"plain FP32 upwind Bloch-Torrey step, {args.n}^3, phi={args.phi}, flow + PGSE "
-                        "G=0.3 T/m, complex (Mx,My)",
+            "workload": f"plain FP32 {'MacCormack' if mak else 'upwind'} Bloch-Torrey step, {args.n}^3, "
+                        f"phi={args.phi}, flow + PGSE G=0.3 T/m, complex (Mx,My)",
             "grid": [args.n] * 3,
-            "bytes_per_voxel_step": BYTES_PER_VOXEL_STEP,
+            "bytes_per_voxel_step": bpv,
         },
         "roofline": {
             "bound": "hbm",
@@ -374,7 +386,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (DEFAULT_TRAFFIC if (args.n, args.phi, args.variant, args.lz) == (256, 0.20, -1, 0)
+            "traffic": (DEFAULT_TRAFFIC if (args.n, args.phi, args.variant, args.lz, mak) == (256, 0.20, -1, 0, False)
                         else None),
             "kernel_us": kernel_s * 1e6,
         },
@@ -382,7 +394,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         cb = generators.case_flow(n=args.n, phi=args.phi, nsteps=404)
-        out["cpu_baseline"] = cpu_baseline(cb, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cb, args.cpu_seconds, scheme=1 if mak else 0)
     print(json.dumps(out), flush=True)
     return 0
 
