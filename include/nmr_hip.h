@@ -49,6 +49,12 @@ extern "C" {
 #define NMR_F_DIFFUSION 1
 #define NMR_F_VELOCITY 2
 #define NMR_F_PRECESSION 4
+/* Upwind scheme only: advection across a face only when the neighbour is a
+ * pore -- the face masks of NMR_2d (NmrRelaxationSolveKernels.cu:190-210), a
+ * kernel the reference defines but never launches.  Without it a pore next
+ * to a solid face that carries velocity takes the solid's value into its
+ * upwind flux, as ExplicitSimple does. */
+#define NMR_F_ADVECT_MASK 8
 
 /* ---- host array layouts accepted by upload/download ---- */
 /* COMPACT: interior only, x fastest.  mask uint8 0/1 [nz][ny][nx];

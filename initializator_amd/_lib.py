@@ -24,6 +24,7 @@ NMR_ERR_UNSUPPORTED = -6
 NMR_F_DIFFUSION = 1
 NMR_F_VELOCITY = 2
 NMR_F_PRECESSION = 4
+NMR_F_ADVECT_MASK = 8
 
 NMR_LAYOUT_COMPACT = 0
 NMR_LAYOUT_PADDED = 1

@@ -275,6 +275,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.pitch1 = ctx->pitch1;
         for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
         a.vel = ctx->vel ? 1 : 0;
+        a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         a.dn = ctx->dn;
         a.r1v = ctx->r1v;
         a.r2v = ctx->r2v;
@@ -340,6 +341,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
         for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
         a.skip_m = ctx->inert;
+        a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         const bool coef = ctx->hetero;
         int64_t done = 0;
         while (done < nsteps) {
@@ -471,11 +473,14 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
         if (p.bc[a] < NMR_BC_NEUMANN || p.bc[a] > NMR_BC_DIRICHLET) return fail(NMR_ERR_INVALID, "bc[%d] = %d", a, p.bc[a]);
         general = general || p.bc[a] != NMR_BC_NEUMANN;
     }
+    // masked-face advection runs in the general sweep (MacCormack masks its
+    // advection by construction; the flag changes nothing there)
+    general = general || ((p.flags & NMR_F_ADVECT_MASK) && (p.flags & NMR_F_VELOCITY));
     if (general && p.phase_mode == NMR_PHASE_EXACT)
         return fail(NMR_ERR_UNSUPPORTED, "exact phase needs the upwind scheme with zero-derivative boundaries");
     if (p.phase_mode != NMR_PHASE_LINEAR && p.phase_mode != NMR_PHASE_EXACT)
         return fail(NMR_ERR_INVALID, "phase_mode %d", p.phase_mode);
-    if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION))
+    if (p.flags & ~(NMR_F_DIFFUSION | NMR_F_VELOCITY | NMR_F_PRECESSION | NMR_F_ADVECT_MASK))
         return fail(NMR_ERR_INVALID, "flags 0x%x", p.flags);
     if (!(p.dx > 0) || !(p.dy > 0) || !(p.dz > 0) || !(p.dt > 0))
         return fail(NMR_ERR_INVALID, "dx, dy, dz, dt must be > 0");

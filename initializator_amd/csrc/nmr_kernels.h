@@ -104,6 +104,7 @@ struct GenArgs {
     int nx, ny, nz, pitch, pitch1;
     int bc[3];              // BC_* per axis
     int vel;                // face velocities present
+    int amask;              // upwind advection only across faces to pores (NMR_F_ADVECT_MASK)
     const float* dn;        // per-voxel diffusivity / reference (null: uniform, from the mask)
     const float* r1v;       // per-voxel dt/T1 with surface terms (null: from the mask)
     const float* r2v;       // per-voxel dt/T2 with surface terms
@@ -132,6 +133,7 @@ struct GSweepArgs {
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
     int bc[3];              // BC_* per axis
     int skip_m;             // all-solid chunks are never read by a pore: not loaded
+    int amask;              // upwind advection only across faces to pores (NMR_F_ADVECT_MASK)
     Coef c;
 };
 

@@ -272,12 +272,16 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
         const float wm = a.vz[c], wp = a.vz[c + plane];
         const float su = um + up, sv = vm + vp, sw = wm + wp;
         if constexpr (ST == GEN_UPWIND) {
-            w[0] = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
-            w[1] = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
-            w[2] = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
-            w[3] = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
-            w[4] = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
-            w[5] = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
+            // with amask a face to a solid carries no advection (NMR_2d, cu:190-210)
+            bool open[6];
+#pragma unroll
+            for (int s = 0; s < 6; s++) open[s] = !a.amask || kn[s] != 0.f;
+            w[0] = (um != 0.f && open[0]) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
+            w[1] = (up != 0.f && open[1]) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
+            w[2] = (vm != 0.f && open[2]) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
+            w[3] = (vp != 0.f && open[3]) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
+            w[4] = (wm != 0.f && open[4]) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
+            w[5] = (wp != 0.f && open[5]) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
         } else if constexpr (ST == GEN_PREDICT) { // + u hx (f_{i-1} - f_i); kn is 0 or 1
             w[0] = fmaf(kn[0] * hxh, su, w[0]);
             w[2] = fmaf(kn[2] * hyh, sv, w[2]);
@@ -1250,7 +1254,8 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
                                               const f4& vy4n, const f4& vz0, const f4& vz1, const f4& dc,
                                               float dxl, float dxr, const f4& dym, const f4& dyp, const f4& dzm,
                                               const f4& dzp, bool r1p, const f4& r1c, const f4& r2c,
-                                              const f4 (&mav)[NC], int ix, float thy, float thz, f4 (&out)[NC])
+                                              const f4 (&mav)[NC], bool amask, int ix, float thy, float thz,
+                                              f4 (&out)[NC])
 {
     const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
     const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
@@ -1276,12 +1281,15 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
             const float wm = vz0[e], wp = vz1[e];
             const float su = um + up, sv = vm + vp, sw = wm + wp;
             if constexpr (ST == GEN_UPWIND) {
-                w[0] = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
-                w[1] = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
-                w[2] = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
-                w[3] = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
-                w[4] = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
-                w[5] = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
+                bool open[6];
+#pragma unroll
+                for (int s = 0; s < 6; s++) open[s] = !amask || kn[s] != 0.f;
+                w[0] = (um != 0.f && open[0]) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
+                w[1] = (up != 0.f && open[1]) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
+                w[2] = (vm != 0.f && open[2]) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
+                w[3] = (vp != 0.f && open[3]) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
+                w[4] = (wm != 0.f && open[4]) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
+                w[5] = (wp != 0.f && open[5]) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
             } else if constexpr (ST == GEN_PREDICT) {
                 w[0] = fmaf(kn[0] * hxh, su, w[0]);
                 w[2] = fmaf(kn[2] * hyh, sv, w[2]);
@@ -1448,7 +1456,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     const int hcx = lx == 0 ? (x0 > 0 ? x0 - 1 : nx - 1) : x0 + TX;
     const bool gxr = xend && bcx == BC_PERIODIC;         // right ghost value from column 0
 
-    const bool skip_m = (ST != GEN_UPWIND) || a.skip_m != 0;
+    const bool skip_m = (ST != GEN_UPWIND) || a.skip_m != 0 || a.amask != 0;
     auto clampz = [&](int kk) { return kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk); };
     auto zsrc = [&](int kk) -> int {
         if (kk >= 0 && kk < nz) return kk;
@@ -1689,7 +1697,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             f4 out[NC];
             gupdate_chunk<NC, VEL, ST, COEF>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, vx4,
                                              vxe, vy4, vy4n, wc, wn, dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
-                                             r2c, mav,
+                                             r2c, mav, a.amask != 0,
                                              ix, thy, thz, out);
             if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
             const uint32_t of = (uint32_t)k * plane_b + rowf;

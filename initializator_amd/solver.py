@@ -29,7 +29,7 @@ class HipSolver:
                  p_eq: float = 0.0, b0_dir=(0.0, 0.0, 1.0), grad=(0.0, 0.0, 0.0), diffusion: bool = True,
                  velocity: bool = False, precession: bool = False, phase_exact: bool = False,
                  buffer=(0, 0, 0), dy: float | None = None, dz: float | None = None,
-                 scheme: int = _lib.NMR_SCHEME_UPWIND, bc=(0, 0, 0)):
+                 scheme: int = _lib.NMR_SCHEME_UPWIND, bc=(0, 0, 0), advect_mask: bool = False):
         self.lib = _lib.load()
         nz, ny, nx = (int(s) for s in shape)
         self.shape = (nz, ny, nx)
@@ -38,7 +38,7 @@ class HipSolver:
         p.nx, p.ny, p.nz = nx, ny, nz
         p.ncomp = self.ncomp
         p.flags = ((_lib.NMR_F_DIFFUSION if diffusion else 0) | (_lib.NMR_F_VELOCITY if velocity else 0)
-                   | (_lib.NMR_F_PRECESSION if precession else 0))
+                   | (_lib.NMR_F_PRECESSION if precession else 0) | (_lib.NMR_F_ADVECT_MASK if advect_mask else 0))
         p.scheme = int(scheme)
         p.phase_mode = _lib.NMR_PHASE_EXACT if phase_exact else _lib.NMR_PHASE_LINEAR
         for a in range(3):

@@ -41,6 +41,7 @@ class OrParams(ctypes.Structure):
         ("b0", c_double * 3),
         ("scheme", c_int32),
         ("bc", c_int32 * 3),
+        ("advect_mask", c_int32),
     ]
 
 
@@ -126,10 +127,12 @@ def _c(a):
 
 
 def params(shape, dx, dt, *, dy=None, dz=None, precession=False, velocity=False, diffusion=True,
-           gamma=G_REF, p_eq=0.0, b0=(0.0, 0.0, 1.0), scheme=UPWIND, bc=(0, 0, 0)) -> OrParams:
+           gamma=G_REF, p_eq=0.0, b0=(0.0, 0.0, 1.0), scheme=UPWIND, bc=(0, 0, 0),
+           advect_mask=False) -> OrParams:
     nz, ny, nx = shape
     p = OrParams()
     p.scheme = int(scheme)
+    p.advect_mask = int(bool(advect_mask))
     for i in range(3):
         p.bc[i] = int(bc[i])
     p.ik, p.jk, p.kk = nx, ny, nz
@@ -243,14 +246,15 @@ def run(p, act, D, t1, t2, vel, nsteps, m, grad_sched=None, grad_const=None, buf
     return sig
 
 
-def setup_case(case, p_eq=0.0, m0=None, scheme=UPWIND, bc=(0, 0, 0)):
+def setup_case(case, p_eq=0.0, m0=None, scheme=UPWIND, bc=(0, 0, 0), advect_mask=False):
     """Padded oracle inputs for a generators.Case (one fluid, uniform
     relaxivity): params, mask, D, T1, T2, velocity, initial M."""
     from initializator_amd import layout  # conversions only
 
     has_v = case.vx is not None
     prec = case.grad_sched is not None
-    p = params(case.shape, case.dx, case.dt, precession=prec, velocity=has_v, p_eq=p_eq, scheme=scheme, bc=bc)
+    p = params(case.shape, case.dx, case.dt, precession=prec, velocity=has_v, p_eq=p_eq, scheme=scheme, bc=bc,
+               advect_mask=advect_mask)
     act = layout.pad_mask(case.mask)
     D, t1, t2, _ = fill_coefficients(p, act, case.diffusivity, case.t1_bulk, case.t2_bulk, case.rho, case.rho)
     if has_v:
@@ -264,12 +268,12 @@ def setup_case(case, p_eq=0.0, m0=None, scheme=UPWIND, bc=(0, 0, 0)):
     return p, act, D, t1, t2, vel, m
 
 
-def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0), scheme=UPWIND, bc=(0, 0, 0)):
+def run_case(case, nsteps=None, p_eq=0.0, m0=None, buf=(0, 0, 0), scheme=UPWIND, bc=(0, 0, 0), advect_mask=False):
     """Oracle run of a Case: (signal (nsteps+1, 3), compact final fields)."""
     from initializator_amd import layout
 
     n = case.nsteps if nsteps is None else nsteps
-    p, act, D, t1, t2, vel, m = setup_case(case, p_eq=p_eq, m0=m0, scheme=scheme, bc=bc)
+    p, act, D, t1, t2, vel, m = setup_case(case, p_eq=p_eq, m0=m0, scheme=scheme, bc=bc, advect_mask=advect_mask)
     gs = None if case.grad_sched is None else case.grad_sched[:n]
     sig = run(p, act, D, t1, t2, vel, n, m, grad_sched=gs, buf=buf)
     return sig, [layout.unpad(x) for x in m]

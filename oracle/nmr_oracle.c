@@ -18,7 +18,10 @@
  *   or_explicit_simple   pinned: bulk T1/T2 closed forms, enclosed-voxel
  *                        surface rate, Neumann cosine eigenmode, 1-D upwind
  *                        brute force (both signs), precession (1-i*theta)^n,
- *                        conservation / fixed-point invariants
+ *                        conservation / fixed-point invariants; the
+ *                        advect_mask variant: hand values of a pore next to
+ *                        a flowing solid face, and agreement with the plain
+ *                        step where no pore-solid face carries velocity
  *   or_maccormack        pinned: Heun factors of the relaxation-only,
  *                        precession-only and diffusion-only (Neumann cosine
  *                        and zero-value sine eigenmodes) problems; the
@@ -61,6 +64,10 @@ typedef struct or_params {
     double b0[3];            /* uniform mainField */
     int32_t scheme;          /* 0: ExplicitSimple (upwind), 1: MacCormack (ExplicitMakUpgrade) */
     int32_t bc[3];           /* per axis: 0 zero derivative, 1 periodic, 2 zero value */
+    int32_t advect_mask;     /* upwind only: the NMR_2d variant (cu:90-232) -- advection
+                                across a face only when the neighbour is active
+                                (cu:190-210), indicators v/(v + 1e-20), harmonic
+                                means without the 1e-50 guard */
 } or_params;
 
 #define OR_BC_NEUMANN 0
@@ -327,21 +334,33 @@ void or_explicit_simple(const or_params* p, const char* act, const double* D,
                 const long ym = IX(p, i, j - 1, k), yp = IX(p, i, j + 1, k);
                 const long zm = IX(p, i, j, k - 1), zp = IX(p, i, j, k + 1);
                 const double tau1 = 1.0 / t1[c], tau2 = 1.0 / t2[c];
-                /* harmonic-mean face diffusivities */
+                /* harmonic-mean face diffusivities (NMR_2d, cu:214-219: no guard) */
                 const double dc = D[c];
-                const double gxp = 2.0 * (D[xp] * dc) / (D[xp] + dc + OR_SMALL);
-                const double gxm = 2.0 * (D[xm] * dc) / (D[xm] + dc + OR_SMALL);
-                const double gyp = 2.0 * (D[yp] * dc) / (D[yp] + dc + OR_SMALL);
-                const double gym = 2.0 * (D[ym] * dc) / (D[ym] + dc + OR_SMALL);
-                const double gzp = 2.0 * (D[zp] * dc) / (D[zp] + dc + OR_SMALL);
-                const double gzm = 2.0 * (D[zm] * dc) / (D[zm] + dc + OR_SMALL);
-                /* staggered face velocities and wall indicators */
+                const double gs = p->advect_mask ? 0.0 : OR_SMALL;
+                const double gxp = 2.0 * (D[xp] * dc) / (D[xp] + dc + gs);
+                const double gxm = 2.0 * (D[xm] * dc) / (D[xm] + dc + gs);
+                const double gyp = 2.0 * (D[yp] * dc) / (D[yp] + dc + gs);
+                const double gym = 2.0 * (D[ym] * dc) / (D[ym] + dc + gs);
+                const double gzp = 2.0 * (D[zp] * dc) / (D[zp] + dc + gs);
+                const double gzm = 2.0 * (D[zm] * dc) / (D[zm] + dc + gs);
+                /* staggered face velocities and wall indicators (NMR_2d: 1e-20,
+                 * cu:143-154); with advect_mask, advection only across faces
+                 * whose neighbour is active (cu:190-210) */
+                const double ie = p->advect_mask ? 1e-20 : OR_SMALL;
                 const double um = vx[c], up = vx[xp];
                 const double vm = vy[c], vp = vy[yp];
                 const double wm = vz[c], wp = vz[zp];
-                const double umm = um / (um + OR_SMALL), upp = up / (up + OR_SMALL);
-                const double vmm = vm / (vm + OR_SMALL), vpp = vp / (vp + OR_SMALL);
-                const double wmm = wm / (wm + OR_SMALL), wpp = wp / (wp + OR_SMALL);
+                double umm = um / (um + ie), upp = up / (up + ie);
+                double vmm = vm / (vm + ie), vpp = vp / (vp + ie);
+                double wmm = wm / (wm + ie), wpp = wp / (wp + ie);
+                if (p->advect_mask) {
+                    umm *= act[xm] == '0' ? 0.0 : 1.0;
+                    upp *= act[xp] == '0' ? 0.0 : 1.0;
+                    vmm *= act[ym] == '0' ? 0.0 : 1.0;
+                    vpp *= act[yp] == '0' ? 0.0 : 1.0;
+                    wmm *= act[zm] == '0' ? 0.0 : 1.0;
+                    wpp *= act[zp] == '0' ? 0.0 : 1.0;
+                }
                 const double u = 0.5 * (um + up), v = 0.5 * (vm + vp), w = 0.5 * (wm + wp);
                 const double ud = fabs(u), vd = fabs(v), wd = fabs(w);
                 /* unit main-field direction */
