@@ -62,7 +62,7 @@ def cpu_baseline(case, target_s: float, scheme: int = 0):
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle (OpenMP restatement of {'ExplicitMakUpgrade' if scheme else 'ExplicitSimple'}), "
-                  f"{case.shape[2]}^3, {n} steps, "
+                  f"{'x'.join(str(v) for v in case.shape[::-1])}, {n} steps, "
                   f"{dt:.1f} s, {threads} threads",
     }
 
@@ -108,6 +108,42 @@ def _batch_instance(i: int):
     g = generators.gradient_for_b(b, 2e-3, 20e-3)
     return dict(shape=shape, phi=phi, rho1=rho, rho2=rho, grad_amp=g, delta=2e-3, big_delta=20e-3,
                 mask=mask, vel=vel)
+
+
+def instance_errors(specs, k, nsteps, dx, dt, D):
+    """Signal and field error against the FP64 oracle of the k smallest
+    instances, advanced nsteps from their initial state in a batch of their
+    own (a batched instance's result does not depend on its batch mates)."""
+    import oracle
+    from initializator_amd import generators
+    from initializator_amd.solver import BatchSolver
+
+    order = sorted(range(len(specs)), key=lambda i: int(np.prod(specs[i]["shape"])))[:k]
+    sub = [specs[i] for i in order]
+    b = BatchSolver(sub, 2, dx, dt, D, 1.0 / 2.0, r1_bulk=1.0 / 2.5e10, velocity=True, precession=True)
+    for j, sp in enumerate(sub):
+        b.upload(j, sp["mask"], [sp["mask"].astype(np.float64), np.zeros(sp["shape"])], sp["vel"])
+    sig = b.run(nsteps)
+    rows = []
+    for j, sp in enumerate(sub):
+        sched = generators.pgse_schedule(nsteps, dt, sp["grad_amp"], sp["delta"], sp["big_delta"])
+        case = generators.Case(sp["shape"], sp["mask"], *sp["vel"], dx, dt, D, 2.0, 2.5e10, sp["rho1"], nsteps,
+                               sched, 2)
+        m0 = [sp["mask"].astype(np.float64), np.zeros(sp["shape"]), np.zeros(sp["shape"])]
+        sig_ref, f_ref = oracle.run_case(case, m0=m0)
+        f = b.download(j)
+        vox = max(float(np.abs(f[q] - f_ref[q]).max()) for q in range(2))
+        scale = max(float(np.abs(f_ref[q]).max()) for q in range(2))
+        rows.append((float(np.linalg.norm(sig[j] - sig_ref[:, :2]) / np.linalg.norm(sig_ref[:, :2])), vox,
+                     vox / max(scale, 1e-300)))
+    b.close()
+    e = np.array(rows)
+    return {"instances": [list(specs[i]["shape"]) for i in order], "steps": nsteps,
+            "signal_rel_l2_max": float(e[:, 0].max()), "max_voxel_max": float(e[:, 1].max()),
+            # the linearised precession grows |M| under the strongest gradients
+            # (b = 1e10 s/m^2 is 7.5 T/m): the field error relative to max |M|
+            "max_voxel_rel_max": float(e[:, 2].max()),
+            "signal_rel_l2": [float(x) for x in e[:, 0]]}
 
 
 def batch_main(args) -> int:
@@ -161,6 +197,7 @@ def batch_main(args) -> int:
                 "steps": args.single_steps, "gpu_ms": loop_ms}
     kernel_s = gpu_ms * 1e-3 / args.steps
     achieved = BYTES_PER_VOXEL_STEP * nvox / kernel_s / 1e9
+    err = instance_errors(specs, args.error_instances, args.error_steps, dx, dt, D) if args.error_instances else None
     out = {
         "metric": "voxel-steps/s (batched uneven instances)",
         "value": nvox * args.steps / wall,
@@ -195,6 +232,7 @@ def batch_main(args) -> int:
         },
         "cpu_baseline": None,
         "single_instance_loop": loop,
+        "per_instance_fp64_error": err,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -303,6 +341,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--phi", type=float, default=0.20)
+    ap.add_argument("--rho", type=float, default=10e-6, help="plain workload: surface relaxivity (m/s)")
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--lz", type=int, default=0)
     ap.add_argument("--scheme", choices=("upwind", "maccormack"), default="upwind",
@@ -322,6 +361,11 @@ def main() -> int:
     ap.add_argument("--graph-steps", type=int, default=64, help="batch workload: steps per captured graph")
     ap.add_argument("--full-grid", action="store_true",
                     help="batch workload with --variant: one workgroup per tile instead of a resident grid")
+    ap.add_argument("--error-instances", type=int, default=8,
+                    help="batch workload: smallest instances checked against the FP64 oracle (0: skip)")
+    ap.add_argument("--error-steps", type=int, default=1000, help="batch workload: steps of that check")
+    ap.add_argument("--shape", type=int, nargs=3, metavar=("NZ", "NY", "NX"), default=None,
+                    help="plain workload: grid (default n^3)")
     ap.add_argument("--single-steps", type=int, default=200,
                     help="batch workload: steps of the one-instance-at-a-time comparison (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -346,7 +390,9 @@ def main() -> int:
     mak = args.scheme == "maccormack"
     bpv = BYTES_PER_VOXEL_STEP_MAK if mak else BYTES_PER_VOXEL_STEP
     total = args.warmup + args.steps
-    case = generators.case_flow(n=args.n, phi=args.phi, nsteps=total)
+    shape = tuple(args.shape) if args.shape else (args.n,) * 3
+    case = generators.case_flow(shape=shape, phi=args.phi, nsteps=total, rho=args.rho,
+                                progress=lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
     s = HipSolver.from_case(case, scheme=_lib.NMR_SCHEME_MACCORMACK if mak else _lib.NMR_SCHEME_UPWIND)
     s.set_tile(args.variant, args.lz)
     s.upload_magnetization(case.mask.astype("float64"), np.zeros(case.shape))
@@ -375,9 +421,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded porosity, divergence-free velocity, PGSE schedule)",
         "config": {
-            "workload": f"plain FP32 {'MacCormack' if mak else 'upwind'} Bloch-Torrey step, {args.n}^3, "
-                        f"phi={args.phi}, flow + PGSE G=0.3 T/m, complex (Mx,My)",
-            "grid": [args.n] * 3,
+            "workload": f"plain FP32 {'MacCormack' if mak else 'upwind'} Bloch-Torrey step, "
+                        f"{'x'.join(str(v) for v in shape[::-1])}, phi={args.phi}, rho={args.rho:g} m/s, "
+                        "flow + PGSE G=0.3 T/m, complex (Mx,My)",
+            "grid": list(shape[::-1]),
             "bytes_per_voxel_step": bpv,
         },
         "roofline": {
@@ -386,14 +433,14 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (DEFAULT_TRAFFIC if (args.n, args.phi, args.variant, args.lz, mak) == (256, 0.20, -1, 0, False)
-                        else None),
+            "traffic": (DEFAULT_TRAFFIC if (shape, args.phi, args.variant, args.lz, mak, args.rho)
+                        == ((256, 256, 256), 0.20, -1, 0, False, 10e-6) else None),
             "kernel_us": kernel_s * 1e6,
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        cb = generators.case_flow(n=args.n, phi=args.phi, nsteps=404)
+        cb = generators.case_flow(shape=shape, phi=args.phi, nsteps=404, rho=args.rho)
         out["cpu_baseline"] = cpu_baseline(cb, args.cpu_seconds, scheme=1 if mak else 0)
     print(json.dumps(out), flush=True)
     return 0

@@ -119,12 +119,16 @@ void auto_launch(const nmr_ctx* c, int variant, int lz, Launch* L)
     L->lz = 1;
     fill_launch(p, L);
     if (lz <= 0) {
-        // one-step: aim for >= 2048 workgroups (8 per CU) with >= 8 planes;
+        // one-step: 8 planes per z-chunk (measured best from 256^3 to
+        // 512x512x1024; longer chunks leave fewer, longer workgroups);
         // two-step (one 8-wave workgroup per CU): >= 512 workgroups
-        const long want = L->steps == 2 ? 512 : 2048;
-        const long tiles_xy = (long)L->ntx * L->nty;
-        const long ntz = std::max<long>(1, (want + tiles_xy - 1) / tiles_xy);
-        lz = (int)std::max<long>(L->steps == 2 ? 4 : 8, (p.nz + ntz - 1) / ntz);
+        if (L->steps == 2) {
+            const long tiles_xy = (long)L->ntx * L->nty;
+            const long ntz = std::max<long>(1, (512 + tiles_xy - 1) / tiles_xy);
+            lz = (int)std::max<long>(4, (p.nz + ntz - 1) / ntz);
+        } else {
+            lz = 8;
+        }
     }
     L->lz = lz;
     fill_launch(p, L);
@@ -302,16 +306,11 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
         if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
         // tile geometry: 256 threads, 4*lanes x (256/lanes) voxels, z-chunks
-        // of lz planes with >= 2048 workgroups where the grid allows
+        // of lz planes (8 unless set)
         const int lanes = nmr::gsweep_lanes(p.nx);
         const int tx = 4 * lanes, ty = 256 / lanes;
         const long ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty;
-        int lz = ctx->lz;
-        if (lz <= 0) {
-            const long want = 2048, tiles_xy = ntx * nty;
-            const long ntz0 = std::max<long>(1, (want + tiles_xy - 1) / tiles_xy);
-            lz = (int)std::max<long>(8, (p.nz + ntz0 - 1) / ntz0);
-        }
+        int lz = ctx->lz > 0 ? ctx->lz : 8;
         lz = std::min(std::max(lz, 1), p.nz);
         const long ntz = (p.nz + lz - 1) / lz;
         const long nwg = ntx * nty * ntz;

@@ -163,13 +163,18 @@ def case_small(seed: int = 0x4E4D52 + 0, n: int = 64, nsteps: int = 2000, phi: f
 
 
 def case_flow(seed: int = 0x4E4D52 + 1000, n: int = 256, phi: float = 0.20, g: float = 0.3, nsteps: int = 5000,
-              delta: float = 2e-3, big_delta: float = 20e-3, rho: float = 10e-6, shape=None) -> Case:
+              delta: float = 2e-3, big_delta: float = 20e-3, rho: float = 10e-6, shape=None, progress=None) -> Case:
     """Flow + PFG: porosity phi, divergence-free velocity (max 1 mm/s), PGSE
-    gradient g along z, complex (Mx, My)."""
+    gradient g along z, complex (Mx, My).  progress(msg), if given, is
+    called after each generation stage (large grids take minutes)."""
     shape = tuple(shape) if shape is not None else (n, n, n)
     dx, D = 1e-6, 2.3e-9
     mask = porosity(shape, phi, seed)
+    if progress:
+        progress(f"porosity {shape} done")
     vx, vy, vz = velocity(mask, seed, vmax=1e-3, d=(dx, dx, dx))
+    if progress:
+        progress(f"velocity {shape} done")
     dt = stable_dt(dx, D)
     sched = pgse_schedule(nsteps, dt, g, delta, big_delta)
     return Case(shape, mask, vx, vy, vz, dx, dt, D, 2.0, 2.5e10, rho, nsteps, sched, 2)
