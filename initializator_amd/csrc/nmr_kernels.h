@@ -214,6 +214,9 @@ struct ApplyArgs {
 };
 
 hipError_t launch_conv(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s);
+// tiles of 16 x 8 x 4 output voxels of the LDS-staged convolution (all rows)
+int conv_tiles(int nx, int ny, int nz);
+hipError_t launch_conv_tiled(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s);
 hipError_t launch_wgrad(int cin, const WgradArgs& a, int grid, hipStream_t s);
 hipError_t launch_wreduce(const float* part, int nwaves, int cin_real, int cout_real, float* grad, hipStream_t s);
 hipError_t launch_pack(const float* w, int cin_real, int cout_real, int cin_store, int backward, uint16_t* frag,

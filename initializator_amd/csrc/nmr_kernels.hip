@@ -1402,7 +1402,6 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     const int xlast = nx - 1 - ix;
     const bool xend = xin && xlast <= 3;
     const bool xpart = xin && xlast < 3;
-    const bool xbeg = ix == 0;
     const bool rin = xin && iy < ny;
     const int c = 4 + 4 * lx;
     const int lr = 1 + ry;

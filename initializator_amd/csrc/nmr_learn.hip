@@ -122,10 +122,14 @@ int repack(nmr_corrector* c)
 }
 
 int conv_grid(int nrows) { return std::max(1, std::min(2048, (nrows + 3) / 4)); }
+// persistent grid of the tiled convolution: 4 resident workgroups per CU
+// (34.6 KB of LDS each at 16 channels) on 256 CUs
+int tile_grid(const Tensors& t) { return std::max(1, std::min(1024, nmr::conv_tiles(t.nx, t.ny, t.nz))); }
 
-// Forward pass on the tensors (x already filled): a1, a2, p.  With row
-// lists (sparse = true), layer l computes only the rows c->rows[l-1]; the
-// lists are nested so that every row a later layer reads is computed.
+// Forward pass on the tensors (x already filled): a1, a2, p.  Layers 1 and
+// 2 run tiled over every row; with row lists (sparse = true) layer 3
+// computes only the rows c->rows[2] (those holding a pore), otherwise it
+// runs tiled too.
 int forward(nmr_corrector* c, bool sparse = false)
 {
     const Tensors& t = c->t;
@@ -140,25 +144,28 @@ int forward(nmr_corrector* c, bool sparse = false)
         a.nrows = sparse ? c->nrows[l] : all;
         return conv_grid(a.nrows);
     };
+    const int tg = tile_grid(t);
     a.in = t.x;
     a.wa = c->f1;
     a.bias = c->w + kB1;
     a.out = t.a1;
-    int grid = rows(0);
-    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(8, 0, a, grid, c->stream));
+    HIP_TRY(nmr::launch_conv_tiled(8, 0, a, tg, c->stream));
     a.in = t.a1;
     a.wa = c->f2;
     a.bias = c->w + kB2;
     a.out = t.a2;
-    grid = rows(1);
-    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 0, a, grid, c->stream));
+    HIP_TRY(nmr::launch_conv_tiled(16, 0, a, tg, c->stream));
     a.in = t.a2;
     a.wa = c->f3;
     a.bias = c->w + kB3;
     a.out = nullptr;
     a.outf = t.p;
-    grid = rows(2);
-    if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 1, a, grid, c->stream));
+    if (sparse) {
+        const int grid = rows(2);
+        if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 1, a, grid, c->stream));
+    } else {
+        HIP_TRY(nmr::launch_conv_tiled(16, 1, a, tg, c->stream));
+    }
     return NMR_OK;
 }
 
@@ -224,17 +231,17 @@ int backward(nmr_corrector* c)
     a.nz = t.nz;
     a.nseg = (t.nx + 15) / 16;
     a.nrows = a.nseg * t.ny * t.nz;
-    const int grid = conv_grid(a.nrows);
+    const int grid = tile_grid(t);
     a.in = t.g3;
     a.wa = c->d3;
     a.ref = t.a2;
     a.out = t.g2;
-    HIP_TRY(nmr::launch_conv(16, 2, a, grid, c->stream));
+    HIP_TRY(nmr::launch_conv_tiled(16, 2, a, grid, c->stream));
     a.in = t.g2;
     a.wa = c->d2;
     a.ref = t.a1;
     a.out = t.g1;
-    HIP_TRY(nmr::launch_conv(16, 2, a, grid, c->stream));
+    HIP_TRY(nmr::launch_conv_tiled(16, 2, a, grid, c->stream));
     nmr::WgradArgs w{};
     w.nx = t.nx;
     w.ny = t.ny;

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "nmr_kernels.h"
 
 namespace nmr {
@@ -106,6 +108,89 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
             *reinterpret_cast<uint2*>(a.out + base * 16 + 4 * g) =
                 make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
         }
+    }
+}
+
+// The same convolution over tiles of 16 x 8 x 4 output voxels: the tile's
+// input region (18 x 10 x 6 voxels, all channels) is staged in LDS once and
+// every B operand is read from there, instead of 27 gathers per voxel from
+// L1/L2.  Each wavefront computes 8 of the tile's 32 output rows with the
+// row kernel's MFMA sequence (same operands, same chunk order), so the two
+// kernels produce the same bits.  Persistent: workgroup b takes tiles b,
+// b + gridDim.x, ...; every row of the grid is computed (no row list).
+constexpr int kTileY = 8, kTileZ = 4;
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) conv_tile_kernel(ConvArgs a)
+{
+    constexpr int TPC = 32 / CIN;
+    constexpr int NCH = (27 + TPC - 1) / TPC;
+    constexpr int HX = 18, HY = kTileY + 2, HZ = kTileZ + 2;
+    constexpr int NV = HX * HY * HZ;
+    constexpr int V16 = CIN / 8; // 16-byte parts per voxel
+    __shared__ uint4 sin[NV * V16];
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    const int wave = threadIdx.x >> 6;
+    bf16x8 A[NCH];
+    const uint4* wa4 = reinterpret_cast<const uint4*>(a.wa);
+#pragma unroll
+    for (int c = 0; c < NCH; c++) A[c] = as_bf8(wa4[c * 64 + lane]);
+    const long px = a.nx + 2;
+    const int nty = (a.ny + kTileY - 1) / kTileY, ntz = (a.nz + kTileZ - 1) / kTileZ;
+    const int ntiles = a.nseg * nty * ntz;
+    const int part0 = ((8 * g) % CIN) / 8;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int seg = tile % a.nseg, r2 = tile / a.nseg;
+        const int x0 = seg * 16, y0 = (r2 % nty) * kTileY, z0 = (r2 / nty) * kTileZ;
+        // staged voxel (xx, yy, zz) = padded voxel (x0 + xx, y0 + yy, z0 + zz),
+        // clamped to the padded grid (clamped ones feed only outputs that are
+        // not stored)
+        for (int i = threadIdx.x; i < NV * V16; i += 256) {
+            const int v = i / V16, pt = i % V16;
+            const int xx = v % HX, yy = (v / HX) % HY, zz = v / (HX * HY);
+            const long gx = min(x0 + xx, a.nx + 1), gy = min(y0 + yy, a.ny + 1), gz = min(z0 + zz, a.nz + 1);
+            sin[i] = *reinterpret_cast<const uint4*>(a.in + ((gz * (a.ny + 2) + gy) * px + gx) * CIN + pt * 8);
+        }
+        __syncthreads();
+        for (int rr = wave; rr < kTileY * kTileZ; rr += 4) {
+            const int ly = rr % kTileY, lz = rr / kTileY;
+            fl4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const int t = c * TPC + (8 * g) / CIN;
+                uint4 B = make_uint4(0u, 0u, 0u, 0u);
+                if (t < 27) {
+                    const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+                    B = sin[(((lz + kz) * HY + (ly + ky)) * HX + (n + kx)) * V16 + part0];
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], as_bf8(B), acc, 0, 0, 0);
+            }
+            const int x = x0 + n, y = y0 + ly, z = z0 + lz;
+            if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
+            const long base = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * px + (x + 1);
+            // lane holds output channels 4g..4g+3 of voxel n
+            if constexpr (MODE == 1) {
+                if (g == 0) {
+                    a.outf[base * 2 + 0] = acc[0] + a.bias[0];
+                    a.outf[base * 2 + 1] = acc[1] + a.bias[1];
+                }
+            } else {
+                uint16_t h[4];
+                if constexpr (MODE == 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) h[j] = f2bf(fmaxf(acc[j] + a.bias[4 * g + j], 0.f));
+                } else {
+                    const uint2 rv = *reinterpret_cast<const uint2*>(a.ref + base * 16 + 4 * g);
+                    const uint16_t rr4[4] = {(uint16_t)(rv.x & 0xffffu), (uint16_t)(rv.x >> 16),
+                                             (uint16_t)(rv.y & 0xffffu), (uint16_t)(rv.y >> 16)};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) h[j] = bf_pos(rr4[j]) ? f2bf(acc[j]) : (uint16_t)0;
+                }
+                *reinterpret_cast<uint2*>(a.out + base * 16 + 4 * g) =
+                    make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -338,59 +423,57 @@ __global__ void __launch_bounds__(256) diff_kernel(const float* __restrict__ ax,
 // state's pore energy sum(Mx^2 + My^2), the largest |correction|, the
 // largest |M| of the state before the correction, a non-finite count, and
 // the signal sums over the trim window: part[block][6].
+constexpr int kApplyBlocks = 2048; // workgroups (= partials) of the apply pass
 __global__ void __launch_bounds__(256) apply_kernel(ApplyArgs a)
 {
-    __shared__ double sh[6][256];
+    __shared__ double sh[4][6];
     const long n = (long)a.nx * a.ny * a.nz;
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double e = 0.0, sx = 0.0, sy = 0.0, bad = 0.0, dmax = 0.0, mmax = 0.0;
-    if (v < n) {
+    double st[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; // energy, max |d|, max |M|, non-finite, sum Mx, sum My
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
         const int x = (int)(v % a.nx), y = (int)((v / a.nx) % a.ny), z = (int)(v / ((long)a.nx * a.ny));
         const size_t c = ((size_t)z * a.ny + y) * a.pitch + x;
-        if (a.mask[c]) {
-            float mx = a.mx[c], my = a.my[c];
-            mmax = fmaxf(fabsf(mx), fabsf(my));
-            if (a.pred) {
-                const long pv = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * (a.nx + 2) + (x + 1);
-                const float d0 = a.scale * a.pred[pv * 2], d1 = a.scale * a.pred[pv * 2 + 1];
-                mx += d0;
-                my += d1;
-                a.mx[c] = mx;
-                a.my[c] = my;
-                dmax = fmaxf(fabsf(d0), fabsf(d1));
-                if (!(fabsf(d0) <= 3.0e38f) || !(fabsf(d1) <= 3.0e38f)) bad = 1.0;
-            }
-            if (!(fabsf(mx) <= 3.0e38f) || !(fabsf(my) <= 3.0e38f)) bad = 1.0;
-            e = (double)mx * mx + (double)my * my;
-            if (x >= a.bx0 && x < a.bx1 && y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1) {
-                sx = mx;
-                sy = my;
-            }
+        if (!a.mask[c]) continue;
+        float mx = a.mx[c], my = a.my[c];
+        st[2] = fmax(st[2], (double)fmaxf(fabsf(mx), fabsf(my)));
+        if (a.pred) {
+            const long pv = ((long)(z + 1) * (a.ny + 2) + (y + 1)) * (a.nx + 2) + (x + 1);
+            const float d0 = a.scale * a.pred[pv * 2], d1 = a.scale * a.pred[pv * 2 + 1];
+            mx += d0;
+            my += d1;
+            a.mx[c] = mx;
+            a.my[c] = my;
+            st[1] = fmax(st[1], (double)fmaxf(fabsf(d0), fabsf(d1)));
+            if (!(fabsf(d0) <= 3.0e38f) || !(fabsf(d1) <= 3.0e38f)) st[3] += 1.0;
+        }
+        if (!(fabsf(mx) <= 3.0e38f) || !(fabsf(my) <= 3.0e38f)) st[3] += 1.0;
+        st[0] += (double)mx * mx + (double)my * my;
+        if (x >= a.bx0 && x < a.bx1 && y >= a.by0 && y < a.by1 && z >= a.bz0 && z < a.bz1) {
+            st[4] += mx;
+            st[5] += my;
         }
     }
-    sh[0][threadIdx.x] = e;
-    sh[1][threadIdx.x] = dmax;
-    sh[2][threadIdx.x] = mmax;
-    sh[3][threadIdx.x] = bad;
-    sh[4][threadIdx.x] = sx;
-    sh[5][threadIdx.x] = sy;
+    // wavefront, then workgroup (fixed order)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const double o = __shfl_xor(st[k], off);
+            st[k] = (k == 1 || k == 2) ? fmax(st[k], o) : st[k] + o;
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; k++) sh[wave][k] = st[k];
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
-            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
-            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
-            sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
-            sh[4][threadIdx.x] += sh[4][threadIdx.x + h];
-            sh[5][threadIdx.x] += sh[5][threadIdx.x + h];
-        }
-        __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double r = sh[0][k];
+        for (int w = 1; w < 4; w++) r = (k == 1 || k == 2) ? fmax(r, sh[w][k]) : r + sh[w][k];
+        a.part[(size_t)blockIdx.x * 6 + k] = r;
     }
-    if (threadIdx.x < 6) a.part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// Final reduction of apply_kernel's per-block statistics into out[6] (sums
-// for entries 0, 3, 4, 5, maxima for 1, 2), one workgroup, fixed order.
 __global__ void __launch_bounds__(256) stats_reduce_kernel(const double* __restrict__ part, int nblocks,
                                                            double* __restrict__ out)
 {
@@ -447,6 +530,22 @@ hipError_t launch_conv(int cin, int mode, const ConvArgs& a, int grid, hipStream
     else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_kernel<16, 0>), g, b, 0, s, a);
     else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_kernel<16, 1>), g, b, 0, s, a);
     else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_kernel<16, 2>), g, b, 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int conv_tiles(int nx, int ny, int nz)
+{
+    return ((nx + 15) / 16) * ((ny + kTileY - 1) / kTileY) * ((nz + kTileZ - 1) / kTileZ);
+}
+
+hipError_t launch_conv_tiled(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s)
+{
+    const dim3 g(grid), b(256);
+    if (cin == 8 && mode == 0) hipLaunchKernelGGL((conv_tile_kernel<8, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_tile_kernel<16, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_tile_kernel<16, 1>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_tile_kernel<16, 2>), g, b, 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -521,7 +620,7 @@ hipError_t launch_diff(const float* ax, const float* ay, const float* bx, const 
     return hipGetLastError();
 }
 
-int apply_blocks(int nx, int ny, int nz) { return blocks_for((long)nx * ny * nz); }
+int apply_blocks(int nx, int ny, int nz) { return (int)std::max<long>(1, std::min<long>(kApplyBlocks, blocks_for((long)nx * ny * nz))); }
 
 hipError_t launch_apply(const ApplyArgs& a, hipStream_t s)
 {
