@@ -216,12 +216,13 @@ int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, doub
 /* signal (nullable): [ninst][nsteps+1][ncomp]; step counting continues
  * across calls (the gradient schedule sees steps 0, 1, ... since create). */
 int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal);
-/* variant (-1 automatic / autotuned), z-chunk (0 automatic), steps per
- * captured graph (0: direct launches, no graph; must be even), grid
- * (1: as many workgroups as fit on the device at once, each looping over
- * tiles; 0: one workgroup per tile). */
+/* variant (-1 automatic / autotuned; -3 each instance its own one-step
+ * tile shape, the least padded one, with one launch per shape), z-chunk
+ * (0 automatic), steps per captured graph (0: direct launches, no graph;
+ * must be even), grid (1: as many workgroups as fit on the device at once,
+ * each looping over tiles; 0: one workgroup per tile). */
 int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps, int32_t persistent);
-/* Times every one-step tile variant, z-chunk and grid kind on this batch
+/* Times every one-step tile variant (and -3), z-chunk and grid kind on this batch
  * (repeated single steps from the current state, which stays unchanged) and
  * keeps the fastest; the choice is cached per set of instance shapes.  Any
  * output pointer may be NULL. */

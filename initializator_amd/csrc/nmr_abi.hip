@@ -1039,21 +1039,48 @@ int balloc(T** p, size_t count)
     return NMR_OK;
 }
 
-// Tile list, descriptors and grid for (variant, lz, grid kind).
+constexpr int kMixedVariant = -3; // every instance takes its own tile shape
+
+// One-step tile shape for an nx x ny instance under variant -3: the least
+// padded tile area weighted by each shape's measured cost per voxel on a
+// 256^3 grid (tile 256x4 fastest, 64x32 slowest).
+int mixed_variant(int nx, int ny)
+{
+    static const double speed[7] = {1.000, 1.088, 1.055, 1.154, 1.161, 1.257, 1.216};
+    int best = 0;
+    double bc = 1e300;
+    for (int v = 0; v < 7; v++) {
+        int lx, rpt, tx, ty, st;
+        if (!nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st) || st != 1) continue;
+        const double area = (double)((nx + tx - 1) / tx * tx) * (double)((ny + ty - 1) / ty * ty);
+        const double c = area * speed[v];
+        if (c < bc) {
+            bc = c;
+            best = v;
+        }
+    }
+    return best;
+}
+
+// Tile list, descriptors and grid for (variant, lz, grid kind); variant -3
+// gives each instance its own tile shape and groups the tiles by shape.
 int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
 {
     if (variant == b->cur_variant && lz == b->cur_lz && persistent == b->cur_persistent) return NMR_OK;
+    const bool mixed = variant == kMixedVariant;
     int lx, rpt, tx, ty, st;
-    if (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1)
+    if (!mixed && (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
         return fail(NMR_ERR_INVALID, "batch variant %d is not a one-step tile", variant);
     const nmr_problem& p = b->prob;
     std::vector<nmr::BatchInst> hi(b->ninst);
-    struct T { int inst, t; long cost; };
+    struct T { int inst, t, variant; long cost; };
     std::vector<T> tiles;
     for (int i = 0; i < b->ninst; i++) {
         const nmr_batch_instance& s = b->spec[i];
         auto& I = b->inst[i];
         nmr::BatchInst& d = hi[i];
+        const int vi = mixed ? mixed_variant(s.nx, s.ny) : variant;
+        nmr::variant_shape(vi, &lx, &rpt, &tx, &ty, &st);
         for (int q = 0; q < 3; q++) {
             d.m[0][q] = I.m[0][q];
             d.m[1][q] = I.m[1][q];
@@ -1091,11 +1118,20 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
         for (int t = 0; t < n; t++) {
             const int tz = t / (d.ntx * d.nty);
             const long planes = std::min(d.lz, s.nz - tz * d.lz);
-            tiles.push_back({i, t, planes * (long)tx * ty});
+            tiles.push_back({i, t, vi, planes * (long)tx * ty});
         }
     }
-    // longest tiles first (stable: ties keep instance/tile order)
-    std::stable_sort(tiles.begin(), tiles.end(), [](const T& a, const T& c) { return a.cost > c.cost; });
+    // by tile shape, then longest tiles first (stable: ties keep
+    // instance/tile order)
+    std::stable_sort(tiles.begin(), tiles.end(), [](const T& a, const T& c) {
+        return a.variant != c.variant ? a.variant < c.variant : a.cost > c.cost;
+    });
+    b->groups.clear();
+    for (size_t k = 0; k < tiles.size(); k++) {
+        if (b->groups.empty() || b->groups.back().variant != tiles[k].variant)
+            b->groups.push_back({tiles[k].variant, (int)k, 0, 0});
+        b->groups.back().count += 1;
+    }
     std::vector<int2> ht(tiles.size());
     std::vector<std::vector<int>> per(b->ninst);
     for (size_t k = 0; k < tiles.size(); k++) {
@@ -1123,13 +1159,16 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
     HIP_TRY(hipMemcpy(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     b->ntiles = (int)tiles.size();
-    if (persistent) {
-        const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, variant);
-        if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
-        b->grid = std::min(res, b->ntiles);
-    } else {
-        b->grid = b->ntiles;
+    for (auto& gr : b->groups) {
+        if (persistent) {
+            const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, gr.variant);
+            if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
+            gr.grid = std::min(res, gr.count);
+        } else {
+            gr.grid = gr.count;
+        }
     }
+    b->grid = b->groups.empty() ? 0 : b->groups[0].grid;
     b->cur_variant = variant;
     b->cur_lz = lz;
     b->cur_persistent = persistent;
@@ -1150,15 +1189,17 @@ int batch_partials(nmr_batch* b, size_t steps)
 
 int batch_launch_step(nmr_batch* b, int local, int parity, double* part)
 {
-    nmr::BatchArgs a{};
-    a.inst = b->d_inst;
-    a.tiles = b->d_tiles;
-    a.ntiles = b->ntiles;
-    a.parity = parity;
-    a.local_step = local;
-    a.step_base = b->d_step;
-    a.partials = part;
-    HIP_TRY(nmr::launch_batch(b->nc, b->vel, b->exact, b->cur_variant, b->grid, a, b->stream));
+    for (const auto& gr : b->groups) {
+        nmr::BatchArgs a{};
+        a.inst = b->d_inst;
+        a.tiles = b->d_tiles + gr.first;
+        a.ntiles = gr.count;
+        a.parity = parity;
+        a.local_step = local;
+        a.step_base = b->d_step;
+        a.partials = part + (size_t)gr.first * b->nc;
+        HIP_TRY(nmr::launch_batch(b->nc, b->vel, b->exact, gr.variant, gr.grid, a, b->stream));
+    }
     return NMR_OK;
 }
 
@@ -1374,7 +1415,8 @@ int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t gr
 {
     if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_set_tile: null batch");
     int lx, rpt, tx, ty, st;
-    if (variant != -1 && (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
+    if (variant != -1 && variant != kMixedVariant &&
+        (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
         return fail(NMR_ERR_INVALID, "batch variant %d", (int)variant);
     if (lz < 0 || graph_steps < 0 || (graph_steps & 1) || (persistent != 0 && persistent != 1))
         return fail(NMR_ERR_INVALID, "lz %d / graph_steps %d / persistent %d", lz, graph_steps, persistent);
@@ -1407,10 +1449,14 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
     const int trials = std::max(1, (int)trial_steps);
     double best = 1e300;
     int bv = 0, bl = 8, bp = 1;
-    for (int v = 0; v < nmr::num_variants(); v++) {
+    for (int v = kMixedVariant; v < nmr::num_variants(); v++) {
         int lx, rpt, tx, ty, st;
-        nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st);
-        if (st != 1) continue;
+        if (v >= 0) {
+            nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st);
+            if (st != 1) continue;
+        } else if (v != kMixedVariant) {
+            continue;
+        }
         for (int cfg = 0; cfg < 10; cfg++) {
             const int l = 4 << (cfg >> 1), per = cfg & 1; // lz 4..64, grid kind
             int rc = batch_prepare(b, v, l, per);

@@ -95,6 +95,10 @@ struct nmr_batch {
     int64_t steps_done = 0;
     // launch state for the current variant / lz / grid kind
     int cur_variant = -2, cur_lz = -1, cur_persistent = -1, grid = 0, ntiles = 0;
+    // per-instance tile shapes (variant -3): one launch per shape, over the
+    // tile range [first, first + count) of the tile list
+    struct Group { int variant, first, count, grid; };
+    std::vector<Group> groups;
     nmr::BatchInst* d_inst = nullptr;
     int2* d_tiles = nullptr;
     int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
