@@ -322,3 +322,31 @@ def test_two_phase_flow_against_fp64_loop(exe, tmp_path, cadence):
     err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
     print(f"[dropin] two-phase sequence 2, {cadence} cadence: {len(rows)} rows, max error {err:.2e} of the scale")
     assert err < 5e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nseq", [5, 9, 98, 99])
+def test_other_gpu_sequences_against_fp64_loop(exe, tmp_path, nseq):
+    """The remaining Sequence_gpu programs (cpp:2681-3146) on a porous box
+    with flow: 5 (one 180 deg pulse), 9 (90 - gradient - 180 - gradient) and
+    98 / 99 (the multiple-gradient pulses of flag 3); every results_gpu.csv
+    row matches the FP64 restatement of the reference's GPU loop."""
+    shape = (10, 9, 12)
+    mask = generators.porosity(shape, 0.55, seed=11, sigma=1.2)
+    vel = generators.velocity(mask, seed=11, sigma=1.5, vmax=3e-3)
+    table = np.linspace(-0.02, 0.02, 399)
+    cfg = _write_case(tmp_path, shape, mask, vel, grad_table=table, NSEQUENCE=nseq, TIME1=2e-4, TIME2=4e-4,
+                      R2_WAT=2e-5, R1_WAT=1e-5, T1_WAT=0.05, Printing_on=4)
+    args = ["350", "200", "100"]
+    c = _config(exe, tmp_path, *args, full=True)
+    cfg["gradientPulse"] = c["gradientPulse"]
+    cfg["multipleGradients"] = tuple(c["multipleGradients"])
+    _run(exe, tmp_path, *args)
+    rows = refio.read_results(tmp_path / "results_gpu.csv")
+    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=True)
+    assert rows.shape == ref.shape, (rows.shape, ref.shape)
+    np.testing.assert_allclose(rows[:, 0], ref[:, 0], rtol=2e-6)
+    scale = np.abs(ref[:, 1:]).max()
+    err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
+    print(f"[dropin] sequence {nseq}: {len(rows)} rows, max error {err:.2e} of the signal scale", flush=True)
+    assert err < 5e-6
