@@ -32,10 +32,10 @@ BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + Vx,
 # predictor) + corrector pass 37 B (reads the predictor and the state, 8 each)
 BYTES_PER_VOXEL_STEP_MAK = 66.0
 # DRAM bytes per sweep launch of the default workload (256^3, phi 0.20,
-# automatic tile): rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per
-# dispatch, scaled by what the same counters read for a plain 1 GiB copy
-# (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
-DEFAULT_TRAFFIC = 320.6e6 + 50.4e6
+# automatic tile, brick layout): rocprofv3 --pmc FETCH_SIZE (x2.0) +
+# WRITE_SIZE (x1.0) per dispatch, scaled by what the same counters read for a
+# plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
+DEFAULT_TRAFFIC = 246.8e6 + 50.4e6
 
 
 def cpu_baseline(case, target_s: float, scheme: int = 0):

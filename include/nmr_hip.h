@@ -172,9 +172,16 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out);
  * the kernels of the last nmr_hip_run, and the number of sweep launches. */
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
-/* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = the
- * per-voxel kernel that MacCormack and non-zero-derivative boundaries use,
- * which gives the same fields as the sweep) and z-chunk length
+/* Device layout of the one-step sweep (no reference counterpart): 1 (the
+ * default) keeps a copy of the state and the velocity in 4x4x2 bricks of
+ * 128 bytes for runs of 16 steps or more with tile variants 0, 2, 4, 6 --
+ * all-solid bricks are then never fetched -- converting the state in and
+ * out once per run; 0 always sweeps the row layout.  The fields are the
+ * same bits either way. */
+int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
+/* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = a
+ * one-thread-per-voxel kernel with the arithmetic of the sweeps, a
+ * cross-check) and z-chunk length
  * (0 = automatic).  nmr_hip_num_variants() gives the variant count;
  * nmr_hip_variant_info() describes one: lanes per row, rows per thread, tile
  * width and height, and time steps per pass (2 = temporally blocked sweep,

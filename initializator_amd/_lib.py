@@ -57,6 +57,7 @@ EXPORTS = (
     "nmr_hip_set_device",
     "nmr_hip_last_run_ms",
     "nmr_hip_set_tile",
+    "nmr_hip_set_layout",
     "nmr_hip_num_variants",
     "nmr_hip_variant_info",
     "nmr_hip_device_bytes",
@@ -192,6 +193,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_set_device": (c_int, [c_int32]),
         "nmr_hip_last_run_ms": (c_int, [vp, dp, POINTER(c_int64)]),
         "nmr_hip_set_tile": (c_int, [vp, c_int32, c_int32]),
+        "nmr_hip_set_layout": (c_int, [vp, c_int32]),
         "nmr_hip_num_variants": (c_int, []),
         "nmr_hip_variant_info": (
             c_int,

@@ -67,8 +67,11 @@ void free_all(nmr_ctx* c)
             if (c->m[b][q]) (void)hipFree(c->m[b][q]);
     for (int q = 0; q < 3; q++)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
-    for (float* q : {c->dn, c->r1v, c->r2v, c->w[0], c->w[1]})
+    for (float* q : {c->dn, c->r1v, c->r2v, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
         if (q) (void)hipFree(q);
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 3; q++)
+            if (c->mb[b][q]) (void)hipFree(c->mb[b][q]);
     if (c->mask) (void)hipFree(c->mask);
     if (c->vx) (void)hipFree(c->vx);
     if (c->vy) (void)hipFree(c->vy);
@@ -388,6 +391,45 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     return NMR_OK;
 }
 
+constexpr int64_t kBrickMinSteps = 16; // shorter runs keep the row layout (3 conversions per run)
+
+// Whether every brick array stays inside the sweep's 32-bit byte offsets.
+bool bricks_fit(const nmr_ctx* c)
+{
+    const nmr_problem& p = c->prob;
+    const int nbx = c->pitch / 4, nbx1 = c->pitch1 / 4;
+    const size_t big = std::max({nmr::brick_floats(nbx, p.ny, p.nz), nmr::brick_floats(nbx1, p.ny, p.nz),
+                                 nmr::brick_floats(nbx, p.ny + 1, p.nz), nmr::brick_floats(nbx, p.ny, p.nz + 1)});
+    return big * 4 < ((size_t)1 << 32) - 256;
+}
+
+// Brick copies of the state (both ping-pong buffers) and of the velocity,
+// allocated on the first brick run.
+int ensure_bricks(nmr_ctx* c)
+{
+    const nmr_problem& p = c->prob;
+    const int nbx = c->pitch / 4, nbx1 = c->pitch1 / 4;
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < c->nc; q++)
+            if (!c->mb[b][q]) {
+                int rc = dev_alloc(c, &c->mb[b][q], nmr::brick_floats(nbx, p.ny, p.nz));
+                if (rc) return rc;
+            }
+    if (c->vel && !c->vxb) {
+        int rc = dev_alloc(c, &c->vxb, nmr::brick_floats(nbx1, p.ny, p.nz));
+        if (!rc) rc = dev_alloc(c, &c->vyb, nmr::brick_floats(nbx, p.ny + 1, p.nz));
+        if (!rc) rc = dev_alloc(c, &c->vzb, nmr::brick_floats(nbx, p.ny, p.nz + 1));
+        if (rc) return rc;
+    }
+    if (c->vel && !c->brick_vel_valid) {
+        HIP_TRY(nmr::launch_brick(c->vx, c->vxb, nbx1, p.ny, p.nz, true, c->stream));
+        HIP_TRY(nmr::launch_brick(c->vy, c->vyb, nbx, p.ny + 1, p.nz, true, c->stream));
+        HIP_TRY(nmr::launch_brick(c->vz, c->vzb, nbx, p.ny, p.nz + 1, true, c->stream));
+        c->brick_vel_valid = true;
+    }
+    return NMR_OK;
+}
+
 // Whether solid magnetization can reach a pore (a face touching a solid
 // carries velocity); recomputed on the device after mask/velocity uploads.
 int refresh_inert(nmr_ctx* c)
@@ -643,6 +685,7 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     ctx->inert_dirty = true;
+    ctx->brick_vel_valid = false;
     return NMR_OK;
 }
 
@@ -791,6 +834,14 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
     return NMR_OK;
 }
 
+int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_set_layout: null context");
+    if (brick != 0 && brick != 1) return fail(NMR_ERR_INVALID, "layout %d", (int)brick);
+    ctx->brick_on = brick != 0;
+    return NMR_OK;
+}
+
 int nmr_hip_set_tile(nmr_ctx* ctx, int32_t variant, int32_t lz)
 {
     if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_set_tile: null context");
@@ -892,6 +943,8 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     if (rc0) return rc0;
     Launch L1{}, L2{};
     auto_launch(ctx, ctx->variant, ctx->lz, &L1);
+    const bool want_brick = ctx->brick_on && nsteps >= kBrickMinSteps && bricks_fit(ctx);
+    if (want_brick && ctx->variant < 0) auto_launch(ctx, nmr::brick_auto_variant(p.nx), ctx->lz, &L1);
     if (L1.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
     if (L1.steps == 2 && p.nx > L1.tx)
         return fail(NMR_ERR_INVALID, "variant %d needs nx <= %d", L1.variant, L1.tx);
@@ -908,11 +961,15 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 
     int w[6];
     signal_window(p, w);
+    // the one-step sweep runs on 4x4x2 bricks when it is long enough to pay
+    // for converting the state in and out (the velocity is converted once)
+    const bool brick = want_brick && L1.steps == 1 && nmr::brick_variant(L1.variant);
+    if (brick && (rc = ensure_bricks(ctx))) return rc;
     nmr::SweepArgs a{};
     a.mask = ctx->mask;
-    a.vx = ctx->vx;
-    a.vy = ctx->vy;
-    a.vz = ctx->vz;
+    a.vx = brick ? ctx->vxb : ctx->vx;
+    a.vy = brick ? ctx->vyb : ctx->vy;
+    a.vz = brick ? ctx->vzb : ctx->vz;
     a.nx = p.nx;
     a.ny = p.ny;
     a.nz = p.nz;
@@ -920,6 +977,8 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     a.pitch1 = ctx->pitch1;
     a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
     a.inert = ctx->inert;
+    a.nby = (p.ny + 3) / 4;
+    a.nbyv = (p.ny + 4) / 4;
     nmr::Sweep2Args b{};
     b.mask = ctx->mask;
     b.vx = ctx->vx;
@@ -944,6 +1003,14 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    // brick state: both brick buffers start as the current state (all-solid
+    // chunks are never written, so both must hold their values)
+    int bpar = 0;
+    if (brick)
+        for (int b = 0; b < 2; b++)
+            for (int q = 0; q < nc; q++)
+                HIP_TRY(nmr::launch_brick(ctx->m[ctx->cur][q], ctx->mb[b][q], ctx->pitch / 4, p.ny, p.nz, true,
+                                          ctx->stream));
     int64_t done = 0;
     while (done < nsteps) {
         const int64_t left = nsteps - done;
@@ -977,18 +1044,23 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
             for (int s = 0; s < chunk; s++) {
                 a.c = coef_of(done + s);
                 for (int q = 0; q < 3; q++) {
-                    a.min[q] = ctx->m[ctx->cur][q];
-                    a.mout[q] = ctx->m[ctx->cur ^ 1][q];
+                    a.min[q] = brick ? ctx->mb[bpar][q] : ctx->m[ctx->cur][q];
+                    a.mout[q] = brick ? ctx->mb[bpar ^ 1][q] : ctx->m[ctx->cur ^ 1][q];
                 }
                 a.partials = ctx->partials + (size_t)s * L.nwg * nc;
-                HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, L.variant, a, ctx->stream));
+                HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, L.variant, a, ctx->stream, brick));
                 ctx->cur ^= 1;
+                bpar ^= 1;
             }
         }
         HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, L.nwg, nc, ctx->dsignal + (size_t)(done + 1) * nc,
                                             ctx->stream));
         done += chunk;
     }
+    if (brick) // the final state back to the row layout of the current buffer
+        for (int q = 0; q < nc; q++)
+            HIP_TRY(nmr::launch_brick(ctx->mb[bpar][q], ctx->m[ctx->cur][q], ctx->pitch / 4, p.ny, p.nz, false,
+                                      ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     if (signal)
         HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * nc, hipMemcpyDeviceToHost,

@@ -32,6 +32,7 @@ struct SweepArgs {
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
     int inert;              // no pore-solid face carries velocity
+    int nby, nbyv;          // brick layout: brick rows per plane of M / Vx / Vz and of Vy
     Coef c;
 };
 
@@ -238,7 +239,16 @@ hipError_t launch_row_flags(const uint8_t* mask, int nx, int ny, int nz, int pit
 int num_variants();
 // steps: time steps per pass (1, or 2 for the temporally blocked sweep)
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);
-hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s);
+hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s,
+                        bool brick = false);
+// whether tile variant v has a brick-layout sweep, and the automatic one
+bool brick_variant(int v);
+int brick_auto_variant(int nx);
+// floats of a field of nc4 4-voxel chunks per row, Y rows, Z planes in
+// bricks of 8 x-voxels by 4 rows
+size_t brick_floats(int nc4, int Y, int Z);
+// row layout [Z][Y][4*nc4] <-> bricks (to_brick: row -> brick)
+hipError_t launch_brick(const float* src, float* dst, int nc4, int Y, int Z, bool to_brick, hipStream_t s);
 hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
 hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s);
 int batch_resident_blocks(int nc, bool vel, bool exact, int variant);

@@ -397,7 +397,11 @@ __device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[N
 
 // One tile of the one-step sweep: tile index t (x fastest, then y, then the
 // z-chunk).  Adds the signal of the tile's updated pore voxels into acc.
-template <int NC, bool VEL, bool EXACT, int LX, int RPT>
+// BRK: the float arrays are stored as bricks of 128 bytes holding 8 x-voxels
+// of 4 rows, so that a DRAM line holds a compact block of voxels (all-solid
+// lines are then never fetched) and a wavefront of 16 lanes x 4 rows reads
+// whole lines.
+template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, double (&acc)[NC])
 {
     constexpr int NT = 256;
@@ -439,6 +443,27 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
     const uint32_t planex_b = (uint32_t)ny * P1b;
     const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
+    // byte offsets of the float arrays along x, y, z (they add up): M and
+    // Vz (rows of pitch floats), Vx (pitch1), Vy (ny+1 rows per plane)
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3; // bricks per row
+    auto xo = [&](int x) -> uint32_t {
+        return BRK ? ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2) : (uint32_t)x * 4u;
+    };
+    auto yo = [&](int y) -> uint32_t {
+        return BRK ? (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * Pb;
+    };
+    auto yo1 = [&](int y) -> uint32_t {
+        return BRK ? (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * P1b;
+    };
+    auto zo = [&](int z) -> uint32_t {
+        return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx * 128u : (uint32_t)z * plane_b;
+    };
+    auto zo1 = [&](int z) -> uint32_t {
+        return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u : (uint32_t)z * planex_b;
+    };
+    auto zov = [&](int z) -> uint32_t {
+        return BRK ? (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u : (uint32_t)z * planey_b;
+    };
 
     int iy[RPT];
     bool rin[RPT];
@@ -447,15 +472,15 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     for (int r = 0; r < RPT; r++) {
         iy[r] = y0 + ry + r * RY;
         rin[r] = xin && (iy[r] < ny);
-        rowf[r] = (uint32_t)iy[r] * Pb + (uint32_t)ix * 4u;
+        rowf[r] = yo(iy[r]) + xo(ix);
         rowm[r] = (uint32_t)iy[r] * (uint32_t)a.pitch + (uint32_t)ix;
-        rowx[r] = (uint32_t)iy[r] * P1b + (uint32_t)ix * 4u;
+        rowx[r] = yo1(iy[r]) + xo(ix);
     }
     const bool top = (ry == 0);
     const bool bot = (ry == RY - 1);
     const int hy = top ? (y0 - 1) : (y0 + TY);
     const bool hrow = (top || bot) && xin && hy >= 0 && hy < ny;
-    const uint32_t hrowf = (uint32_t)(hrow ? hy : 0) * Pb + (uint32_t)ix * 4u;
+    const uint32_t hrowf = yo(hrow ? hy : 0) + xo(ix);
     const uint32_t hrowm = (uint32_t)(hrow ? hy : 0) * (uint32_t)a.pitch + (uint32_t)ix;
     const bool hcl = (lx == 0) && x0 > 0;
     const bool hcr = (lx == LX - 1) && x0 + TX < nx;
@@ -494,7 +519,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
         }
     };
     auto load_m = [&](int kk, const uint32_t (&kw)[RPT], f4 (&m)[RPT][NC]) {
-        const uint32_t pf = (uint32_t)clampz(kk) * plane_b;
+        const uint32_t pf = zo(clampz(kk));
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             if (rin[r] && (!skip_m || kw[r] != 0)) {
@@ -512,7 +537,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     };
     auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
         if (hrow && (!skip_m || hk != 0)) {
-            const uint32_t pf = (uint32_t)clampz(kk) * plane_b;
+            const uint32_t pf = zo(clampz(kk));
 #pragma unroll
             for (int q = 0; q < NC; q++) h[q] = ld4(a.min[q], pf + hrowf);
         } else {
@@ -546,11 +571,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     };
     auto load_xhalo = [&](int kk, float (&xv)[RPT][NC], uint32_t (&kx)[RPT]) {
         if (hcl || hcr) {
-            const uint32_t pf = (uint32_t)clampz(kk) * plane_b, pm = (uint32_t)clampz(kk) * plane_m;
+            const uint32_t pf = zo(clampz(kk)), pm = (uint32_t)clampz(kk) * plane_m;
 #pragma unroll
             for (int r = 0; r < RPT; r++) {
                 if (iy[r] < ny) {
-                    const uint32_t of = pf + (uint32_t)iy[r] * Pb + (uint32_t)hcx * 4u;
+                    const uint32_t of = pf + yo(iy[r]) + xo(hcx);
 #pragma unroll
                     for (int q = 0; q < NC; q++) xv[r][q] = ld1(a.min[q], of);
                     kx[r] = a.mask[pm + (uint32_t)iy[r] * (uint32_t)a.pitch + (uint32_t)hcx];
@@ -563,16 +588,16 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     auto load_vel = [&](int kk, const uint32_t (&kw)[RPT], f4 (&vx4)[RPT], float (&vxe)[RPT], f4 (&vy4)[RPT],
                         f4 (&vy4n)[RPT], f4 (&wc)[RPT], f4 (&wn)[RPT]) {
         const bool inz = kk < nz;
-        const uint32_t px = (uint32_t)kk * planex_b, py = (uint32_t)kk * planey_b, pz = (uint32_t)kk * plane_b;
+        const uint32_t px = zo1(kk), py = zov(kk), pz = zo(kk), pz1 = zo(kk + 1);
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
             if (inz && rin[r] && kw[r] != 0) {
                 vx4[r] = ld4_nt(a.vx, px + rowx[r]);
-                vxe[r] = (xlast >= 3) ? ld1(a.vx, px + rowx[r] + 16u) : 0.f; // face ix+4 <= nx
+                vxe[r] = (xlast >= 3) ? ld1(a.vx, px + yo1(iy[r]) + xo(ix + 4)) : 0.f; // face ix+4 <= nx
                 vy4[r] = ld4_nt(a.vy, py + rowf[r]);
-                vy4n[r] = ld4(a.vy, py + rowf[r] + Pb);
+                vy4n[r] = ld4(a.vy, py + yo(iy[r] + 1) + xo(ix));
                 wc[r] = ld4(a.vz, pz + rowf[r]);
-                wn[r] = ld4(a.vz, pz + plane_b + rowf[r]);
+                wn[r] = ld4(a.vz, pz1 + rowf[r]);
             } else {
                 vx4[r] = vy4[r] = vy4n[r] = wc[r] = wn[r] = f4{0.f, 0.f, 0.f, 0.f};
                 vxe[r] = 0.f;
@@ -708,7 +733,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
                                          kn[r], vx4[r], vxe[r], vy4[r], vy4n[r], wc[r], wn[r], ix, thy, thz, sn,
                                          cm1, out);
             if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
-            const uint32_t of = (uint32_t)k * plane_b + rowf[r];
+            const uint32_t of = zo(k) + rowf[r];
 #pragma unroll
             for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
         }
@@ -737,14 +762,14 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     }
 }
 
-template <int NC, bool VEL, bool EXACT, int LX, int RPT>
+template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
     __shared__ double sred[4][NC];
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) acc[q] = 0.0;
-    sweep_tile<NC, VEL, EXACT, LX, RPT>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    sweep_tile<NC, VEL, EXACT, LX, RPT, BRK>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
     block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
@@ -1792,6 +1817,54 @@ bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps)
     return true;
 }
 
+// Row layout [Z][Y][4*nc4] floats (nc4 4-voxel chunks per row) <-> bricks
+// of 8 x-voxels by 4 rows: brick (z, y/4, c/2) holds chunk c of rows
+// y&~3 .. +3 at float (y&3)*8 + (c&1)*4.  One thread per 16-byte chunk.
+__global__ void __launch_bounds__(256) brick_kernel(const float* __restrict__ src, float* __restrict__ dst, int nc4,
+                                                    int Y, int Z, int to_brick)
+{
+    const long nch = (long)nc4 * Y * Z;
+    const long nby = (Y + 3) / 4, nb8 = (nc4 + 1) / 2;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nch; i += (long)gridDim.x * 256) {
+        const int cx = (int)(i % nc4);
+        const long r = i / nc4;
+        const int y = (int)(r % Y), z = (int)(r / Y);
+        const size_t ro = (size_t)i * 4;
+        const size_t bo = (((size_t)z * nby + (y >> 2)) * nb8 + (cx >> 1)) * 32 + (size_t)((y & 3) * 8 + (cx & 1) * 4);
+        if (to_brick)
+            *reinterpret_cast<f4*>(dst + bo) = *reinterpret_cast<const f4*>(src + ro);
+        else
+            *reinterpret_cast<f4*>(dst + ro) = *reinterpret_cast<const f4*>(src + bo);
+    }
+}
+
+size_t brick_floats(int nc4, int Y, int Z) { return (size_t)((nc4 + 1) / 2) * ((Y + 3) / 4) * Z * 32; }
+
+hipError_t launch_brick(const float* src, float* dst, int nbx, int Y, int Z, bool to_brick, hipStream_t s)
+{
+    const long nch = (long)nbx * Y * Z;
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((nch + 255) / 256, 16384));
+    hipLaunchKernelGGL(brick_kernel, dim3(grid), dim3(256), 0, s, src, dst, nbx, Y, Z, to_brick ? 1 : 0);
+    return hipGetLastError();
+}
+
+bool brick_variant(int v) { return v == 0 || v == 2 || v == 4 || v == 6; }
+int brick_auto_variant(int nx) { return nx > 32 ? 4 : 6; }
+
+template <int NC, bool VEL, bool EXACT>
+static hipError_t launch_nc_brick(int v, const SweepArgs& a, hipStream_t s)
+{
+    const dim3 grid(a.nblocks), block(256);
+    switch (v) {
+    case 0: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 64, 1, true>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 32, 1, true>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 16, 1, true>), grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 8, 1, true>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int NC, bool VEL, bool EXACT>
 static hipError_t launch_nc(int v, const SweepArgs& a, hipStream_t s)
 {
@@ -1809,8 +1882,20 @@ static hipError_t launch_nc(int v, const SweepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s)
+hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s, bool brick)
 {
+    if (brick) {
+        if (nc == 1)
+            return vel ? launch_nc_brick<1, true, false>(variant, a, s) : launch_nc_brick<1, false, false>(variant, a, s);
+        if (nc == 2) {
+            if (exact)
+                return vel ? launch_nc_brick<2, true, true>(variant, a, s) : launch_nc_brick<2, false, true>(variant, a, s);
+            return vel ? launch_nc_brick<2, true, false>(variant, a, s) : launch_nc_brick<2, false, false>(variant, a, s);
+        }
+        if (nc == 3)
+            return vel ? launch_nc_brick<3, true, false>(variant, a, s) : launch_nc_brick<3, false, false>(variant, a, s);
+        return hipErrorInvalidValue;
+    }
     if (nc == 1) return vel ? launch_nc<1, true, false>(variant, a, s) : launch_nc<1, false, false>(variant, a, s);
     if (nc == 2) {
         if (exact) return vel ? launch_nc<2, true, true>(variant, a, s) : launch_nc<2, false, true>(variant, a, s);

@@ -1,0 +1,73 @@
+"""The one-step sweep on 4x4x2 bricks (nmr_hip_set_layout, the default for
+runs of 16 steps or more) against the row layout: the same operations on
+the same values, so the fields must agree bit for bit and the per-step
+signal to FP64 summation order; shapes cover x, y and z extents that are
+not multiples of the brick (4, 4, 2), more than one x tile, every tile
+variant with a brick sweep, one to three components and the exact phase."""
+import numpy as np
+import pytest
+
+from initializator_amd import generators
+from initializator_amd.solver import HipSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(case, m0, brick, variant=-1, nsteps=None, **kw):
+    n = case.nsteps if nsteps is None else nsteps
+    with HipSolver.from_case(case, **kw) as s:
+        s.set_layout(brick)
+        s.set_tile(variant, 0)
+        s.upload_magnetization(*m0[: case.ncomp])
+        sched = None if case.grad_sched is None else case.grad_sched[:n]
+        sig = s.run(n, sched)
+        f = s.download_magnetization()
+        sig2 = s.run(n, sched)  # a second run continues from the converted-back state
+        f2 = s.download_magnetization()
+    return sig, f, sig2, f2
+
+
+def _same(name, a, b):
+    for x, y in zip(a[1] + a[3], b[1] + b[3]):
+        assert np.array_equal(x, y), name
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-12, atol=1e-300)
+    print(f"[brick] {name}: fields bit-identical, signal within 1e-12", flush=True)
+
+
+@pytest.mark.parametrize("shape,variant", [((23, 37, 45), -1), ((9, 6, 261), 0), ((17, 14, 90), 2),
+                                           ((13, 21, 50), 4), ((11, 33, 30), 6), ((5, 3, 7), -1)])
+def test_brick_sweep_equals_row_sweep(shape, variant):
+    case = generators.case_flow(seed=sum(shape), shape=shape, phi=0.35, nsteps=40, big_delta=6e-4, delta=2e-4)
+    rng = np.random.default_rng(1)
+    m0 = [np.where(case.mask != 0, 1.0, rng.random(shape)), rng.random(shape) - 0.5, np.zeros(shape)]
+    _same(f"{shape} variant {variant}", _run(case, m0, True, variant), _run(case, m0, False, variant))
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+def test_brick_sweep_components(nc):
+    shape = (14, 19, 27)
+    case = generators.case_flow(seed=5, shape=shape, phi=0.4, nsteps=24)
+    case.ncomp = nc
+    rng = np.random.default_rng(nc)
+    m0 = [rng.random(shape) for _ in range(3)]
+    kw = {}
+    if nc == 1:
+        case.grad_sched = None
+    else:
+        case.grad_sched = np.tile([0.02, -0.01, 0.3], (case.nsteps, 1))
+        kw = dict(b0_dir=(0.2, 0.1, 0.9), p_eq=0.2)
+    _same(f"ncomp {nc}", _run(case, m0, True, **kw), _run(case, m0, False, **kw))
+
+
+def test_brick_sweep_exact_phase_and_velocity_on_solid_faces():
+    shape = (12, 15, 22)
+    case = generators.case_flow(seed=8, shape=shape, phi=0.4, nsteps=30)
+    rng = np.random.default_rng(4)
+    nz, ny, nx = shape
+    case.vx = rng.uniform(-1e-3, 1e-3, (nz, ny, nx + 1))
+    case.vy = rng.uniform(-1e-3, 1e-3, (nz, ny + 1, nx))
+    case.vz = rng.uniform(-1e-3, 1e-3, (nz + 1, ny, nx))
+    m0 = [rng.random(shape), rng.random(shape), None]
+    _same("exact phase, flowing solid faces", _run(case, m0, True, phase_exact=True),
+          _run(case, m0, False, phase_exact=True))
