@@ -72,6 +72,8 @@ void free_all(nmr_ctx* c)
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < 3; q++)
             if (c->mb[b][q]) (void)hipFree(c->mb[b][q]);
+    for (int q = 0; q < 3; q++)
+        if (c->pb[q]) (void)hipFree(c->pb[q]);
     if (c->mask) (void)hipFree(c->mask);
     if (c->vx) (void)hipFree(c->vx);
     if (c->vy) (void)hipFree(c->vy);
@@ -246,6 +248,9 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
 }
 
 int refresh_inert(nmr_ctx* c);
+bool bricks_fit(const nmr_ctx* c);
+int ensure_bricks(nmr_ctx* c);
+extern const int64_t kBrickMinSteps;
 
 // Steps of the general problems (MacCormack, non-zero-derivative boundaries,
 // per-voxel coefficients, two-phase sums): the general z-sweep, or with
@@ -308,9 +313,30 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // solid values, e.g. through periodic ghosts).
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
         if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
+        // brick layout (uniform coefficients, plain sums, long runs): the
+        // state in both brick buffers -- the output one from the row buffer
+        // whose solids were just cleared -- and, for MacCormack, a brick
+        // predictor
+        const bool gbrick = ctx->brick_on && !ctx->hetero && !ctx->weighted && nsteps >= kBrickMinSteps &&
+                            bricks_fit(ctx);
+        const int c0 = ctx->cur;
+        const int nc4 = ctx->pitch / 4;
+        if (gbrick) {
+            if ((rc = ensure_bricks(ctx))) return rc;
+            if (mak)
+                for (int q = 0; q < nc; q++)
+                    if (!ctx->pb[q] && (rc = dev_alloc(ctx, &ctx->pb[q], nmr::brick_floats(nc4, p.ny, p.nz))))
+                        return rc;
+            for (int q = 0; q < nc; q++) {
+                HIP_TRY(nmr::launch_brick(ctx->m[c0][q], ctx->mb[0][q], nc4, p.ny, p.nz, true, ctx->stream));
+                HIP_TRY(nmr::launch_brick(ctx->m[zero_first ? c0 ^ 1 : c0][q], ctx->mb[1][q], nc4, p.ny, p.nz, true,
+                                          ctx->stream));
+            }
+        }
+        int bpar = 0;
         // tile geometry: 256 threads, 4*lanes x (256/lanes) voxels, z-chunks
         // of lz planes (8 unless set)
-        const int lanes = nmr::gsweep_lanes(p.nx);
+        const int lanes = gbrick ? 16 : nmr::gsweep_lanes(p.nx);
         const int tx = 4 * lanes, ty = 256 / lanes;
         const long ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty;
         int lz = ctx->lz > 0 ? ctx->lz : 8;
@@ -344,31 +370,47 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
         a.skip_m = ctx->inert;
         a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
+        a.nby = (p.ny + 3) / 4;
+        a.nbyv = (p.ny + 4) / 4;
+        if (gbrick) {
+            a.vx = ctx->vxb;
+            a.vy = ctx->vyb;
+            a.vz = ctx->vzb;
+        }
         const bool coef = ctx->hetero;
         int64_t done = 0;
         while (done < nsteps) {
             const int chunk = (int)std::min<int64_t>(kChunkSteps, nsteps - done);
             for (int s = 0; s < chunk; s++) {
                 a.c = coef_of(done + s);
+                float* const* cin = gbrick ? ctx->mb[bpar] : ctx->m[ctx->cur];
+                float* const* cout = gbrick ? ctx->mb[bpar ^ 1] : ctx->m[ctx->cur ^ 1];
+                float* const* pred = gbrick ? ctx->pb : ctx->pred;
                 if (mak) {
                     for (int q = 0; q < 3; q++) {
-                        a.f[q] = ctx->m[ctx->cur][q];
-                        a.out[q] = ctx->pred[q];
+                        a.f[q] = cin[q];
+                        a.out[q] = pred[q];
                     }
                     a.partials = nullptr;
-                    HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, nmr::GEN_PREDICT, coef, lanes, a, ctx->stream));
+                    HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, nmr::GEN_PREDICT, coef, lanes, a, ctx->stream, gbrick));
                 }
                 for (int q = 0; q < 3; q++) {
-                    a.f[q] = mak ? ctx->pred[q] : ctx->m[ctx->cur][q];
-                    a.mavg[q] = ctx->m[ctx->cur][q];
-                    a.out[q] = ctx->m[ctx->cur ^ 1][q];
+                    a.f[q] = mak ? pred[q] : cin[q];
+                    a.mavg[q] = cin[q];
+                    a.out[q] = cout[q];
                 }
                 a.partials = fused ? ctx->partials + (size_t)s * nwg * nc : nullptr;
                 HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, mak ? nmr::GEN_CORRECT : nmr::GEN_UPWIND, coef, lanes, a,
-                                           ctx->stream));
+                                           ctx->stream, gbrick));
                 ctx->cur ^= 1;
+                bpar ^= 1;
                 if (zero_first && done + s == 0) {
-                    HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
+                    if (gbrick) // the next output buffer: the cleared row buffer's solids
+                        for (int q = 0; q < nc; q++)
+                            HIP_TRY(nmr::launch_brick(ctx->m[c0 ^ 1][q], ctx->mb[bpar ^ 1][q], nc4, p.ny, p.nz, true,
+                                                      ctx->stream));
+                    else
+                        HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
                     ctx->solids_zeroed = true;
                 }
                 if (!fused && (rc = enqueue_field_sum(ctx, (size_t)(done + s) + 1))) return rc;
@@ -377,6 +419,12 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                 HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)nwg, nc,
                                                     ctx->dsignal + (size_t)(done + 1) * nc, ctx->stream));
             done += chunk;
+        }
+        if (gbrick) { // the final state back to the current row buffer
+            for (int q = 0; q < nc; q++)
+                HIP_TRY(nmr::launch_brick(ctx->mb[bpar][q], ctx->m[ctx->cur][q], nc4, p.ny, p.nz, false, ctx->stream));
+            if (zero_first) // both row buffers hold 0 at solids from now on
+                HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -391,7 +439,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     return NMR_OK;
 }
 
-constexpr int64_t kBrickMinSteps = 16; // shorter runs keep the row layout (3 conversions per run)
+extern const int64_t kBrickMinSteps = 16; // shorter runs keep the row layout (3 conversions per run)
 
 // Whether every brick array stays inside the sweep's 32-bit byte offsets.
 bool bricks_fit(const nmr_ctx* c)

@@ -55,6 +55,7 @@ struct nmr_ctx {
     bool brick_on = true;       // nmr_hip_set_layout
     float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;
+    float* pb[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor in bricks
     bool brick_vel_valid = false;
     int nsig() const { return weighted ? 2 * nc : nc; }
     uint8_t* mask = nullptr;

@@ -135,6 +135,7 @@ struct GSweepArgs {
     int bc[3];              // BC_* per axis
     int skip_m;             // all-solid chunks are never read by a pore: not loaded
     int amask;              // upwind advection only across faces to pores (NMR_F_ADVECT_MASK)
+    int nby, nbyv;          // brick layout (lanes 16): brick rows per plane of M / Vx / Vz and of Vy
     Coef c;
 };
 
@@ -262,7 +263,8 @@ hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
 // lanes per tile row of the general sweep for a grid of width nx (tile 4*lanes wide)
 int gsweep_lanes(int nx);
-hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s);
+hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
+                         bool brick = false);
 // m[q][i] = 0 where mask[i] == 0, i < n, for q < nc
 hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s);
 // workgroups (= partials) of a field sum over the signal window

@@ -1394,7 +1394,7 @@ __device__ __forceinline__ f4 fill_tail(f4 v, int last, float g)
 // every voxel that is updated.  Ghost rows and columns are staged in LDS by
 // the threads that own the tile's halo; ghost planes are loaded in place of
 // the out-of-range planes.
-template <int NC, bool VEL, int ST, bool COEF, int LX>
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false>
 __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, double (&acc)[NC])
 {
     constexpr int NT = 256;
@@ -1437,10 +1437,29 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
     const uint32_t planex_b = (uint32_t)ny * P1b;
     const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
+    // byte offsets along x, y, z of the float arrays (row layout, or with
+    // BRK the 128-byte bricks of 8 x-voxels by 4 rows of the plain sweep);
+    // the per-voxel coefficient arrays always use the row layout
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3;
+    auto xo = [&](int x) -> uint32_t {
+        return BRK ? ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2) : (uint32_t)x * 4u;
+    };
+    auto yo = [&](int y) -> uint32_t {
+        return BRK ? (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * Pb;
+    };
+    auto yo1 = [&](int y) -> uint32_t {
+        return BRK ? (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * P1b;
+    };
+    auto zo = [&](int z) -> uint32_t {
+        return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx * 128u : (uint32_t)z * plane_b;
+    };
+    auto zo1 = [&](int z) -> uint32_t { return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u : (uint32_t)z * planex_b; };
+    auto zov = [&](int z) -> uint32_t { return BRK ? (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u : (uint32_t)z * planey_b; };
     const int yc = iy < ny ? iy : ny - 1; // rows past ny never load
-    const uint32_t rowf = (uint32_t)yc * Pb + (uint32_t)(xin ? ix : 0) * 4u;
+    const uint32_t rowf = yo(yc) + xo(xin ? ix : 0);
+    const uint32_t rowc = (uint32_t)yc * Pb + (uint32_t)(xin ? ix : 0) * 4u; // coefficient arrays
     const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
-    const uint32_t rowx = (uint32_t)yc * P1b + (uint32_t)(xin ? ix : 0) * 4u;
+    const uint32_t rowx = yo1(yc) + xo(xin ? ix : 0);
 
     // halo / ghost row: the top thread stages row y0-1, the bottom thread
     // row y0+TY, or the ghost row ny at LDS row 1 + (ny - y0) when the tile
@@ -1467,7 +1486,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         }
     }
     const bool hrow = (top || bot) && xin;
-    const uint32_t hrowf = (uint32_t)(hsrc >= 0 ? hsrc : 0) * Pb + (uint32_t)(xin ? ix : 0) * 4u;
+    const uint32_t hrowf = yo(hsrc >= 0 ? hsrc : 0) + xo(xin ? ix : 0);
     const uint32_t hrowm = (uint32_t)hmrow * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
 
     // x halo column: lane 0 loads column x0-1 (or, for periodic x in the
@@ -1507,7 +1526,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     auto load_m = [&](int kk, uint32_t kw, f4 (&m)[NC]) {
         const int zs = zsrc(kk);
         if (xin && zs >= 0 && (!skip_m || kw != 0)) {
-            const uint32_t of = (uint32_t)zs * plane_b + rowf;
+            const uint32_t of = zo(zs) + rowf;
 #pragma unroll
             for (int q = 0; q < NC; q++) m[q] = ld4_nt(a.f[q], of);
         } else {
@@ -1517,7 +1536,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     };
     auto load_d = [&](int kk) -> f4 {
         if constexpr (COEF) {
-            if (xin) return ld4_nt(a.dn, (uint32_t)clampz(kk) * plane_b + rowf);
+            if (xin) return ld4_nt(a.dn, (uint32_t)clampz(kk) * plane_b + rowc);
         }
         return f4{0.f, 0.f, 0.f, 0.f};
     };
@@ -1527,7 +1546,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
         const int zs = zsrc(kk);
         if (hrow && hsrc >= 0 && zs >= 0 && (!skip_m || hk != 0)) {
-            const uint32_t of = (uint32_t)zs * plane_b + hrowf;
+            const uint32_t of = zo(zs) + hrowf;
 #pragma unroll
             for (int q = 0; q < NC; q++) h[q] = ld4(a.f[q], of);
         } else {
@@ -1546,7 +1565,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         const int zs = zsrc(kk);
         if ((hcl || hcr) && iy < ny) {
             if (zs >= 0) {
-                const uint32_t of = (uint32_t)zs * plane_b + (uint32_t)iy * Pb + (uint32_t)hcx * 4u;
+                const uint32_t of = zo(zs) + yo(iy) + xo(hcx);
 #pragma unroll
                 for (int q = 0; q < NC; q++) xv[q] = ld1(a.f[q], of);
             } else {
@@ -1561,7 +1580,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         }
         if (gxr && iy < ny) {
             if (zs >= 0) {
-                const uint32_t of = (uint32_t)zs * plane_b + (uint32_t)iy * Pb;
+                const uint32_t of = zo(zs) + yo(iy) + xo(0);
 #pragma unroll
                 for (int q = 0; q < NC; q++) gv[q] = ld1(a.f[q], of);
             } else {
@@ -1599,20 +1618,21 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     // per-voxel inputs of plane kk (own chunk; only chunks holding a pore)
     auto load_own = [&](int kk, uint32_t kw) {
         if (rin && kw != 0) {
-            const uint32_t pz = (uint32_t)kk * plane_b;
+            const uint32_t pz = zo(kk);
             if constexpr (VEL) {
-                const uint32_t px = (uint32_t)kk * planex_b, py = (uint32_t)kk * planey_b;
+                const uint32_t px = zo1(kk), py = zov(kk);
                 vx4 = ld4_nt(a.vx, px + rowx);
-                vxe = (xlast >= 3) ? ld1(a.vx, px + rowx + 16u) : 0.f;
+                vxe = (xlast >= 3) ? ld1(a.vx, px + yo1(yc) + xo(ix + 4)) : 0.f;
                 vy4 = ld4_nt(a.vy, py + rowf);
-                vy4n = ld4(a.vy, py + rowf + Pb);
+                vy4n = ld4(a.vy, py + yo(yc + 1) + xo(ix));
                 wc = ld4(a.vz, pz + rowf);
-                wn = ld4(a.vz, pz + plane_b + rowf);
+                wn = ld4(a.vz, zo(kk + 1) + rowf);
             }
             if constexpr (COEF) {
-                r2c = ld4_nt(a.r2v, pz + rowf);
+                const uint32_t pc = (uint32_t)kk * plane_b + rowc;
+                r2c = ld4_nt(a.r2v, pc);
                 if constexpr (NC == 3) {
-                    if (a.r1v) r1c = ld4_nt(a.r1v, pz + rowf);
+                    if (a.r1v) r1c = ld4_nt(a.r1v, pc);
                 }
             }
             if constexpr (ST == GEN_CORRECT) {
@@ -1724,7 +1744,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                                              r2c, mav, a.amask != 0,
                                              ix, thy, thz, out);
             if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
-            const uint32_t of = (uint32_t)k * plane_b + rowf;
+            const uint32_t of = zo(k) + rowf;
 #pragma unroll
             for (int q = 0; q < NC; q++) st4_nt(a.out[q], of, out[q]);
         }
@@ -1754,14 +1774,14 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     }
 }
 
-template <int NC, bool VEL, int ST, bool COEF, int LX>
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false>
 __global__ void __launch_bounds__(256) gsweep_kernel(GSweepArgs a)
 {
     __shared__ double sred[4][NC];
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) acc[q] = 0.0;
-    gsweep_tile<NC, VEL, ST, COEF, LX>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    gsweep_tile<NC, VEL, ST, COEF, LX, BRK>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
     if (ST != GEN_PREDICT && a.partials) block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
@@ -2069,8 +2089,31 @@ static hipError_t launch_gs(int stage, int lanes, const GSweepArgs& a, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s)
+template <int NC, bool VEL>
+static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 {
+    const dim3 grid(a.nblocks), block(256);
+    switch (stage) {
+    case GEN_UPWIND: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, false, 16, true>), grid, block, 0, s, a); break;
+    case GEN_PREDICT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, false, 16, true>), grid, block, 0, s, a); break;
+    case GEN_CORRECT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, false, 16, true>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
+                         bool brick)
+{
+    if (brick) { // lanes 16, uniform coefficients
+        if (coef || lanes != 16) return hipErrorInvalidValue;
+        switch (nc) {
+        case 1: return vel ? launch_gs_brick<1, true>(stage, a, s) : launch_gs_brick<1, false>(stage, a, s);
+        case 2: return vel ? launch_gs_brick<2, true>(stage, a, s) : launch_gs_brick<2, false>(stage, a, s);
+        case 3: return vel ? launch_gs_brick<3, true>(stage, a, s) : launch_gs_brick<3, false>(stage, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
 #define NMR_GSC(NC_)                                                                \
     if (coef) return vel ? launch_gs<NC_, true, true>(stage, lanes, a, s)           \
                          : launch_gs<NC_, false, true>(stage, lanes, a, s);         \
