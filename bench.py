@@ -36,6 +36,10 @@ BYTES_PER_VOXEL_STEP_MAK = 66.0
 # WRITE_SIZE (x1.0) per dispatch, scaled by what the same counters read for a
 # plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
 DEFAULT_TRAFFIC = 246.8e6 + 50.4e6
+# The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
+# predictor and the corrector launch of one step together (summary under
+# profiles/, gsweep_maccormack_256cube_flow_pgse.md).
+MAK_TRAFFIC = (247.2e6 + 50.4e6) + (314.9e6 + 50.4e6)
 
 
 def cpu_baseline(case, target_s: float, scheme: int = 0):
@@ -433,8 +437,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (DEFAULT_TRAFFIC if (shape, args.phi, args.variant, args.lz, mak, args.rho)
-                        == ((256, 256, 256), 0.20, -1, 0, False, 10e-6) else None),
+            "traffic": ((MAK_TRAFFIC if mak else DEFAULT_TRAFFIC)
+                        if (shape, args.phi, args.variant, args.lz, args.rho)
+                        == ((256, 256, 256), 0.20, -1, 0, 10e-6) else None),
             "kernel_us": kernel_s * 1e6,
         },
         "cpu_baseline": None,
