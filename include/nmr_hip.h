@@ -272,6 +272,18 @@ int nmr_hip_corrector_set_params(nmr_corrector* net, const float* params);
  * its targets). */
 int nmr_hip_corrector_set_scales(nmr_corrector* net, double vscale, double out_scale);
 int nmr_hip_corrector_get_scales(nmr_corrector* net, double* vscale, double* out_scale);
+/* Convolution kernel of the corrector (all give the same bits):
+ * NMR_CONV_ZMARCH (default) marches columns of 16 x 16 output rows along z
+ * through a ring of LDS planes, zc planes per column (0: 16);
+ * NMR_CONV_TILES stages 16 x 8 x 4-voxel tiles; NMR_CONV_ROWS gathers each
+ * row's operands from memory; NMR_CONV_ZMARCH_SPARSE is NMR_CONV_ZMARCH with
+ * the learned solve's last layer computed only on rows that hold a pore
+ * (as NMR_CONV_TILES and NMR_CONV_ROWS do). */
+#define NMR_CONV_ZMARCH 0
+#define NMR_CONV_TILES 1
+#define NMR_CONV_ROWS 2
+#define NMR_CONV_ZMARCH_SPARSE 3
+int nmr_hip_corrector_set_conv(nmr_corrector* net, int32_t kind, int32_t zc);
 /* The net on host data: features [nz][ny][nx][6] -> out [nz][ny][nx][2]. */
 int nmr_hip_corrector_forward(nmr_corrector* net, int32_t nx, int32_t ny, int32_t nz, const float* features,
                               float* out);

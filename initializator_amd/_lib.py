@@ -36,6 +36,11 @@ NMR_BC_NEUMANN = 0
 NMR_BC_PERIODIC = 1
 NMR_BC_DIRICHLET = 2
 
+NMR_CONV_ZMARCH = 0
+NMR_CONV_TILES = 1
+NMR_CONV_ROWS = 2
+NMR_CONV_ZMARCH_SPARSE = 3
+
 NMR_PHASE_LINEAR = 0
 NMR_PHASE_EXACT = 1
 
@@ -76,6 +81,7 @@ EXPORTS = (
     "nmr_hip_corrector_set_params",
     "nmr_hip_corrector_set_scales",
     "nmr_hip_corrector_get_scales",
+    "nmr_hip_corrector_set_conv",
     "nmr_hip_corrector_forward",
     "nmr_hip_corrector_backward",
     "nmr_hip_corrector_train",
@@ -215,6 +221,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_corrector_set_params": (c_int, [vp, POINTER(ctypes.c_float)]),
         "nmr_hip_corrector_set_scales": (c_int, [vp, c_double, c_double]),
         "nmr_hip_corrector_get_scales": (c_int, [vp, dp, dp]),
+        "nmr_hip_corrector_set_conv": (c_int, [vp, c_int32, c_int32]),
         "nmr_hip_corrector_forward": (c_int, [vp, c_int32, c_int32, c_int32, POINTER(ctypes.c_float),
                                               POINTER(ctypes.c_float)]),
         "nmr_hip_corrector_backward": (c_int, [vp, c_int32, c_int32, c_int32, POINTER(ctypes.c_float),

@@ -171,6 +171,7 @@ struct ConvArgs {
     const int* rows;        // row list, row = (z*ny + y)*nseg + seg; null = all rows
     int nrows;
     int nx, ny, nz, nseg;   // nseg = ceil(nx/16)
+    int zc;                 // output planes per column of the z-marching kernel (0 = 16)
 };
 
 struct WgradArgs {
@@ -219,6 +220,9 @@ hipError_t launch_conv(int cin, int mode, const ConvArgs& a, int grid, hipStream
 // tiles of 16 x 8 x 4 output voxels of the LDS-staged convolution (all rows)
 int conv_tiles(int nx, int ny, int nz);
 hipError_t launch_conv_tiled(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s);
+// columns of 16 x 16 output rows by zc planes of the z-marching convolution
+int conv_columns(int nx, int ny, int nz, int zc);
+hipError_t launch_conv_zm(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s);
 hipError_t launch_wgrad(int cin, const WgradArgs& a, int grid, hipStream_t s);
 hipError_t launch_wreduce(const float* part, int nwaves, int cin_real, int cout_real, float* grad, hipStream_t s);
 hipError_t launch_pack(const float* w, int cin_real, int cout_real, int cin_store, int backward, uint16_t* frag,

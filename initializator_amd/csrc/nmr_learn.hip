@@ -62,6 +62,9 @@ struct nmr_corrector {
     int64_t rows_version = -1;
     int* rows[3] = {nullptr, nullptr, nullptr};
     int nrows[3] = {0, 0, 0};
+    // convolution kernel (NMR_CONV_*) and planes per z-marching column
+    int conv = 0;
+    int zc = 16;
 };
 
 namespace {
@@ -125,6 +128,29 @@ int conv_grid(int nrows) { return std::max(1, std::min(2048, (nrows + 3) / 4)); 
 // persistent grid of the tiled convolution: 4 resident workgroups per CU
 // (34.6 KB of LDS each at 16 channels) on 256 CUs
 int tile_grid(const Tensors& t) { return std::max(1, std::min(1024, nmr::conv_tiles(t.nx, t.ny, t.nz))); }
+// persistent grid of the z-marching convolution: 3 resident workgroups per
+// CU at 16 input channels (41.5 KB of LDS each), 4 at 8
+int zm_grid(const Tensors& t, int cin, int zc)
+{
+    return std::max(1, std::min(cin == 8 ? 1024 : 768, nmr::conv_columns(t.nx, t.ny, t.nz, zc)));
+}
+
+// One dense convolution layer with the context's kernel choice.
+int conv_dense(nmr_corrector* c, int cin, int mode, nmr::ConvArgs a)
+{
+    const Tensors& t = c->t;
+    if (c->conv == NMR_CONV_TILES) {
+        HIP_TRY(nmr::launch_conv_tiled(cin, mode, a, tile_grid(t), c->stream));
+    } else if (c->conv == NMR_CONV_ROWS) {
+        a.rows = nullptr;
+        a.nrows = a.nseg * t.ny * t.nz;
+        HIP_TRY(nmr::launch_conv(cin, mode, a, conv_grid(a.nrows), c->stream));
+    } else {
+        a.zc = c->zc;
+        HIP_TRY(nmr::launch_conv_zm(cin, mode, a, zm_grid(t, cin, c->zc), c->stream));
+    }
+    return NMR_OK;
+}
 
 // Forward pass on the tensors (x already filled): a1, a2, p.  Layers 1 and
 // 2 run tiled over every row; with row lists (sparse = true) layer 3
@@ -144,29 +170,29 @@ int forward(nmr_corrector* c, bool sparse = false)
         a.nrows = sparse ? c->nrows[l] : all;
         return conv_grid(a.nrows);
     };
-    const int tg = tile_grid(t);
     a.in = t.x;
     a.wa = c->f1;
     a.bias = c->w + kB1;
     a.out = t.a1;
-    HIP_TRY(nmr::launch_conv_tiled(8, 0, a, tg, c->stream));
+    int rc = conv_dense(c, 8, 0, a);
+    if (rc) return rc;
     a.in = t.a1;
     a.wa = c->f2;
     a.bias = c->w + kB2;
     a.out = t.a2;
-    HIP_TRY(nmr::launch_conv_tiled(16, 0, a, tg, c->stream));
+    rc = conv_dense(c, 16, 0, a);
+    if (rc) return rc;
     a.in = t.a2;
     a.wa = c->f3;
     a.bias = c->w + kB3;
     a.out = nullptr;
     a.outf = t.p;
-    if (sparse) {
+    if (sparse && c->conv != NMR_CONV_ZMARCH) {
         const int grid = rows(2);
         if (a.nrows > 0) HIP_TRY(nmr::launch_conv(16, 1, a, grid, c->stream));
-    } else {
-        HIP_TRY(nmr::launch_conv_tiled(16, 1, a, tg, c->stream));
+        return NMR_OK;
     }
-    return NMR_OK;
+    return conv_dense(c, 16, 1, a);
 }
 
 // Row lists of a context's mask: layer 3 computes the 16-voxel rows that
@@ -231,17 +257,18 @@ int backward(nmr_corrector* c)
     a.nz = t.nz;
     a.nseg = (t.nx + 15) / 16;
     a.nrows = a.nseg * t.ny * t.nz;
-    const int grid = tile_grid(t);
     a.in = t.g3;
     a.wa = c->d3;
     a.ref = t.a2;
     a.out = t.g2;
-    HIP_TRY(nmr::launch_conv_tiled(16, 2, a, grid, c->stream));
+    int rc = conv_dense(c, 16, 2, a);
+    if (rc) return rc;
     a.in = t.g2;
     a.wa = c->d2;
     a.ref = t.a1;
     a.out = t.g1;
-    HIP_TRY(nmr::launch_conv_tiled(16, 2, a, grid, c->stream));
+    rc = conv_dense(c, 16, 2, a);
+    if (rc) return rc;
     nmr::WgradArgs w{};
     w.nx = t.nx;
     w.ny = t.ny;
@@ -413,6 +440,17 @@ int nmr_hip_corrector_set_scales(nmr_corrector* c, double vscale, double out_sca
     if (!(vscale > 0) || !(out_scale >= 0)) return fail(NMR_ERR_INVALID, "scales %g %g", vscale, out_scale);
     c->vscale = vscale;
     c->out_scale = out_scale;
+    return NMR_OK;
+}
+
+int nmr_hip_corrector_set_conv(nmr_corrector* c, int32_t kind, int32_t zc)
+{
+    if (!c) return fail(NMR_ERR_NULL, "nmr_hip_corrector_set_conv: null argument");
+    if (kind < NMR_CONV_ZMARCH || kind > NMR_CONV_ZMARCH_SPARSE)
+        return fail(NMR_ERR_INVALID, "nmr_hip_corrector_set_conv: kind %d", kind);
+    if (zc < 0 || zc > 4096) return fail(NMR_ERR_INVALID, "nmr_hip_corrector_set_conv: zc %d", zc);
+    c->conv = kind;
+    c->zc = zc > 0 ? zc : 16;
     return NMR_OK;
 }
 

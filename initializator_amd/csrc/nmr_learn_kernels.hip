@@ -194,6 +194,177 @@ __global__ void __launch_bounds__(256) conv_tile_kernel(ConvArgs a)
     }
 }
 
+// The same convolution marching along z through columns of 16 x 16 output
+// rows: the input planes of the column (18 x 18 voxels, all channels) pass
+// through a ring of four LDS slots, so that each is loaded once per column
+// instead of once per 16 x 8 x 4 tile (1.27 input reads per output voxel
+// instead of 2.1), and the next plane's global loads are in flight while the
+// current output plane is computed.  Output plane z reads padded input
+// planes z, z + 1, z + 2 (slots z & 3, ...); plane z + 3 goes into slot
+// (z + 3) & 3, which no wavefront reads during plane z, and the barrier at
+// the end of each plane orders it before its first read.  Wavefront w
+// computes rows w, w + 4, w + 8, w + 12 of a plane as four independent
+// accumulators that share each weight fragment; per row the MFMA sequence is
+// the row kernel's (same operands, same chunk order), so the bits equal
+// those of conv_kernel and conv_tile_kernel.  Persistent: workgroup b takes
+// columns b, b + gridDim.x, ...; a column is a.zc output planes (the last
+// one what is left).
+constexpr int kZmY = 16;
+template <int N>
+struct Plane {
+    uint4 v[N];
+};
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
+{
+    constexpr int TPC = 32 / CIN;
+    constexpr int NCH = (27 + TPC - 1) / TPC;
+    constexpr int HX = 18, HY = kZmY + 2;
+    constexpr int V16 = CIN / 8;
+    constexpr int PV = HX * HY * V16;          // uint4 per staged plane
+    constexpr int NLD = (PV + 255) / 256;
+    constexpr int ROWB = HX * V16 * 16;         // bytes per staged row
+    __shared__ uint4 sin[4 * PV];
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, n = lane & 15;
+    const int wave = threadIdx.x >> 6;
+    bf16x8 A[NCH];
+    const uint4* wa4 = reinterpret_cast<const uint4*>(a.wa);
+#pragma unroll
+    for (int c = 0; c < NCH; c++) A[c] = as_bf8(wa4[c * 64 + lane]);
+    // this lane's B element of chunk c within a staged plane, in bytes from
+    // the wavefront's first row; hi bit c: the lane's tap lies in the later
+    // of the two planes a chunk spans; `last`: the lane's tap of the last
+    // chunk exists (t < 27)
+    int inner[NCH];
+    uint32_t hi = 0;
+    const int tg = (8 * g) / CIN, part0 = ((8 * g) % CIN) / 8;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        const int t = min(c * TPC + tg, 26);
+        const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+        inner[c] = ((ky * HX + n + kx) * V16 + part0) * 16;
+        if (kz != (c * TPC) / 9) hi |= 1u << c;
+    }
+    const bool last = (NCH - 1) * TPC + tg < 27;
+    float bias[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) bias[j] = a.bias[4 * g + j];
+    } else if constexpr (MODE == 1) {
+        bias[0] = a.bias[0];
+        bias[1] = a.bias[1];
+    }
+    const long px = a.nx + 2;
+    const long pstride = (long)(a.ny + 2) * px * CIN; // elements per padded plane
+    const int zc = a.zc > 0 ? a.zc : 16;
+    const int nyb = (a.ny + kZmY - 1) / kZmY, nzb = (a.nz + zc - 1) / zc;
+    const int ncol = a.nseg * nyb * nzb;
+    const char* lds = reinterpret_cast<const char*>(sin);
+    for (int col = blockIdx.x; col < ncol; col += gridDim.x) {
+        const int seg = col % a.nseg, r2 = col / a.nseg;
+        const int x0 = seg * 16, y0 = (r2 % nyb) * kZmY, zs = (r2 / nyb) * zc;
+        const int ze = min(zs + zc, a.nz);
+        // staged element i = threadIdx.x + 256 j of a plane: voxel (xx, yy) =
+        // padded (x0 + xx, y0 + yy), clamped to the padded grid (clamped ones
+        // feed only outputs that are not stored); a thread's element past the
+        // plane loads element 0 again and is not stored
+        long goff[NLD];
+#pragma unroll
+        for (int j = 0; j < NLD; j++) {
+            const int i = threadIdx.x + 256 * j < PV ? threadIdx.x + 256 * j : 0;
+            const int v = i / V16, pt = i % V16;
+            const long gx = min(x0 + v % HX, a.nx + 1), gy = min(y0 + v / HX, a.ny + 1);
+            goff[j] = (gy * px + gx) * CIN + pt * 8;
+        }
+        auto load = [&](int pz) {
+            Plane<NLD> r;
+#pragma unroll
+            for (int j = 0; j < NLD; j++)
+                r.v[j] = *reinterpret_cast<const uint4*>(a.in + (long)pz * pstride + goff[j]);
+            return r;
+        };
+        auto stash = [&](int pz, const Plane<NLD>& r) {
+#pragma unroll
+            for (int j = 0; j < NLD; j++)
+                if (threadIdx.x + 256 * j < PV) sin[(pz & 3) * PV + threadIdx.x + 256 * j] = r.v[j];
+        };
+        {
+            const Plane<NLD> r0 = load(zs), r1 = load(zs + 1), r2v = load(zs + 2);
+            stash(zs, r0);
+            stash(zs + 1, r1);
+            stash(zs + 2, r2v);
+        }
+        __syncthreads();
+        const int x = x0 + n;
+        for (int z = zs; z < ze; z++) {
+            // output voxel of row i of this wavefront
+            long base[4];
+            bool store[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int y = y0 + wave + 4 * i;
+                store[i] = x < a.nx && y < a.ny;
+                base[i] = ((long)(z + 1) * (a.ny + 2) + (min(y, a.ny - 1) + 1)) * px + (min(x, a.nx - 1) + 1);
+            }
+            uint2 gate[4];
+            if constexpr (MODE == 2) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) gate[i] = *reinterpret_cast<const uint2*>(a.ref + base[i] * 16 + 4 * g);
+            }
+            const bool more = z + 3 <= ze + 1;
+            Plane<NLD> nx4{};
+            if (more) nx4 = load(z + 3);
+            int sb[3];
+#pragma unroll
+            for (int kz = 0; kz < 3; kz++) sb[kz] = ((z + kz) & 3) * PV * 16 + wave * ROWB;
+            fl4 acc[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[i] = fl4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const int kzl = (c * TPC) / 9, kzh = min(c * TPC + TPC - 1, 26) / 9;
+                const int ad = ((kzl == kzh || !((hi >> c) & 1u)) ? sb[kzl] : sb[kzh]) + inner[c];
+                uint4 B[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    B[i] = *reinterpret_cast<const uint4*>(lds + ad + i * 4 * ROWB);
+                    if (c == NCH - 1 && !last) B[i] = make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], as_bf8(B[i]), acc[i], 0, 0, 0);
+            }
+            // lane holds output channels 4g..4g+3 of voxel n
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!store[i]) continue;
+                if constexpr (MODE == 1) {
+                    if (g == 0) {
+                        a.outf[base[i] * 2 + 0] = acc[i][0] + bias[0];
+                        a.outf[base[i] * 2 + 1] = acc[i][1] + bias[1];
+                    }
+                } else {
+                    uint16_t h[4];
+                    if constexpr (MODE == 0) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++) h[j] = f2bf(fmaxf(acc[i][j] + bias[j], 0.f));
+                    } else {
+                        const uint16_t rr4[4] = {(uint16_t)(gate[i].x & 0xffffu), (uint16_t)(gate[i].x >> 16),
+                                                 (uint16_t)(gate[i].y & 0xffffu), (uint16_t)(gate[i].y >> 16)};
+#pragma unroll
+                        for (int j = 0; j < 4; j++) h[j] = bf_pos(rr4[j]) ? f2bf(acc[i][j]) : (uint16_t)0;
+                    }
+                    *reinterpret_cast<uint2*>(a.out + base[i] * 16 + 4 * g) =
+                        make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+                }
+            }
+            if (more) stash(z + 3, nx4);
+            __syncthreads();
+        }
+    }
+}
+
 // Weight gradient dW[t][ci][co] = sum_v in[v + off(t)][ci] * gout[v][co] and
 // bias gradient sum_v gout[v][co], per wavefront over 32-voxel x segments,
 // into part[wave][27*256 + 16].  MFMA: A = input (16 ci x 32 voxels), B =
@@ -546,6 +717,22 @@ hipError_t launch_conv_tiled(int cin, int mode, const ConvArgs& a, int grid, hip
     else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_tile_kernel<16, 0>), g, b, 0, s, a);
     else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_tile_kernel<16, 1>), g, b, 0, s, a);
     else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_tile_kernel<16, 2>), g, b, 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int conv_columns(int nx, int ny, int nz, int zc)
+{
+    return ((nx + 15) / 16) * ((ny + kZmY - 1) / kZmY) * ((nz + zc - 1) / zc);
+}
+
+hipError_t launch_conv_zm(int cin, int mode, const ConvArgs& a, int grid, hipStream_t s)
+{
+    const dim3 g(grid), b(256);
+    if (cin == 8 && mode == 0) hipLaunchKernelGGL((conv_zm_kernel<8, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_zm_kernel<16, 0>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_zm_kernel<16, 1>), g, b, 0, s, a);
+    else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_zm_kernel<16, 2>), g, b, 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

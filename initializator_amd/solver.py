@@ -345,6 +345,12 @@ class Corrector:
         check(self.lib.nmr_hip_corrector_set_scales(self.h, float(vscale), float(out_scale)),
               "nmr_hip_corrector_set_scales")
 
+    def set_conv(self, kind: int = 0, zc: int = 0) -> None:
+        """Convolution kernel (0 z-marching columns, 1 tiles, 2 row gathers,
+        3 z-marching with the learned solve's last layer on pore rows only);
+        zc: output planes per z-marching column (0: 16).  Same bits for all."""
+        check(self.lib.nmr_hip_corrector_set_conv(self.h, int(kind), int(zc)), "nmr_hip_corrector_set_conv")
+
     def scales(self):
         v, o = ctypes.c_double(), ctypes.c_double()
         check(self.lib.nmr_hip_corrector_get_scales(self.h, ctypes.byref(v), ctypes.byref(o)),

@@ -74,6 +74,39 @@ def test_backward_against_emulating_oracle(shape, seed):
         assert e < 5e-3, name
 
 
+@pytest.mark.parametrize("shape,seed", [((7, 9, 10), 10), ((13, 11, 37), 11), ((3, 4, 5), 12), ((6, 2, 70), 13),
+                                        ((35, 19, 17), 14), ((20, 33, 48), 15)])
+def test_convolution_kernels_give_the_same_bits(shape, seed):
+    """z-marching columns (several column heights, including one plane and
+    more planes than the grid), staged tiles and row gathers feed the same
+    MFMA operands in the same order: forward outputs and parameter gradients
+    agree bit for bit."""
+    rng = np.random.default_rng(seed)
+    params = oc.random_params(seed).astype(np.float32)
+    feats = rng.normal(0.0, 1.0, shape + (6,)).astype(np.float32)
+    gout = rng.normal(0.0, 1.0, shape + (2,)).astype(np.float32)
+    res = {}
+    with Corrector() as net:
+        net.params = params
+        for kind, zc in [(0, 0), (0, 1), (0, 3), (0, 64), (1, 0), (2, 0), (3, 5)]:
+            net.set_conv(kind, zc)
+            res[(kind, zc)] = (net.forward(feats), net.backward(feats, gout))
+    ref_out, ref_g = res[(2, 0)]
+    assert np.all(np.isfinite(ref_out)) and np.all(np.isfinite(ref_g))
+    for k, (o, g) in res.items():
+        np.testing.assert_array_equal(o, ref_out, err_msg=str(k))
+        np.testing.assert_array_equal(g, ref_g, err_msg=str(k))
+
+
+def test_convolution_kernel_choice_is_checked():
+    from initializator_amd._lib import NmrError
+    with Corrector() as net:
+        for kind, zc in [(4, 0), (-1, 0), (0, -1), (0, 5000)]:
+            with pytest.raises(NmrError):
+                net.set_conv(kind, zc)
+        net.set_conv(0, 0)
+
+
 DX, D = 1e-6, 2.3e-9
 DT = generators.stable_dt(DX, D)
 
@@ -160,6 +193,30 @@ def test_fallback_reproduces_the_plain_solve():
     for a, b in zip(f, want_f):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_allclose(sig, want_sig, rtol=1e-12, atol=1e-12 * np.abs(want_sig).max())
+
+
+def test_learned_solve_same_bits_for_every_convolution_kernel():
+    """The learned solve with the last layer on every row (z-marching) or on
+    the rows that hold a pore only (tiles, z-marching sparse): corrections
+    are read at pores only, so fields and signals are the same bits."""
+    ncoarse, factor = 4, 8
+    sched = generators.pgse_schedule(ncoarse * factor, DT, 0.2, 2e-4, 6e-4)
+    out = {}
+    with Corrector(5) as net:
+        net.set_scales(1e-3, 1e-4)
+        for kind in (0, 1, 3):
+            net.set_conv(kind)
+            s, _, _ = _solver()
+            sig, nf = net.run_learned(s, ncoarse, factor, sched, tol=1e30, reasons=True)
+            out[kind] = (sig, nf, s.download_magnetization())
+            s.close()
+    sig0, nf0, f0 = out[0]
+    assert nf0[0] < ncoarse and np.all(np.isfinite(sig0))
+    for kind, (sig, nf, f) in out.items():
+        assert nf == nf0, kind
+        np.testing.assert_array_equal(sig, sig0, err_msg=str(kind))
+        for a, b in zip(f, f0):
+            np.testing.assert_array_equal(a, b, err_msg=str(kind))
 
 
 def test_oversized_corrections_fall_back():
