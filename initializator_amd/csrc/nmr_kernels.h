@@ -172,6 +172,9 @@ struct ConvArgs {
     int nrows;
     int nx, ny, nz, nseg;   // nseg = ceil(nx/16)
     int zc;                 // output planes per column of the z-marching kernel (0 = 16)
+    const float* smx;       // mode 4: input channels 0, 1 from this state (Mx, My times the pore
+    const float* smy;       //   channel), compact with pitch spitch
+    int spitch;
 };
 
 struct WgradArgs {

@@ -153,10 +153,12 @@ int conv_dense(nmr_corrector* c, int cin, int mode, nmr::ConvArgs a)
 }
 
 // Forward pass on the tensors (x already filled): a1, a2, p.  Layers 1 and
-// 2 run tiled over every row; with row lists (sparse = true) layer 3
-// computes only the rows c->rows[2] (those holding a pore), otherwise it
-// runs tiled too.
-int forward(nmr_corrector* c, bool sparse = false)
+// 2 run over every row; with row lists (sparse = true) layer 3 computes
+// only the rows c->rows[2] (those holding a pore) unless the z-marching
+// kernel runs it over every row.  With a state (smx, smy; z-marching kernels
+// only) layer 1 takes input channels 0 and 1 from it instead of from x.
+int forward(nmr_corrector* c, bool sparse = false, const float* smx = nullptr, const float* smy = nullptr,
+            int spitch = 0)
 {
     const Tensors& t = c->t;
     nmr::ConvArgs a{};
@@ -174,8 +176,17 @@ int forward(nmr_corrector* c, bool sparse = false)
     a.wa = c->f1;
     a.bias = c->w + kB1;
     a.out = t.a1;
-    int rc = conv_dense(c, 8, 0, a);
-    if (rc) return rc;
+    int rc = NMR_OK;
+    if (smx) {
+        a.smx = smx;
+        a.smy = smy;
+        a.spitch = spitch;
+        a.zc = c->zc;
+        HIP_TRY(nmr::launch_conv_zm(8, 4, a, zm_grid(t, 8, c->zc), c->stream));
+        a.smx = a.smy = nullptr;
+    } else if ((rc = conv_dense(c, 8, 0, a))) {
+        return rc;
+    }
     a.in = t.a1;
     a.wa = c->f2;
     a.bias = c->w + kB2;
@@ -727,6 +738,28 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
     int64_t nfall = 0, why[3] = {0, 0, 0};
     std::vector<double> sig8((size_t)(factor + 1) * 2);
     const float vinv = (float)(1.0 / c->vscale);
+    // the z-marching kernels stage (Mx, My) from the state: the feature
+    // tensor's other channels (pore, velocity) are written once per run
+    const bool state_fed = c->conv == NMR_CONV_ZMARCH || c->conv == NMR_CONV_ZMARCH_SPARSE;
+    auto features = [&](int buf) -> int {
+        nmr::FeatureArgs f{};
+        f.mx = ctx->m[buf][0];
+        f.my = ctx->m[buf][1];
+        f.mask = ctx->mask;
+        f.vx = ctx->vx;
+        f.vy = ctx->vy;
+        f.vz = ctx->vz;
+        f.out = c->t.x;
+        f.nx = p.nx;
+        f.ny = p.ny;
+        f.nz = p.nz;
+        f.pitch = ctx->pitch;
+        f.pitch1 = ctx->pitch1;
+        f.vinv = ctx->vel ? vinv : 0.f;
+        HIP_TRY(nmr::launch_features(f, ctx->stream));
+        return NMR_OK;
+    };
+    if (state_fed && ncoarse > 0 && (rc = features(ctx->cur))) return rc;
     for (int64_t k = 0; k < ncoarse; k++) {
         const int in = ctx->cur, out = in ^ 1;
         double gsum = 0.0;
@@ -740,25 +773,11 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
             }
         // coarse step into the other buffer
         if ((rc = nmri::sweep_step(ctx, coarse_coef(p, factor, &gsum), true, in, out))) return rc;
-        nmr::FeatureArgs f{};
-        f.mx = ctx->m[in][0];
-        f.my = ctx->m[in][1];
-        f.mask = ctx->mask;
-        f.vx = ctx->vx;
-        f.vy = ctx->vy;
-        f.vz = ctx->vz;
-        f.out = c->t.x;
-        f.nx = p.nx;
-        f.ny = p.ny;
-        f.nz = p.nz;
-        f.pitch = ctx->pitch;
-        f.pitch1 = ctx->pitch1;
-        f.vinv = ctx->vel ? vinv : 0.f;
-        HIP_TRY(nmr::launch_features(f, ctx->stream));
+        if (!state_fed && (rc = features(in))) return rc;
         // the net runs on the context's stream
         hipStream_t keep_stream = c->stream;
         c->stream = ctx->stream;
-        rc = forward(c, true);
+        rc = state_fed ? forward(c, true, ctx->m[in][0], ctx->m[in][1], ctx->pitch) : forward(c, true);
         c->stream = keep_stream;
         if (rc) return rc;
         double nxt[6];

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "nmr_kernels.h"
 
@@ -208,11 +209,20 @@ __global__ void __launch_bounds__(256) conv_tile_kernel(ConvArgs a)
 // the row kernel's (same operands, same chunk order), so the bits equal
 // those of conv_kernel and conv_tile_kernel.  Persistent: workgroup b takes
 // columns b, b + gridDim.x, ...; a column is a.zc output planes (the last
-// one what is left).
+// one what is left).  MODE 4 (8 input channels) is MODE 0 with the input's
+// channels 0 and 1 staged from the FP32 state a.smx/smy times the pore
+// channel, rounded as feature_kernel rounds them, so that the feature
+// tensor's other channels (pore, velocity), which do not change during a
+// run, are written once.
 constexpr int kZmY = 16;
 template <int N>
 struct Plane {
     uint4 v[N];
+};
+template <int N>
+struct PlaneS {
+    uint4 v[N];
+    float fx[N], fy[N];
 };
 template <int CIN, int MODE>
 __global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
@@ -247,8 +257,9 @@ __global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
         if (kz != (c * TPC) / 9) hi |= 1u << c;
     }
     const bool last = (NCH - 1) * TPC + tg < 27;
+    static_assert(MODE != 4 || CIN == 8, "state-fed input is the first layer's");
     float bias[4] = {0.f, 0.f, 0.f, 0.f};
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 4) {
 #pragma unroll
         for (int j = 0; j < 4; j++) bias[j] = a.bias[4 * g + j];
     } else if constexpr (MODE == 1) {
@@ -270,27 +281,48 @@ __global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
         // feed only outputs that are not stored); a thread's element past the
         // plane loads element 0 again and is not stored
         long goff[NLD];
+        long sidx[NLD]; // MODE 4: the state index of an interior voxel in its plane, else -1
 #pragma unroll
         for (int j = 0; j < NLD; j++) {
             const int i = threadIdx.x + 256 * j < PV ? threadIdx.x + 256 * j : 0;
             const int v = i / V16, pt = i % V16;
             const long gx = min(x0 + v % HX, a.nx + 1), gy = min(y0 + v / HX, a.ny + 1);
             goff[j] = (gy * px + gx) * CIN + pt * 8;
+            sidx[j] = (gx >= 1 && gx <= a.nx && gy >= 1 && gy <= a.ny) ? (gy - 1) * a.spitch + gx - 1 : -1;
         }
+        using PL = typename std::conditional<MODE == 4, PlaneS<NLD>, Plane<NLD>>::type;
+        const long splane = (long)a.ny * a.spitch;
         auto load = [&](int pz) {
-            Plane<NLD> r;
+            PL r;
 #pragma unroll
-            for (int j = 0; j < NLD; j++)
+            for (int j = 0; j < NLD; j++) {
                 r.v[j] = *reinterpret_cast<const uint4*>(a.in + (long)pz * pstride + goff[j]);
+                if constexpr (MODE == 4) {
+                    const bool in = sidx[j] >= 0 && pz >= 1 && pz <= a.nz;
+                    const long c = in ? (long)(pz - 1) * splane + sidx[j] : 0;
+                    r.fx[j] = in ? a.smx[c] : 0.f;
+                    r.fy[j] = in ? a.smy[c] : 0.f;
+                }
+            }
             return r;
         };
-        auto stash = [&](int pz, const Plane<NLD>& r) {
+        auto stash = [&](int pz, const PL& r) {
 #pragma unroll
             for (int j = 0; j < NLD; j++)
-                if (threadIdx.x + 256 * j < PV) sin[(pz & 3) * PV + threadIdx.x + 256 * j] = r.v[j];
+                if (threadIdx.x + 256 * j < PV) {
+                    uint4 w = r.v[j];
+                    if constexpr (MODE == 4) {
+                        if (sidx[j] >= 0 && pz >= 1 && pz <= a.nz) {
+                            // feature_kernel's channels 0, 1: (Mx, My) * pore
+                            const float pore = (w.y & 0xffffu) != 0u ? 1.f : 0.f;
+                            w.x = (uint32_t)f2bf(r.fx[j] * pore) | ((uint32_t)f2bf(r.fy[j] * pore) << 16);
+                        }
+                    }
+                    sin[(pz & 3) * PV + threadIdx.x + 256 * j] = w;
+                }
         };
         {
-            const Plane<NLD> r0 = load(zs), r1 = load(zs + 1), r2v = load(zs + 2);
+            const PL r0 = load(zs), r1 = load(zs + 1), r2v = load(zs + 2);
             stash(zs, r0);
             stash(zs + 1, r1);
             stash(zs + 2, r2v);
@@ -313,7 +345,7 @@ __global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
                 for (int i = 0; i < 4; i++) gate[i] = *reinterpret_cast<const uint2*>(a.ref + base[i] * 16 + 4 * g);
             }
             const bool more = z + 3 <= ze + 1;
-            Plane<NLD> nx4{};
+            PL nx4{};
             if (more) nx4 = load(z + 3);
             int sb[3];
 #pragma unroll
@@ -346,7 +378,7 @@ __global__ void __launch_bounds__(256, 3) conv_zm_kernel(ConvArgs a)
                     }
                 } else {
                     uint16_t h[4];
-                    if constexpr (MODE == 0) {
+                    if constexpr (MODE == 0 || MODE == 4) {
 #pragma unroll
                         for (int j = 0; j < 4; j++) h[j] = f2bf(fmaxf(acc[i][j] + bias[j], 0.f));
                     } else {
@@ -730,6 +762,7 @@ hipError_t launch_conv_zm(int cin, int mode, const ConvArgs& a, int grid, hipStr
 {
     const dim3 g(grid), b(256);
     if (cin == 8 && mode == 0) hipLaunchKernelGGL((conv_zm_kernel<8, 0>), g, b, 0, s, a);
+    else if (cin == 8 && mode == 4) hipLaunchKernelGGL((conv_zm_kernel<8, 4>), g, b, 0, s, a);
     else if (cin == 16 && mode == 0) hipLaunchKernelGGL((conv_zm_kernel<16, 0>), g, b, 0, s, a);
     else if (cin == 16 && mode == 1) hipLaunchKernelGGL((conv_zm_kernel<16, 1>), g, b, 0, s, a);
     else if (cin == 16 && mode == 2) hipLaunchKernelGGL((conv_zm_kernel<16, 2>), g, b, 0, s, a);
