@@ -597,7 +597,7 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     c->pitch1 = round_up4(p.nx + 1);
     c->n_m = (size_t)p.nz * p.ny * c->pitch;
     // the sweep addresses every array with 32-bit byte offsets
-    const size_t biggest = std::max((size_t)(p.nz + 1) * p.ny * c->pitch, (size_t)p.nz * p.ny * c->pitch1) * 4;
+    const size_t biggest = nmri::biggest_array_bytes(p.nx, p.ny, p.nz);
     if (biggest >= ((size_t)1 << 32) - 64) {
         delete c;
         return fail(NMR_ERR_UNSUPPORTED, "arrays of %zu bytes exceed the 4 GiB addressing of the sweep", biggest);
@@ -1366,6 +1366,9 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
         const nmr_batch_instance& s = inst[i];
         if (s.nx < 1 || s.ny < 1 || s.nz < 1 || (long)s.nx * s.ny * s.nz > (1L << 30))
             return fail(NMR_ERR_INVALID, "instance %d grid %d x %d x %d", i, s.nx, s.ny, s.nz);
+        // the batched sweep addresses every array with 32-bit byte offsets
+        if (nmri::biggest_array_bytes(s.nx, s.ny, s.nz) >= ((size_t)1 << 32) - 64)
+            return fail(NMR_ERR_UNSUPPORTED, "instance %d: arrays above the 4 GiB addressing of the sweep", i);
         if (!(s.delta >= 0) || !(s.big_delta >= 0) || !std::isfinite(s.grad_amp))
             return fail(NMR_ERR_INVALID, "instance %d gradient schedule", i);
         for (int a = 0; a < 3; a++)

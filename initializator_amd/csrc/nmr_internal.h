@@ -21,6 +21,15 @@ namespace nmri {
 // Record a status detail for nmr_hip_last_error() and return code.
 int fail(int code, const char* fmt, ...);
 inline int round_up4(int n) { return (n + 3) & ~3; }
+// Bytes of the largest FP32 array of an nx*ny*nz grid in the row layout: the
+// state (rows of round_up4(nx) floats) and the face arrays Vx (rows of
+// round_up4(nx + 1)), Vy (ny + 1 rows per plane) and Vz (nz + 1 planes).
+inline size_t biggest_array_bytes(int nx, int ny, int nz)
+{
+    const size_t p = (size_t)round_up4(nx), p1 = (size_t)round_up4(nx + 1);
+    const size_t vx = p1 * ny * nz, vy = p * (ny + 1) * nz, vz = p * ny * (nz + 1);
+    return 4 * std::max(vx, std::max(vy, vz));
+}
 } // namespace nmri
 
 #define HIP_TRY(expr)                                                                          \

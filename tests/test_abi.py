@@ -104,6 +104,10 @@ def _problem(**kw):
     (dict(grad=(0.1, 0.0, 0.0)), _lib.NMR_ERR_UNSUPPORTED),
     (dict(phase_mode=_lib.NMR_PHASE_EXACT, ncomp=3), _lib.NMR_ERR_UNSUPPORTED),
     (dict(nx=4096, ny=4096, nz=4096), _lib.NMR_ERR_INVALID),
+    # below 2^31 voxels, but a face array (Vx; Vy alone) passes the 4 GiB
+    # that the sweep's 32-bit byte offsets address
+    (dict(nx=1021, ny=1024, nz=1024), _lib.NMR_ERR_UNSUPPORTED),
+    (dict(nx=1023, ny=1, nz=600000), _lib.NMR_ERR_UNSUPPORTED),
     (dict(bc=(0, 3, 0)), _lib.NMR_ERR_INVALID),
     (dict(scheme=_lib.NMR_SCHEME_MACCORMACK, phase_mode=_lib.NMR_PHASE_EXACT), _lib.NMR_ERR_UNSUPPORTED),
     (dict(bc=(1, 0, 0), phase_mode=_lib.NMR_PHASE_EXACT), _lib.NMR_ERR_UNSUPPORTED),
@@ -155,6 +159,7 @@ def _inst(n, **kw):
     (dict(), dict(), 0, _lib.NMR_ERR_INVALID),
     (dict(), dict(), 70000, _lib.NMR_ERR_INVALID),
     (dict(), dict(nx=0), 3, _lib.NMR_ERR_INVALID),
+    (dict(), dict(nx=1023, ny=1, nz=600000), 2, _lib.NMR_ERR_UNSUPPORTED),
     (dict(), dict(delta=-1.0), 2, _lib.NMR_ERR_INVALID),
     (dict(scheme=3), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
     (dict(ncomp=1, flags=_lib.NMR_F_PRECESSION), dict(), 2, _lib.NMR_ERR_UNSUPPORTED),
