@@ -1589,12 +1589,18 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             }
         }
     };
-    // ghost value past nx for the tail of a chunk and LDS column c+4
+    // ghost value past nx for the tail of a chunk and LDS column c+4: the
+    // edge element xlast (zero derivative), the periodic value or 0.  Written
+    // as value selects only: the same choice as nested conditionals with
+    // branches, which hipcc once compiled into a selection that stored
+    // element 1 in place of element 3 when nx is a multiple of 4.
     auto right_ghost = [&](const f4 (&m)[NC], const float (&gv)[NC], int q) -> float {
-        if (bcx == BC_PERIODIC) return gv[q];
-        if (bcx == BC_DIRICHLET) return 0.f;
         const int xl = xlast < 3 ? xlast : 3;
-        return xl == 0 ? m[q].x : (xl == 1 ? m[q].y : (xl == 2 ? m[q].z : m[q].w));
+        const float e01 = xl == 0 ? m[q].x : m[q].y;
+        const float e23 = xl == 2 ? m[q].z : m[q].w;
+        const float edge = xl < 2 ? e01 : e23;
+        const float other = bcx == BC_PERIODIC ? gv[q] : 0.f;
+        return bcx == BC_NEUMANN ? edge : other;
     };
     // Ghost fix of the centre plane's registers: the element past nx takes
     // the x ghost value, its mask and diffusivity those of element xlast.

@@ -77,7 +77,7 @@ MAK = 1
 PER, DIR = 1, 2
 
 
-@pytest.mark.parametrize("shape", [(17, 14, 90), (9, 6, 261), (5, 7, 3)])
+@pytest.mark.parametrize("shape", [(17, 14, 90), (9, 6, 261), (5, 7, 3), (6, 9, 64)])
 @pytest.mark.parametrize("scheme,bc", [(MAK, (0, 0, 0)), (MAK, (PER, PER, PER)), (MAK, (DIR, PER, 0)),
                                        (0, (PER, DIR, PER))])
 def test_brick_general_sweep_equals_row_and_per_voxel(shape, scheme, bc):

@@ -2,7 +2,8 @@
 per-voxel coefficients) against the per-voxel kernel (tile variant -2), which
 computes every voxel with the same operations: fields bit for bit, signal
 rows to FP64 summation order.  Shapes cover partial x, y and z tiles, more
-than one x tile (nx > 256) and chunks that reach past nx."""
+than one x tile (nx > 256), chunks that reach past nx and an nx that is a
+multiple of 4 (the last chunk full, its ghost in the next LDS column)."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 MAK, UPW = _lib.NMR_SCHEME_MACCORMACK, _lib.NMR_SCHEME_UPWIND
 NEU, PER, DIR = _lib.NMR_BC_NEUMANN, _lib.NMR_BC_PERIODIC, _lib.NMR_BC_DIRICHLET
 
-SHAPES = [(13, 17, 21), (9, 6, 261), (7, 5, 3)]
+SHAPES = [(13, 17, 21), (9, 6, 261), (7, 5, 3), (6, 9, 12)]
 BCS = [(NEU, NEU, NEU), (PER, PER, PER), (DIR, DIR, DIR), (PER, DIR, NEU), (DIR, PER, PER)]
 
 
@@ -154,3 +155,28 @@ def test_masked_face_advection(bc):
     assert e_sig <= 1e-5 and e_vox <= 1e-4
     plain = _pair(case, m0, UPW, bc)[0]
     assert np.abs(plain[0] - sig).max() > 1e-6
+
+
+@pytest.mark.parametrize("nx", [12, 13, 64])
+@pytest.mark.parametrize("nsteps", [1, 20])
+def test_maccormack_keeps_uniform_equilibrium(nx, nsteps):
+    """Mz = p_eq at every pore, Mx = My = 0, with flow, diffusion and both
+    relaxations: every difference, rate and rotation term is zero, so the
+    predictor and the corrector return the field unchanged (to the FP32
+    value of p_eq) -- in the row layout (1 step) and the brick layout (20
+    steps), whatever nx is modulo 4."""
+    shape = (7, 9, nx)
+    case = _case(shape, nsteps, seed=nx)
+    peq = 0.55
+    s = HipSolver(shape, 3, case.dx, case.dt, case.diffusivity, 1.0 / case.t2_bulk, r1_bulk=20.0, rho1=1e-5,
+                  rho2=2e-5, p_eq=peq, velocity=True, precession=True, scheme=MAK)
+    s.upload_mask(case.mask)
+    s.upload_velocity(case.vx, case.vy, case.vz)
+    pore = case.mask != 0
+    s.upload_magnetization(np.zeros(shape), np.zeros(shape), peq * pore)
+    s.run(nsteps)
+    f = s.download_magnetization()
+    s.close()
+    assert np.all(f[0] == 0.0) and np.all(f[1] == 0.0)
+    np.testing.assert_array_equal(f[2][pore], np.float32(peq))
+    assert np.all(f[2][~pore] == 0.0)
