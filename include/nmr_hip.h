@@ -172,12 +172,13 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out);
  * the kernels of the last nmr_hip_run, and the number of sweep launches. */
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
-/* Device layout of the one-step sweep (no reference counterpart): 1 (the
- * default) keeps a copy of the state and the velocity in 4x4x2 bricks of
- * 128 bytes for runs of 16 steps or more with tile variants 0, 2, 4, 6 --
- * all-solid bricks are then never fetched -- converting the state in and
- * out once per run; 0 always sweeps the row layout.  The fields are the
- * same bits either way. */
+/* Device layout of the sweeps (no reference counterpart): 1 (the default)
+ * keeps a copy of the state and the velocity in bricks of 128 bytes (8
+ * x-voxels by 4 rows of one plane) for runs of 16 steps or more -- the
+ * one-step sweep with any one-step tile variant, the general sweep with
+ * uniform coefficients and plain sums -- so that all-solid bricks are never
+ * fetched, converting the state in and out once per run; 0 always sweeps
+ * the row layout.  The fields are the same bits either way. */
 int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
 /* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = a
  * one-thread-per-voxel kernel with the arithmetic of the sweeps, a

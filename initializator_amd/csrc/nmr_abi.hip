@@ -1009,7 +1009,7 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 
     int w[6];
     signal_window(p, w);
-    // the one-step sweep runs on 4x4x2 bricks when it is long enough to pay
+    // the one-step sweep runs on 8x4 bricks when it is long enough to pay
     // for converting the state in and out (the velocity is converted once)
     const bool brick = want_brick && L1.steps == 1 && nmr::brick_variant(L1.variant);
     if (brick && (rc = ensure_bricks(ctx))) return rc;

@@ -60,7 +60,7 @@ struct nmr_ctx {
     double d_ref = 0.0;
     bool hetero = false, weighted = false;
     bool solids_zeroed = false; // MacCormack: both state buffers hold 0 at solids
-    // the plain sweep in 4x4x2 bricks: state copies and velocity in bricks
+    // the sweeps in 128-byte bricks (8 x-voxels by 4 rows): state copies and velocity
     bool brick_on = true;       // nmr_hip_set_layout
     float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;

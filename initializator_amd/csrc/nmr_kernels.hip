@@ -1874,7 +1874,7 @@ hipError_t launch_brick(const float* src, float* dst, int nbx, int Y, int Z, boo
     return hipGetLastError();
 }
 
-bool brick_variant(int v) { return v == 0 || v == 2 || v == 4 || v == 6; }
+bool brick_variant(int v) { return v >= 0 && v <= 6; }
 int brick_auto_variant(int nx) { return nx > 32 ? 4 : 6; }
 
 template <int NC, bool VEL, bool EXACT>
@@ -1883,8 +1883,11 @@ static hipError_t launch_nc_brick(int v, const SweepArgs& a, hipStream_t s)
     const dim3 grid(a.nblocks), block(256);
     switch (v) {
     case 0: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 64, 1, true>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 64, 2, true>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 32, 1, true>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 32, 2, true>), grid, block, 0, s, a); break;
     case 4: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 16, 1, true>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 16, 2, true>), grid, block, 0, s, a); break;
     case 6: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 8, 1, true>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }

@@ -134,7 +134,7 @@ class HipSolver:
         check(self.lib.nmr_hip_set_tile(self.ctx, int(variant), int(lz)), "nmr_hip_set_tile")
 
     def set_layout(self, brick: bool = True) -> None:
-        """Sweep the state in 4x4x2 bricks (default) or in the row layout."""
+        """Sweep the state in 128-byte bricks of 8 x 4 voxels (default) or in the row layout."""
         check(self.lib.nmr_hip_set_layout(self.ctx, int(bool(brick))), "nmr_hip_set_layout")
 
     def run(self, nsteps: int, grad_sched: np.ndarray | None = None, want_signal: bool = True):

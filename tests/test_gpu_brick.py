@@ -1,4 +1,4 @@
-"""The one-step sweep on 4x4x2 bricks (nmr_hip_set_layout, the default for
+"""The one-step sweep on bricks of 8 x-voxels by 4 rows (nmr_hip_set_layout, the default for
 runs of 16 steps or more) against the row layout: the same operations on
 the same values, so the fields must agree bit for bit and the per-step
 signal to FP64 summation order; shapes cover x, y and z extents that are
@@ -36,7 +36,8 @@ def _same(name, a, b):
 
 
 @pytest.mark.parametrize("shape,variant", [((23, 37, 45), -1), ((9, 6, 261), 0), ((17, 14, 90), 2),
-                                           ((13, 21, 50), 4), ((11, 33, 30), 6), ((5, 3, 7), -1)])
+                                           ((13, 21, 50), 4), ((11, 33, 30), 6), ((5, 3, 7), -1),
+                                           ((9, 19, 261), 1), ((12, 37, 90), 3), ((13, 70, 48), 5)])
 def test_brick_sweep_equals_row_sweep(shape, variant):
     case = generators.case_flow(seed=sum(shape), shape=shape, phi=0.35, nsteps=40, big_delta=6e-4, delta=2e-4)
     rng = np.random.default_rng(1)
