@@ -137,6 +137,15 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
  * rho1/rho2 of the problem from then on; diffusion between two pores uses
  * the harmonic mean 2 Da Db / (Da + Db) (cpp:3248-3259).  Linear phase only. */
 int nmr_hip_upload_coefficients(nmr_ctx* ctx, const double* D, const double* r1, const double* r2, int32_t layout);
+
+/* Per-voxel field b (T) along the main-field axis (the main field must lie
+ * along one axis), in place of the constant-gradient field on that axis:
+ * the reference's noise field, noiseField.<axis> = AMPL_NOISE * (rand() %
+ * 2000 + 1 - 1000) / 1000 on every cell when AMPL_NOISE and NDIRECT_FIELD
+ * are set (FillInMainField, src/NmrRelaxationSolve.cpp:1973-1978), which
+ * ExplicitSimple turns into precession (cpp:3218-3235).  NULL removes it.
+ * Runs in the general z-sweep; linear phase only. */
+int nmr_hip_upload_field(nmr_ctx* ctx, const double* b, int32_t layout);
 /* Phase weights of the two-phase outputs (OutputX/Y/Z with NCOMP=2,
  * cpp:3324-3350; w = c * [c1 > 0.8 or c2 > 0.8]).  With weights uploaded
  * every signal row holds 2*ncomp sums: sum(w1*M_q) for q < ncomp, then

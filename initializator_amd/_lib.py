@@ -54,6 +54,7 @@ EXPORTS = (
     "nmr_hip_upload_magnetization",
     "nmr_hip_download_magnetization",
     "nmr_hip_upload_coefficients",
+    "nmr_hip_upload_field",
     "nmr_hip_upload_phase_weights",
     "nmr_hip_run",
     "nmr_hip_signal",
@@ -191,6 +192,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_upload_magnetization": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_download_magnetization": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_upload_coefficients": (c_int, [vp, dp, dp, dp, c_int32]),
+        "nmr_hip_upload_field": (c_int, [vp, dp, c_int32]),
         "nmr_hip_upload_phase_weights": (c_int, [vp, dp, dp, c_int32]),
         "nmr_hip_run": (c_int, [vp, c_int64, dp, dp]),
         "nmr_hip_signal": (c_int, [vp, dp]),

@@ -67,7 +67,7 @@ void free_all(nmr_ctx* c)
             if (c->m[b][q]) (void)hipFree(c->m[b][q]);
     for (int q = 0; q < 3; q++)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
-    for (float* q : {c->dn, c->r1v, c->r2v, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
+    for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
         if (q) (void)hipFree(q);
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < 3; q++)
@@ -169,6 +169,7 @@ nmr::Coef make_coef(const nmr_ctx* c, const double g[3])
         k.bgx = (float)(b * g[0] * p.dx);
         k.bgy = (float)(b * g[1] * p.dy);
         k.bgz = (float)(b * g[2] * p.dz);
+        k.dtg = (float)b;
     }
     return k;
 }
@@ -291,6 +292,8 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.dn = ctx->dn;
         a.r1v = ctx->r1v;
         a.r2v = ctx->r2v;
+        a.bn = ctx->bn;
+        a.bnax = ctx->bnax;
         for (int64_t s = 0; s < nsteps; s++) {
             a.c = coef_of(s);
             for (int q = 0; q < 3; q++) {
@@ -317,8 +320,8 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // state in both brick buffers -- the output one from the row buffer
         // whose solids were just cleared -- and, for MacCormack, a brick
         // predictor
-        const bool gbrick = ctx->brick_on && !ctx->hetero && !ctx->weighted && nsteps >= kBrickMinSteps &&
-                            bricks_fit(ctx);
+        const bool gbrick = ctx->brick_on && !ctx->hetero && !ctx->weighted && !ctx->bn &&
+                            nsteps >= kBrickMinSteps && bricks_fit(ctx);
         const int c0 = ctx->cur;
         const int nc4 = ctx->pitch / 4;
         if (gbrick) {
@@ -356,6 +359,8 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.dn = ctx->dn;
         a.r1v = ctx->r1v;
         a.r2v = ctx->r2v;
+        a.bn = ctx->bn;
+        a.bnax = ctx->bnax;
         a.nx = p.nx;
         a.ny = p.ny;
         a.nz = p.nz;
@@ -822,6 +827,36 @@ int nmr_hip_upload_coefficients(nmr_ctx* ctx, const double* D, const double* r1,
     if (r1 && (rc = put_voxels(ctx, &ctx->r1v, a, dt))) return rc;
     ctx->d_ref = dref;
     ctx->hetero = true;
+    ctx->general = true;
+    return NMR_OK;
+}
+
+int nmr_hip_upload_field(nmr_ctx* ctx, const double* b, int32_t layout)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_upload_field: null context");
+    int rc = check_layout(layout);
+    if (rc) return rc;
+    if (!b) { // back to the gradient field
+        if (ctx->bn) (void)hipFree(ctx->bn);
+        ctx->bn = nullptr;
+        ctx->bnax = -1;
+        return NMR_OK;
+    }
+    if (ctx->exact) return fail(NMR_ERR_UNSUPPORTED, "per-voxel fields use the linear phase");
+    const nmr_problem& p = ctx->prob;
+    int ax = -1, naxes = 0;
+    for (int q = 0; q < 3; q++)
+        if (p.b0_dir[q] != 0.0) {
+            ax = q;
+            naxes += 1;
+        }
+    if (naxes != 1) return fail(NMR_ERR_UNSUPPORTED, "a per-voxel field needs the main field along one axis");
+    std::vector<double> v;
+    if ((rc = read_voxels(ctx, b, layout, v))) return rc;
+    for (double x : v)
+        if (!std::isfinite(x)) return fail(NMR_ERR_INVALID, "per-voxel field values must be finite");
+    if ((rc = put_voxels(ctx, &ctx->bn, v, 1.0))) return rc;
+    ctx->bnax = ax;
     ctx->general = true;
     return NMR_OK;
 }

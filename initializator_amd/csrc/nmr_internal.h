@@ -56,6 +56,8 @@ struct nmr_ctx {
     bool general = false; // MacCormack, non-zero-derivative boundaries or per-voxel data: per-voxel kernel
     // per-voxel coefficients (diffusivity / d_ref, dt/T1, dt/T2) and phase weights
     float *dn = nullptr, *r1v = nullptr, *r2v = nullptr;
+    float* bn = nullptr; // per-voxel field along the main-field axis bnax (nmr_hip_upload_field)
+    int bnax = -1;
     float* w[2] = {nullptr, nullptr};
     double d_ref = 0.0;
     bool hetero = false, weighted = false;

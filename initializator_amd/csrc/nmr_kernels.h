@@ -17,6 +17,7 @@ struct Coef {
     float peq;              // equilibrium magnetization
     float n0, n1, n2;       // unit main-field direction
     float bgx, bgy, bgz;    // dt*g*G_a*d_a: precession angle per index unit
+    float dtg;              // dt*g (0 without precession): angle per tesla of a per-voxel field
 };
 
 struct SweepArgs {
@@ -109,6 +110,8 @@ struct GenArgs {
     const float* dn;        // per-voxel diffusivity / reference (null: uniform, from the mask)
     const float* r1v;       // per-voxel dt/T1 with surface terms (null: from the mask)
     const float* r2v;       // per-voxel dt/T2 with surface terms
+    const float* bn;        // per-voxel field (T) along axis bnax, in place of that axis's gradient field; or null
+    int bnax;
     Coef c;
 };
 
@@ -128,6 +131,8 @@ struct GSweepArgs {
     const float* dn;        // per-voxel diffusivity / reference (coefficient variant)
     const float* r1v;       // per-voxel dt/T1
     const float* r2v;       // per-voxel dt/T2
+    const float* bn;        // per-voxel field (T) along axis bnax, in place of that axis's gradient field; or null
+    int bnax;
     double* partials;       // [nblocks][nc] pore sums of the result, or null
     int nx, ny, nz, pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;

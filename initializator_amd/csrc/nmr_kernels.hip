@@ -315,7 +315,12 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
         t = fmaf(w[4], f[4] - m, t);
         d[q] = fmaf(w[5], f[5] - m, t);
     }
-    const float thy = cf.bgy * ((float)y + 0.5f), thz = cf.bgz * ((float)z + 0.5f);
+    // precession angles: the gradient field's, or along axis bnax the
+    // per-voxel field's (value selects, as in gupdate_chunk)
+    const int bnax = a.bn ? a.bnax : -1;
+    const float tb = a.bn ? cf.dtg * a.bn[c] : 0.f;
+    const float thy = bnax == 1 ? tb : cf.bgy * ((float)y + 0.5f);
+    const float thz = bnax == 2 ? tb : cf.bgz * ((float)z + 0.5f);
     if constexpr (NC == 3) {
         const float rate1 = a.r1v ? a.r1v[c] : fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
         const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
@@ -326,7 +331,7 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
             d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
             d[q] = fmaf(rate2, mpar - m0[q], d[q]);
         }
-        const float thx = cf.bgx * ((float)x + 0.5f);
+        const float thx = bnax == 0 ? tb : cf.bgx * ((float)x + 0.5f);
         d[0] = fmaf(thz, m0[1], fmaf(-thy, m0[2], d[0]));
         d[1] = fmaf(-thz, m0[0], fmaf(thx, m0[2], d[1]));
         d[2] = fmaf(thy, m0[0], fmaf(-thx, m0[1], d[2]));
@@ -1280,7 +1285,7 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
                                               float dxl, float dxr, const f4& dym, const f4& dyp, const f4& dzm,
                                               const f4& dzp, bool r1p, const f4& r1c, const f4& r2c,
                                               const f4 (&mav)[NC], bool amask, int ix, float thy, float thz,
-                                              f4 (&out)[NC])
+                                              const f4& bnc, int bnax, float dtg, f4 (&out)[NC])
 {
     const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
     const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
@@ -1329,6 +1334,10 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
         float rate2;
         if constexpr (COEF) rate2 = r2c[e];
         else rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+        // precession angles: the gradient field's, or along axis bnax the
+        // per-voxel field's (value selects, as in general_kernel)
+        const float tb = bnax >= 0 ? dtg * bnc[e] : 0.f;
+        const float ty = bnax == 1 ? tb : thy, tz = bnax == 2 ? tb : thz;
         float m0[NC], d[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++) {
@@ -1355,16 +1364,16 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
                 d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
                 d[q] = fmaf(rate2, mpar - m0[q], d[q]);
             }
-            const float thx = cf.bgx * ((float)(ix + e) + 0.5f);
-            d[0] = fmaf(thz, m0[1], fmaf(-thy, m0[2], d[0]));
-            d[1] = fmaf(-thz, m0[0], fmaf(thx, m0[2], d[1]));
-            d[2] = fmaf(thy, m0[0], fmaf(-thx, m0[1], d[2]));
+            const float tx = bnax == 0 ? tb : cf.bgx * ((float)(ix + e) + 0.5f);
+            d[0] = fmaf(tz, m0[1], fmaf(-ty, m0[2], d[0]));
+            d[1] = fmaf(-tz, m0[0], fmaf(tx, m0[2], d[1]));
+            d[2] = fmaf(ty, m0[0], fmaf(-tx, m0[1], d[2]));
         } else {
 #pragma unroll
             for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
             if constexpr (NC == 2) {
-                d[0] = fmaf(thz, m0[1], d[0]);
-                d[1] = fmaf(-thz, m0[0], d[1]);
+                d[0] = fmaf(tz, m0[1], d[0]);
+                d[1] = fmaf(-tz, m0[0], d[1]);
             }
         }
         const bool pore = ((kc >> (8 * e)) & 0xffu) != 0;
@@ -1616,8 +1625,9 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         }
     };
 
-    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, wc = vx4, wn = vx4, r1c = vx4, r2c = vx4;
+    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, wc = vx4, wn = vx4, r1c = vx4, r2c = vx4, bnc = vx4;
     float vxe = 0.f;
+    const int bnax = a.bn ? a.bnax : -1;
     f4 mav[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) mav[q] = f4{0.f, 0.f, 0.f, 0.f};
@@ -1641,6 +1651,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                     if (a.r1v) r1c = ld4_nt(a.r1v, pc);
                 }
             }
+            if (bnax >= 0) bnc = ld4_nt(a.bn, (uint32_t)kk * plane_b + rowc); // row layout, as the coefficients
             if constexpr (ST == GEN_CORRECT) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) mav[q] = ld4_nt(a.mavg[q], pz + rowf);
@@ -1748,7 +1759,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             gupdate_chunk<NC, VEL, ST, COEF>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, vx4,
                                              vxe, vy4, vy4n, wc, wn, dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
                                              r2c, mav, a.amask != 0,
-                                             ix, thy, thz, out);
+                                             ix, thy, thz, bnc, bnax, cf.dtg, out);
             if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
             const uint32_t of = zo(k) + rowf;
 #pragma unroll

@@ -5,6 +5,7 @@
 #include "NmrRelaxationSolve.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -27,6 +28,39 @@ double rmin(double a, double b) { return a <= b ? a : b; } // cpp:3289-3293
     std::cerr << "NmrRelaxationSolve: " << what << std::endl;
     std::exit(1);
 }
+
+// The C library's rand() as the reference calls it (glibc, never seeded:
+// the state of srand(1)), restated so that the noise field does not depend
+// on other rand() users in the process: glibc's TYPE_3 additive-feedback
+// generator, r[0] = 1, r[i] = 16807 r[i-1] mod (2^31 - 1) for i < 31,
+// r[i] = r[i-31] for i = 31..33, then r[i] = r[i-31] + r[i-3] mod 2^32, the
+// first 310 of those discarded; each call returns r >> 1.
+class GlibcRand {
+public:
+    GlibcRand()
+    {
+        int64_t w = 1;
+        r_[0] = 1u;
+        for (int i = 1; i < 31; i++) {
+            w = (16807LL * w) % 2147483647LL;
+            r_[i] = (uint32_t)w;
+        }
+        for (int i = 31; i < 34; i++) r_[i] = r_[i - 31];
+        for (int k = 0; k < 310; k++) next();
+    }
+    int operator()() { return (int)(next() >> 1); }
+
+private:
+    uint32_t next()
+    {
+        const uint32_t v = r_[(i_ - 31) % 34] + r_[(i_ - 3) % 34];
+        r_[i_ % 34] = v;
+        i_ += 1;
+        return v;
+    }
+    uint32_t r_[34];
+    long i_ = 34;
+};
 
 } // namespace
 
@@ -196,12 +230,11 @@ static void boundary_modes(const int nb[6], int out[3])
 }
 
 // Inputs the GPU path implements (one or two fluids, any relaxivity files,
-// no random noise field, boundary flags that define every ghost).
+// the random noise field, boundary flags that define every ghost).
 void NmrRelaxationSolve::Validate() const
 {
     if (nComponents != 1 && nComponents != 2) unsupported("NCOMP must be 1 or 2");
     if (nComponents == 1 && nFluid != 1 && nFluid != 2) unsupported("NFLUID must be 1 or 2");
-    if (noiseAmpl != 0.0 && nDirectionField != 0) unsupported("the random noise field (AMPL_NOISE) is not implemented");
     int modes[3];
     boundary_modes(nBoundary, modes);
     for (int a = 0; a < 3; a++)
@@ -552,6 +585,16 @@ void NmrRelaxationSolve::create_context(int scheme)
     if (perVoxel)
         check(nmr_hip_upload_coefficients(ctx_, dvox.data(), r1vox.data(), r2vox.data(), NMR_LAYOUT_PADDED),
               "nmr_hip_upload_coefficients");
+    if (noiseAmpl != 0.0 && nDirectionField != 0) {
+        // FillInMainField's noise (cpp:1973-1978): one rand() per padded cell,
+        // k outermost, on the main-field axis, in place of its gradient field
+        GlibcRand rnd;
+        std::vector<double> nf(actCell.size());
+        for (int k = 0; k < kk + 2; k++)
+            for (int j = 0; j < jk + 2; j++)
+                for (int i = 0; i < ik + 2; i++) nf[index(i, j, k)] = noiseAmpl * (rnd() % 2000 + 1 - 1000) / 1000;
+        check(nmr_hip_upload_field(ctx_, nf.data(), NMR_LAYOUT_PADDED), "nmr_hip_upload_field");
+    }
     if (nComponents == 2) { // two-phase outputs: weights c1, c2 on cells with c1 or c2 above 0.8
         std::vector<double> w1(actCell.size(), 0.0), w2(actCell.size(), 0.0);
         for (size_t c = 0; c < actCell.size(); c++) {

@@ -114,6 +114,12 @@ class HipSolver:
         check(self.lib.nmr_hip_upload_coefficients(self.ctx, *[_dptr(a) for a in arrs], layout),
               "nmr_hip_upload_coefficients")
 
+    def upload_field(self, b, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
+        """Per-voxel field (T) along the main-field axis in place of that axis's
+        gradient field (the reference's noise field); None removes it."""
+        arr = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        check(self.lib.nmr_hip_upload_field(self.ctx, _dptr(arr), layout), "nmr_hip_upload_field")
+
     def upload_phase_weights(self, w1, w2, layout: int = _lib.NMR_LAYOUT_COMPACT) -> None:
         """Two-phase output weights; signal rows then hold 2*ncomp sums."""
         a, b = (np.ascontiguousarray(x, dtype=np.float64) for x in (w1, w2))

@@ -97,6 +97,10 @@ def lib():
         L.or_fill_coefficients.argtypes = [P, cp, c_double, c_double, c_double, c_double, c_double,
                                            dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_fill_gradient.argtypes = [P, c_double, c_double, c_double, dp, dp, dp]
+        L.or_glibc_rand.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.or_glibc_rand.restype = None
+        L.or_fill_noise.argtypes = [P, c_double, dp]
+        L.or_fill_noise.restype = None
         L.or_step_time.restype = c_double
         L.or_step_time.argtypes = [c_double] * 7
         L.or_explicit_simple.argtypes = [P, cp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
@@ -204,6 +208,18 @@ def fill_gradient(p, g):
     nf = [np.zeros(shp) for _ in range(3)]
     lib().or_fill_gradient(ctypes.byref(p), g[0], g[1], g[2], *[_d(a) for a in nf])
     return nf
+
+
+def glibc_rand(n):
+    """The first n outputs of glibc's rand() from its default state."""
+    out = np.empty(n, dtype=np.int32)
+    lib().or_glibc_rand(int(n), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out
+
+
+def fill_noise(p, ampl, nf_axis):
+    """Overwrite one padded field component with the reference's noise field."""
+    lib().or_fill_noise(ctypes.byref(p), float(ampl), _d(nf_axis))
 
 
 def step_time(dt_noise, dt_grad, dt_vel, dt_diff, dt_rel, time1, time2):
@@ -465,6 +481,8 @@ def run_dropin(cfg, mask, vel_raw, gpu=True, scheme=None, n1=None, surf=None):
     pad_pore = act == ord("1")
     m = [np.where(pad_pore, peq * d[q], 0.0) for q in range(3)]
     nf = fill_gradient(p, grads if cfg["NVECTOR"] == 1 else (0.0, 0.0, 0.0))
+    if noise != 0.0 and ndf != 0:  # FillInMainField's noise replaces that axis's gradient field
+        fill_noise(p, noise, nf[ndf - 1])
     a = [x.copy() for x in m]
     pc = [np.zeros(padded_shape(p)) for _ in range(6)]
     for x in m:

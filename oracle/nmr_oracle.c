@@ -39,6 +39,10 @@
  *                        face relaxivities per side and fluid; gate and
  *                        weights of the two-phase sums by brute force
  *   or_fill_gradient     pinned: closed form G*d*(idx-1/2)
+ *   or_glibc_rand        pinned: the C library's own rand() after srand(1)
+ *                        (glibc, first 10^5 outputs)
+ *   or_fill_noise        pinned: the reference's expression evaluated with
+ *                        the C library's rand() in the same cell order
  *   or_output            pinned: brute-force numpy sum with trimming
  *   or_step_time         pinned: formula arithmetic
  *   or_add_impulse       pinned: Rodrigues rotation closed form (sign
@@ -292,6 +296,52 @@ void or_fill_gradient(const or_params* p, double gx, double gy, double gz,
                 nfy[c] = gy * p->dy * (j - 0.5);
                 nfz[c] = gz * p->dz * (k - 0.5);
             }
+}
+
+/* glibc's rand() from its default state (no srand: seed 1), restated from
+ * the published description of the TYPE_3 additive-feedback generator
+ * behind random_r: r[0] = seed, r[i] = 16807 r[i-1] mod (2^31 - 1) for
+ * i < 31, r[i] = r[i-31] for i = 31..33, r[i] = r[i-31] + r[i-3] (mod 2^32)
+ * afterwards; the first 310 of the latter are discarded and output k is
+ * r[344 + k] >> 1.  out[n]. */
+void or_glibc_rand(int32_t n, int32_t* out)
+{
+    uint32_t r[34];
+    int64_t w = 1;
+    uint32_t init[34];
+    init[0] = 1u;
+    for (int i = 1; i < 31; i++) {
+        w = (16807LL * w) % 2147483647LL;
+        if (w < 0) w += 2147483647LL;
+        init[i] = (uint32_t)w;
+    }
+    for (int i = 31; i < 34; i++) init[i] = init[i - 31];
+    for (int i = 0; i < 34; i++) r[i] = init[i];
+    /* ring of the last 34 values: slot i % 34 holds r[i] */
+    for (long i = 34; i < 344L + n; i++) {
+        const uint32_t v = r[(i - 31) % 34] + r[(i - 3) % 34];
+        r[i % 34] = v;
+        if (i >= 344) out[i - 344] = (int32_t)(v >> 1);
+    }
+}
+
+/* The noise field of FillInMainField (NmrRelaxationSolve.cpp:1973-1978):
+ * with NDIRECT_FIELD = axis + 1 and AMPL_NOISE = ampl != 0, component
+ * `axis` of noiseField becomes ampl * (rand() % 2000 + 1 - 1000) / 1000 on
+ * every padded cell, k outermost and i innermost, one rand() per cell,
+ * replacing the constant-gradient value there.  nf: that component. */
+void or_fill_noise(const or_params* p, double ampl, double* nf)
+{
+    const long n = or_num_cells(p);
+    int32_t* r = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!r) return;
+    or_glibc_rand((int32_t)n, r);
+    long t = 0;
+    for (int k = 0; k < p->kk + 2; k++)
+        for (int j = 0; j < p->jk + 2; j++)
+            for (int i = 0; i < p->ik + 2; i++)
+                nf[IX(p, i, j, k)] = ampl * (double)(r[t++] % 2000 + 1 - 1000) / 1000;
+    free(r);
 }
 
 /* dt from the stability limits (NmrRelaxationSolve.cpp:2519-2534). */

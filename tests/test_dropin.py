@@ -270,6 +270,37 @@ def test_pfg_flow_against_fp64_loop(exe, tmp_path, cadence, scheme, nb):
     assert err < 5e-6
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("cadence", ["gpu", "cpu"])
+def test_noise_field_against_fp64_loop(exe, tmp_path, cadence):
+    """AMPL_NOISE with NDIRECT_FIELD = 3: FillInMainField draws one glibc
+    rand() per padded cell for the field along z (cpp:1973-1978), the field
+    then dephases the spins every step; sequence 2 with flow on both loops,
+    every CSV row against the FP64 restatement (the oracle draws the same
+    sequence from its own restatement of the generator)."""
+    shape = (9, 10, 12)
+    mask = generators.porosity(shape, 0.55, seed=14, sigma=1.2)
+    vel = generators.velocity(mask, seed=14, sigma=1.5, vmax=3e-3)
+    table = np.linspace(-0.03, 0.03, 399)
+    cfg = _write_case(tmp_path, shape, mask, vel, grad_table=table, NSEQUENCE=2, TIME1=2e-4, TIME2=4e-4,
+                      R2_WAT=2e-5, R1_WAT=1e-5, T1_WAT=0.05, Printing_on=5, AMPL_NOISE=2e-5, NVECTOR=1)
+    cfg["gradientPulse"] = float(table[349])
+    args = ["350", "0", "0"] + (["--cpu-cadence"] if cadence == "cpu" else [])
+    _run(exe, tmp_path, *args)
+    rows = refio.read_results(tmp_path / f"results_{cadence}.csv")
+    ref, dt, nstep = oracle.run_dropin(cfg, mask, vel, gpu=(cadence == "gpu"))
+    assert rows.shape == ref.shape, (rows.shape, ref.shape)
+    np.testing.assert_allclose(rows[:, 0], ref[:, 0], rtol=2e-6)
+    scale = np.abs(ref[:, 1:]).max()
+    err = np.abs(rows[:, 1:] - ref[:, 1:]).max() / scale
+    quiet, _, _ = oracle.run_dropin(dict(cfg, AMPL_NOISE=0.0), mask, vel, gpu=(cadence == "gpu"))
+    effect = np.abs(quiet[:, 1:] - ref[:, 1:]).max() / scale if quiet.shape == ref.shape else np.inf
+    print(f"[dropin] noise field, {cadence} cadence: {len(rows)} rows, max error {err:.2e} of the signal scale "
+          f"(the field moves the rows by {effect:.2e})", flush=True)
+    assert err < 5e-6
+    assert effect > 1e-3
+
+
 @pytest.mark.parametrize("nb,modes", [
     ((1, 1, 1, 1, 1, 1), [0, 0, 0]),
     ((3, 1, 3, 1, 3, 3), [1, 0, 1]),

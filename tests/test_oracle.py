@@ -399,3 +399,35 @@ def test_gradient_pulse_phase_linear_in_position():
     oracle.add_impulse(p, act, 1, -G, g, m)
     for a, b in zip(m, m0):
         np.testing.assert_allclose(a, b, atol=1e-13)
+
+
+def test_glibc_rand_restatement_matches_the_c_library():
+    """The oracle's restatement of glibc rand() (the generator the reference's
+    noise field draws from, never seeded) against the C library itself."""
+    import ctypes
+
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    want = np.array([libc.rand() for _ in range(100000)], dtype=np.int32)
+    np.testing.assert_array_equal(oracle.glibc_rand(100000), want)
+
+
+def test_noise_field_follows_the_reference_expression():
+    """FillInMainField's noise (cpp:1973-1978): AMPL_NOISE * (rand() % 2000 +
+    1 - 1000) / 1000 on every padded cell, k outermost, i innermost."""
+    import ctypes
+
+    p = oracle.params((4, 3, 5), dx=1e-6, dt=1e-5)
+    nf = np.full(oracle.padded_shape(p), 7.0)
+    oracle.fill_noise(p, 3.5e-6, nf)
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    want = np.empty(oracle.padded_shape(p))
+    for k in range(want.shape[0]):
+        for j in range(want.shape[1]):
+            for i in range(want.shape[2]):
+                want[k, j, i] = 3.5e-6 * (libc.rand() % 2000 + 1 - 1000) / 1000
+    np.testing.assert_array_equal(nf, want)
+    assert np.abs(want).max() <= 3.5e-6 and np.abs(want).min() >= 0.0
