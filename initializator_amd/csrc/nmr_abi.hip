@@ -1164,7 +1164,7 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
 namespace {
 
 std::mutex g_tune_mu;
-struct Tuned { int variant, lz, persistent; };
+struct Tuned { int variant, lz, persistent; std::array<double, 7> cost; };
 std::map<std::vector<int>, Tuned> g_tune_cache; // shapes -> fastest launch
 
 void batch_free(nmr_batch* b)
@@ -1199,16 +1199,17 @@ constexpr int kMixedVariant = -3; // every instance takes its own tile shape
 // One-step tile shape for an nx x ny instance under variant -3: the least
 // padded tile area weighted by each shape's measured cost per voxel on a
 // 256^3 grid (tile 256x4 fastest, 64x32 slowest).
-int mixed_variant(int nx, int ny)
+// The tile shape that pads an nx x ny instance least, weighed by each
+// shape's time per padded voxel (cost[v]).
+int mixed_variant(int nx, int ny, const double* cost)
 {
-    static const double speed[7] = {1.000, 1.088, 1.055, 1.154, 1.161, 1.257, 1.216};
     int best = 0;
     double bc = 1e300;
     for (int v = 0; v < 7; v++) {
         int lx, rpt, tx, ty, st;
         if (!nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st) || st != 1) continue;
         const double area = (double)((nx + tx - 1) / tx * tx) * (double)((ny + ty - 1) / ty * ty);
-        const double c = area * speed[v];
+        const double c = area * cost[v];
         if (c < bc) {
             bc = c;
             best = v;
@@ -1234,7 +1235,7 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
         const nmr_batch_instance& s = b->spec[i];
         auto& I = b->inst[i];
         nmr::BatchInst& d = hi[i];
-        const int vi = mixed ? mixed_variant(s.nx, s.ny) : variant;
+        const int vi = mixed ? mixed_variant(s.nx, s.ny, b->tile_cost.data()) : variant;
         nmr::variant_shape(vi, &lx, &rpt, &tx, &ty, &st);
         for (int q = 0; q < 3; q++) {
             d.m[0][q] = I.m[0][q];
@@ -1598,6 +1599,8 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
             b->variant = it->second.variant;
             b->lz = it->second.lz;
             b->persistent = it->second.persistent;
+            b->tile_cost = it->second.cost;
+            b->cur_variant = -2; // the per-instance shapes may have changed
             if (variant) *variant = b->variant;
             if (lz) *lz = b->lz;
             if (persistent) *persistent = b->persistent;
@@ -1607,13 +1610,32 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
     const int trials = std::max(1, (int)trial_steps);
     double best = 1e300;
     int bv = 0, bl = 8, bp = 1;
-    for (int v = kMixedVariant; v < nmr::num_variants(); v++) {
+    // every one-step tile shape on the whole batch, then per-instance shapes
+    // (variant -3) weighed by what the uniform trials measured per padded voxel
+    std::vector<int> order;
+    for (int v = 0; v < nmr::num_variants(); v++) {
         int lx, rpt, tx, ty, st;
-        if (v >= 0) {
-            nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st);
-            if (st != 1) continue;
-        } else if (v != kMixedVariant) {
-            continue;
+        if (nmr::variant_shape(v, &lx, &rpt, &tx, &ty, &st) && st == 1 && v < 7) order.push_back(v);
+    }
+    order.push_back(kMixedVariant);
+    std::array<double, 7> vbest;
+    vbest.fill(1e300);
+    for (int v : order) {
+        if (v == kMixedVariant) {
+            // time per padded voxel of each shape on this batch
+            bool all = true;
+            for (int u = 0; u < 7; u++) all = all && vbest[u] < 1e299;
+            if (all) {
+                for (int u = 0; u < 7; u++) {
+                    int lx, rpt, tx, ty, st;
+                    nmr::variant_shape(u, &lx, &rpt, &tx, &ty, &st);
+                    double area = 0.0;
+                    for (const auto& sp : b->spec)
+                        area += (double)((sp.nx + tx - 1) / tx * tx) * (double)((sp.ny + ty - 1) / ty * ty) * sp.nz;
+                    b->tile_cost[u] = vbest[u] / area;
+                }
+                b->cur_variant = -2;
+            }
         }
         for (int cfg = 0; cfg < 10; cfg++) {
             const int l = 4 << (cfg >> 1), per = cfg & 1; // lz 4..64, grid kind
@@ -1630,6 +1652,7 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
             HIP_TRY(hipEventSynchronize(b->ev1));
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (v >= 0 && v < 7) vbest[v] = std::min(vbest[v], (double)ms);
             if (ms < best) {
                 best = ms;
                 bv = v;
@@ -1643,7 +1666,7 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
     b->persistent = bp;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tune_cache[key] = {bv, bl, bp};
+        g_tune_cache[key] = {bv, bl, bp, b->tile_cost};
     }
     if (variant) *variant = bv;
     if (lz) *lz = bl;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -116,6 +117,9 @@ struct nmr_batch {
     // tile range [first, first + count) of the tile list
     struct Group { int variant, first, count, grid; };
     std::vector<Group> groups;
+    // time per padded voxel of each one-step tile shape (variant -3's
+    // weights): single-domain figures until the autotuner measures the batch
+    std::array<double, 7> tile_cost = {1.000, 1.088, 1.055, 1.154, 1.161, 1.257, 1.216};
     nmr::BatchInst* d_inst = nullptr;
     int2* d_tiles = nullptr;
     int *d_off = nullptr, *d_ids = nullptr, *d_step = nullptr;
