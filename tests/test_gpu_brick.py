@@ -37,7 +37,8 @@ def _same(name, a, b):
 
 @pytest.mark.parametrize("shape,variant", [((23, 37, 45), -1), ((9, 6, 261), 0), ((17, 14, 90), 2),
                                            ((13, 21, 50), 4), ((11, 33, 30), 6), ((5, 3, 7), -1),
-                                           ((9, 19, 261), 1), ((12, 37, 90), 3), ((13, 70, 48), 5)])
+                                           ((9, 19, 261), 1), ((12, 37, 90), 3), ((13, 70, 48), 5),
+                                           ((7, 9, 12), -1), ((6, 10, 20), 4), ((5, 13, 36), 2), ((8, 8, 64), 0)])
 def test_brick_sweep_equals_row_sweep(shape, variant):
     case = generators.case_flow(seed=sum(shape), shape=shape, phi=0.35, nsteps=40, big_delta=6e-4, delta=2e-4)
     rng = np.random.default_rng(1)
