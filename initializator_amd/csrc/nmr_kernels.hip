@@ -68,6 +68,18 @@ __device__ __forceinline__ uint32_t ldw_nt(const uint8_t* base, uint32_t boff)
     return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(base + boff));
 }
 
+// Harmonic mean 2ab/(a + b) of two (scaled) diffusivities for the face
+// between them, with the hardware reciprocal (v_rcp_f32, about 1 ulp) in
+// place of an IEEE division.  Sums below 1e-30 -- solids, which the
+// reference gives D = 1e-50 -- give 0: their reciprocal would overflow, and
+// the exact mean is below 1e-30 there anyway.  Both general kernels use it,
+// so they stay bit-identical.
+__device__ __forceinline__ float hmean(float a, float b)
+{
+    const float s = a + b;
+    return s > 1e-30f ? (2.f * a * b) * __builtin_amdgcn_rcpf(s) : 0.f;
+}
+
 // byte e (0/1) of a packed mask word as a float
 __device__ __forceinline__ float ub(uint32_t w, int e) { return (float)((w >> (8 * e)) & 0xffu); }
 
@@ -258,7 +270,7 @@ __global__ void __launch_bounds__(256) general_kernel(GenArgs a)
 #pragma unroll
         for (int s = 0; s < 6; s++) {
             const float dnb = kind[s] == 0 || kind[s] == 3 ? (kind[s] == 3 ? dc : a.dn[ni[s]]) : dc;
-            const float hm = (dc + dnb) > 0.f ? 2.f * dc * dnb / (dc + dnb) : 0.f;
+            const float hm = hmean(dc, dnb);
             w[s] = cd[s / 2] * hm * kn[s];
         }
     }
@@ -1298,7 +1310,7 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
             const float dnb[6] = {e > 0 ? dc[e - 1] : dxl, e < 3 ? dc[e + 1] : dxr, dym[e], dyp[e], dzm[e], dzp[e]};
 #pragma unroll
             for (int s = 0; s < 6; s++) {
-                const float hm = (d0 + dnb[s]) > 0.f ? 2.f * d0 * dnb[s] / (d0 + dnb[s]) : 0.f;
+                const float hm = hmean(d0, dnb[s]);
                 w[s] = cd[s / 2] * hm * kn[s];
             }
         } else {
