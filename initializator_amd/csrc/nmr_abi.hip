@@ -316,12 +316,12 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // solid values, e.g. through periodic ghosts).
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
         if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
-        // brick layout (uniform coefficients, plain sums, long runs): the
+        // brick layout (plain sums, long runs; per-voxel coefficients and
+        // fields stay in the row layout): the
         // state in both brick buffers -- the output one from the row buffer
         // whose solids were just cleared -- and, for MacCormack, a brick
         // predictor
-        const bool gbrick = ctx->brick_on && !ctx->hetero && !ctx->weighted && !ctx->bn &&
-                            nsteps >= kBrickMinSteps && bricks_fit(ctx);
+        const bool gbrick = ctx->brick_on && !ctx->weighted && nsteps >= kBrickMinSteps && bricks_fit(ctx);
         const int c0 = ctx->cur;
         const int nc4 = ctx->pitch / 4;
         if (gbrick) {

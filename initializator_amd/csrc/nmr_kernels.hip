@@ -2121,14 +2121,14 @@ static hipError_t launch_gs(int stage, int lanes, const GSweepArgs& a, hipStream
     return hipGetLastError();
 }
 
-template <int NC, bool VEL>
+template <int NC, bool VEL, bool COEF>
 static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 {
     const dim3 grid(a.nblocks), block(256);
     switch (stage) {
-    case GEN_UPWIND: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, false, 16, true>), grid, block, 0, s, a); break;
-    case GEN_PREDICT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, false, 16, true>), grid, block, 0, s, a); break;
-    case GEN_CORRECT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, false, 16, true>), grid, block, 0, s, a); break;
+    case GEN_UPWIND: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true>), grid, block, 0, s, a); break;
+    case GEN_PREDICT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true>), grid, block, 0, s, a); break;
+    case GEN_CORRECT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2137,14 +2137,19 @@ static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
                          bool brick)
 {
-    if (brick) { // lanes 16, uniform coefficients
-        if (coef || lanes != 16) return hipErrorInvalidValue;
+    if (brick) { // lanes 16; the coefficient arrays stay in the row layout
+        if (lanes != 16) return hipErrorInvalidValue;
+#define NMR_GSB(NC_)                                                                         \
+    if (coef) return vel ? launch_gs_brick<NC_, true, true>(stage, a, s)                      \
+                         : launch_gs_brick<NC_, false, true>(stage, a, s);                    \
+    return vel ? launch_gs_brick<NC_, true, false>(stage, a, s) : launch_gs_brick<NC_, false, false>(stage, a, s);
         switch (nc) {
-        case 1: return vel ? launch_gs_brick<1, true>(stage, a, s) : launch_gs_brick<1, false>(stage, a, s);
-        case 2: return vel ? launch_gs_brick<2, true>(stage, a, s) : launch_gs_brick<2, false>(stage, a, s);
-        case 3: return vel ? launch_gs_brick<3, true>(stage, a, s) : launch_gs_brick<3, false>(stage, a, s);
+        case 1: NMR_GSB(1)
+        case 2: NMR_GSB(2)
+        case 3: NMR_GSB(3)
         default: return hipErrorInvalidValue;
         }
+#undef NMR_GSB
     }
 #define NMR_GSC(NC_)                                                                \
     if (coef) return vel ? launch_gs<NC_, true, true>(stage, lanes, a, s)           \

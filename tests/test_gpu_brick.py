@@ -13,12 +13,16 @@ from initializator_amd.solver import HipSolver
 pytestmark = pytest.mark.gpu
 
 
-def _run(case, m0, brick, variant=-1, nsteps=None, **kw):
+def _run(case, m0, brick, variant=-1, nsteps=None, coef=None, field=None, **kw):
     n = case.nsteps if nsteps is None else nsteps
     with HipSolver.from_case(case, **kw) as s:
         s.set_layout(brick)
         s.set_tile(variant, 0)
         s.upload_magnetization(*m0[: case.ncomp])
+        if coef is not None:
+            s.upload_coefficients(coef[0], coef[1], r1=coef[2])
+        if field is not None:
+            s.upload_field(field)
         sched = None if case.grad_sched is None else case.grad_sched[:n]
         sig = s.run(n, sched)
         f = s.download_magnetization()
@@ -95,3 +99,24 @@ def test_brick_general_sweep_equals_row_and_per_voxel(shape, scheme, bc):
     row = _run(case, m0, False, **kw)
     _same(f"general {shape} scheme {scheme} bc {bc}: brick vs row", brick, row)
     _same(f"general {shape} scheme {scheme} bc {bc}: brick vs per-voxel", brick, _run(case, m0, False, -2, **kw))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("scheme", [MAK, 0])
+def test_brick_general_sweep_per_voxel_coefficients_and_field(nc, scheme):
+    """Per-voxel D, 1/T1, 1/T2 and a per-voxel field (row-layout arrays) with
+    the state on bricks: the same bits as the row layout and the per-voxel
+    kernel."""
+    shape = (11, 13, 36)
+    case = generators.case_flow(seed=77 + nc, shape=shape, phi=0.35, nsteps=20, big_delta=6e-4, delta=2e-4)
+    case.ncomp = nc
+    rng = np.random.default_rng(nc)
+    m0 = [rng.random(shape) for _ in range(3)]
+    coef = (rng.uniform(0.5e-9, 3e-9, shape), rng.uniform(0.2, 40.0, shape),
+            rng.uniform(0.1, 5.0, shape) if nc == 3 else None)
+    field = rng.uniform(-2e-5, 2e-5, shape)
+    kw = dict(scheme=scheme, coef=coef, field=field)
+    brick = _run(case, m0, True, **kw)
+    _same(f"coefficients + field ncomp {nc} scheme {scheme}: brick vs row", brick, _run(case, m0, False, **kw))
+    _same(f"coefficients + field ncomp {nc} scheme {scheme}: brick vs per-voxel", brick,
+          _run(case, m0, False, -2, **kw))
