@@ -185,7 +185,7 @@ int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
  * keeps a copy of the state and the velocity in bricks of 128 bytes (8
  * x-voxels by 4 rows of one plane) for runs of 16 steps or more -- the
  * one-step sweep with any one-step tile variant, the general sweep with
- * uniform coefficients and plain sums -- so that all-solid bricks are never
+ * plain sums; per-voxel coefficients and fields stay in rows -- so that all-solid bricks are never
  * fetched, converting the state in and out once per run; 0 always sweeps
  * the row layout.  The fields are the same bits either way. */
 int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
