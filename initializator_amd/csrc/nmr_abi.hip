@@ -256,7 +256,7 @@ extern const int64_t kBrickMinSteps;
 // Steps of the general problems (MacCormack, non-zero-derivative boundaries,
 // per-voxel coefficients, two-phase sums): the general z-sweep, or with
 // variant -2 the per-voxel kernel; each step's signal is fused into the sweep
-// (plain sums) or taken by the field sum (phase-weighted sums).
+// (plain sums; phase-weighted sums on bricks) or taken by the field sum.
 int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
 {
     const nmr_problem& p = ctx->prob;
@@ -316,12 +316,11 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // solid values, e.g. through periodic ghosts).
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
         if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
-        // brick layout (plain sums, long runs; per-voxel coefficients and
-        // fields stay in the row layout): the
-        // state in both brick buffers -- the output one from the row buffer
-        // whose solids were just cleared -- and, for MacCormack, a brick
-        // predictor
-        const bool gbrick = ctx->brick_on && !ctx->weighted && nsteps >= kBrickMinSteps && bricks_fit(ctx);
+        // brick layout (long runs; per-voxel coefficients, fields and phase
+        // weights stay in the row layout): the state in both brick buffers --
+        // the output one from the row buffer whose solids were just cleared
+        // -- and, for MacCormack, a brick predictor
+        const bool gbrick = ctx->brick_on && nsteps >= kBrickMinSteps && bricks_fit(ctx);
         const int c0 = ctx->cur;
         const int nc4 = ctx->pitch / 4;
         if (gbrick) {
@@ -347,8 +346,10 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         const long ntz = (p.nz + lz - 1) / lz;
         const long nwg = ntx * nty * ntz;
         if (nwg > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
-        const bool fused = !ctx->weighted;
-        if (fused && (rc = ensure_partials(ctx, (size_t)kChunkSteps * nwg * nc))) return rc;
+        // each step's signal: fused into the sweep (plain sums; weighted sums
+        // on bricks), else a field sum after the step
+        const bool fused = !ctx->weighted || gbrick;
+        if (fused && (rc = ensure_partials(ctx, (size_t)kChunkSteps * nwg * ns))) return rc;
         int w[6];
         signal_window(p, w);
         nmr::GSweepArgs a{};
@@ -381,6 +382,10 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
             a.vx = ctx->vxb;
             a.vy = ctx->vyb;
             a.vz = ctx->vzb;
+            if (ctx->weighted) {
+                a.w[0] = ctx->w[0];
+                a.w[1] = ctx->w[1];
+            }
         }
         const bool coef = ctx->hetero;
         int64_t done = 0;
@@ -404,7 +409,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                     a.mavg[q] = cin[q];
                     a.out[q] = cout[q];
                 }
-                a.partials = fused ? ctx->partials + (size_t)s * nwg * nc : nullptr;
+                a.partials = fused ? ctx->partials + (size_t)s * nwg * ns : nullptr;
                 HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, mak ? nmr::GEN_CORRECT : nmr::GEN_UPWIND, coef, lanes, a,
                                            ctx->stream, gbrick));
                 ctx->cur ^= 1;
@@ -421,8 +426,8 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                 if (!fused && (rc = enqueue_field_sum(ctx, (size_t)(done + s) + 1))) return rc;
             }
             if (fused)
-                HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)nwg, nc,
-                                                    ctx->dsignal + (size_t)(done + 1) * nc, ctx->stream));
+                HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)nwg, ns,
+                                                    ctx->dsignal + (size_t)(done + 1) * ns, ctx->stream));
             done += chunk;
         }
         if (gbrick) { // the final state back to the current row buffer

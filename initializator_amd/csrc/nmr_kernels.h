@@ -134,6 +134,8 @@ struct GSweepArgs {
     const float* bn;        // per-voxel field (T) along axis bnax, in place of that axis's gradient field; or null
     int bnax;
     double* partials;       // [nblocks][nc] pore sums of the result, or null
+    const float* w[2];      // brick sweep: phase weights (row layout) of fused weighted sums,
+                            // partials then [nblocks][2*nc]; or null
     int nx, ny, nz, pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis

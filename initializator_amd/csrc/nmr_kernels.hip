@@ -387,6 +387,25 @@ __device__ __forceinline__ void accumulate(double (&acc)[NC], const f4 (&out)[NC
         }
 }
 
+// Phase-weighted pore sums of the chunk: acc[q] += v*w1, acc[NC+q] += v*w2
+// (FP32 operands, exact FP64 products, as field_sum_kernel forms them).
+template <int NC>
+__device__ __forceinline__ void accumulate_w(double (&acc)[2 * NC], const f4 (&out)[NC], uint32_t kc,
+                                             const bool (&tx)[4], const f4& w1, const f4& w2)
+{
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        if (tx[e] && ((kc >> (8 * e)) & 0xffu) != 0) {
+            const double a1 = (double)w1[e], a2 = (double)w2[e];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const double v = (double)out[q][e];
+                acc[q] += v * a1;
+                acc[NC + q] += v * a2;
+            }
+        }
+}
+
 // Workgroup sum of acc into part[blockIdx.x*NC + q] (fixed order).
 template <int NC, int NT>
 __device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[NC], double* dst)
@@ -1415,8 +1434,8 @@ __device__ __forceinline__ f4 fill_tail(f4 v, int last, float g)
 // every voxel that is updated.  Ghost rows and columns are staged in LDS by
 // the threads that own the tile's halo; ghost planes are loaded in place of
 // the out-of-range planes.
-template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false>
-__device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, double (&acc)[NC])
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false>
+__device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, double (&acc)[W ? 2 * NC : NC])
 {
     constexpr int NT = 256;
     constexpr int RY = NT / LX;
@@ -1638,6 +1657,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     };
 
     f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, wc = vx4, wn = vx4, r1c = vx4, r2c = vx4, bnc = vx4;
+    f4 pw1 = vx4, pw2 = vx4; // W: phase weights of the own chunk
     float vxe = 0.f;
     const int bnax = a.bn ? a.bnax : -1;
     f4 mav[NC];
@@ -1664,6 +1684,10 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                 }
             }
             if (bnax >= 0) bnc = ld4_nt(a.bn, (uint32_t)kk * plane_b + rowc); // row layout, as the coefficients
+            if constexpr (W && ST != GEN_PREDICT) {
+                pw1 = ld4_nt(a.w[0], (uint32_t)kk * plane_b + rowc);
+                pw2 = ld4_nt(a.w[1], (uint32_t)kk * plane_b + rowc);
+            }
             if constexpr (ST == GEN_CORRECT) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) mav[q] = ld4_nt(a.mavg[q], pz + rowf);
@@ -1772,7 +1796,11 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                                              vxe, vy4, vy4n, wc, wn, dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
                                              r2c, mav, a.amask != 0,
                                              ix, thy, thz, bnc, bnax, cf.dtg, out);
-            if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
+            if constexpr (W) {
+                if (ST != GEN_PREDICT && trim) accumulate_w<NC>(acc, out, kc, trim_x, pw1, pw2);
+            } else {
+                if (ST != GEN_PREDICT && trim) accumulate<NC>(acc, out, kc, trim_x);
+            }
             const uint32_t of = zo(k) + rowf;
 #pragma unroll
             for (int q = 0; q < NC; q++) st4_nt(a.out[q], of, out[q]);
@@ -1803,15 +1831,18 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     }
 }
 
-template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false>
+// W: phase-weighted sums, 2*NC per workgroup (the NC sums weighted by w[0],
+// then the NC weighted by w[1], as field_sum_kernel lays them out)
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false>
 __global__ void __launch_bounds__(256) gsweep_kernel(GSweepArgs a)
 {
-    __shared__ double sred[4][NC];
-    double acc[NC];
+    constexpr int NS = W ? 2 * NC : NC;
+    __shared__ double sred[4][NS];
+    double acc[NS];
 #pragma unroll
-    for (int q = 0; q < NC; q++) acc[q] = 0.0;
-    gsweep_tile<NC, VEL, ST, COEF, LX, BRK>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
-    if (ST != GEN_PREDICT && a.partials) block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
+    for (int q = 0; q < NS; q++) acc[q] = 0.0;
+    gsweep_tile<NC, VEL, ST, COEF, LX, BRK, W>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    if (ST != GEN_PREDICT && a.partials) block_reduce<NS, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NS);
 }
 
 __global__ void __launch_bounds__(256) zero_solids_kernel(float* m0, float* m1, float* m2, int nc,
@@ -2106,6 +2137,7 @@ int gsweep_lanes(int nx) { return nx > 128 ? 64 : 32; }
 template <int NC, bool VEL, bool COEF>
 static hipError_t launch_gs(int stage, int lanes, const GSweepArgs& a, hipStream_t s)
 {
+    if (a.w[0]) return hipErrorInvalidValue; // weighted sums are fused on bricks only
     const dim3 grid(a.nblocks), block(256);
 #define NMR_GS(ST_)                                                                                     \
     if (lanes == 64) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, ST_, COEF, 64>), grid, block, 0, s, a); \
@@ -2125,10 +2157,18 @@ template <int NC, bool VEL, bool COEF>
 static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 {
     const dim3 grid(a.nblocks), block(256);
+    const bool w = a.w[0] != nullptr; // fused phase-weighted sums
+    if (w && !a.w[1]) return hipErrorInvalidValue;
     switch (stage) {
-    case GEN_UPWIND: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true>), grid, block, 0, s, a); break;
+    case GEN_UPWIND:
+        if (w) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true>), grid, block, 0, s, a);
+        break;
     case GEN_PREDICT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true>), grid, block, 0, s, a); break;
-    case GEN_CORRECT: hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true>), grid, block, 0, s, a); break;
+    case GEN_CORRECT:
+        if (w) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true>), grid, block, 0, s, a);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

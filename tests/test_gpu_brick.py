@@ -13,7 +13,7 @@ from initializator_amd.solver import HipSolver
 pytestmark = pytest.mark.gpu
 
 
-def _run(case, m0, brick, variant=-1, nsteps=None, coef=None, field=None, **kw):
+def _run(case, m0, brick, variant=-1, nsteps=None, coef=None, field=None, weights=None, **kw):
     n = case.nsteps if nsteps is None else nsteps
     with HipSolver.from_case(case, **kw) as s:
         s.set_layout(brick)
@@ -23,6 +23,8 @@ def _run(case, m0, brick, variant=-1, nsteps=None, coef=None, field=None, **kw):
             s.upload_coefficients(coef[0], coef[1], r1=coef[2])
         if field is not None:
             s.upload_field(field)
+        if weights is not None:
+            s.upload_phase_weights(*weights)
         sched = None if case.grad_sched is None else case.grad_sched[:n]
         sig = s.run(n, sched)
         f = s.download_magnetization()
@@ -119,4 +121,35 @@ def test_brick_general_sweep_per_voxel_coefficients_and_field(nc, scheme):
     brick = _run(case, m0, True, **kw)
     _same(f"coefficients + field ncomp {nc} scheme {scheme}: brick vs row", brick, _run(case, m0, False, **kw))
     _same(f"coefficients + field ncomp {nc} scheme {scheme}: brick vs per-voxel", brick,
+          _run(case, m0, False, -2, **kw))
+
+
+@pytest.mark.parametrize("nc,scheme,buffer", [(1, 0, (0, 0, 0)), (2, MAK, (2, 1, 3)), (3, 0, (1, 2, 0)),
+                                              (2, 0, (0, 0, 0))])
+def test_brick_general_sweep_fused_phase_weighted_sums(nc, scheme, buffer):
+    """Two-phase weighted sums: on bricks they are fused into the sweep
+    (2*ncomp sums per workgroup, weights in rows), in the row layout and the
+    per-voxel kernel a field sum follows each step; the fields are the same
+    bits and the 2*ncomp signal columns agree to FP64 summation order, also
+    with a trimmed signal window."""
+    shape = (13, 11, 40)
+    case = generators.case_flow(seed=31 + nc, shape=shape, phi=0.4, nsteps=24, big_delta=6e-4, delta=2e-4)
+    case.ncomp = nc
+    rng = np.random.default_rng(10 + nc)
+    m0 = [rng.random(shape) for _ in range(3)]
+    w1 = rng.random(shape)
+    kw = dict(scheme=scheme, weights=(w1, 1.0 - w1), buffer=buffer,
+              coef=(rng.uniform(0.5e-9, 3e-9, shape), rng.uniform(0.2, 40.0, shape), None) if nc == 2 else None)
+    if nc == 1:  # one component: no gradient precession
+        case.grad_sched = None
+    brick = _run(case, m0, True, **kw)
+    assert brick[0].shape == (case.nsteps + 1, 2 * nc)
+    # the weights split the plain sum: w1 + w2 = 1 at every voxel, up to
+    # their rounding to FP32
+    plain = _run(case, m0, True, **dict(kw, weights=None))
+    np.testing.assert_allclose(brick[0][:, :nc] + brick[0][:, nc:], plain[0], rtol=0,
+                               atol=1e-6 * np.abs(plain[0]).max())
+    _same(f"weighted sums ncomp {nc} scheme {scheme} buffer {buffer}: brick vs row", brick,
+          _run(case, m0, False, **kw))
+    _same(f"weighted sums ncomp {nc} scheme {scheme} buffer {buffer}: brick vs per-voxel", brick,
           _run(case, m0, False, -2, **kw))
