@@ -267,6 +267,11 @@ hipError_t launch_brick(const float* src, float* dst, int nc4, int Y, int Z, boo
 hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
 hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s);
 int batch_resident_blocks(int nc, bool vel, bool exact, int variant);
+// Workgroups of the one-step sweep (variant 0..6, row or brick layout) and of
+// the general sweep's upwind / corrector kernel resident on the device at
+// once (CUs x occupancy); -1 if the query fails.
+int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick);
+int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick);
 hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
                                const int* step_base, int row0, int64_t rows, double* signal, int nsteps, int ninst,
                                hipStream_t s);

@@ -2040,6 +2040,17 @@ hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, con
     return hipErrorInvalidValue;
 }
 
+// Workgroups of a kernel resident on the whole device at once.
+static int resident_blocks(const void* f)
+{
+    int per_cu = 0;
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) return -1;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    return per_cu * ncu;
+}
+
 int batch_resident_blocks(int nc, bool vel, bool exact, int variant)
 {
     const void* f = nullptr;
@@ -2060,12 +2071,53 @@ int batch_resident_blocks(int nc, bool vel, bool exact, int variant)
     else if (nc == 3) { if (vel) { NMR_PICK(3, true, false) } else { NMR_PICK(3, false, false) } }
     else return -1;
 #undef NMR_PICK
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess) return -1;
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    return per_cu * ncu;
+    return resident_blocks(f);
+}
+
+int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
+{
+    const void* f = nullptr;
+#define NMR_PICK(NC_, V_, E_, B_)                                                               \
+    switch (variant) {                                                                          \
+    case 0: f = (const void*)sweep_kernel<NC_, V_, E_, 64, 1, B_>; break;                       \
+    case 1: f = (const void*)sweep_kernel<NC_, V_, E_, 64, 2, B_>; break;                       \
+    case 2: f = (const void*)sweep_kernel<NC_, V_, E_, 32, 1, B_>; break;                       \
+    case 3: f = (const void*)sweep_kernel<NC_, V_, E_, 32, 2, B_>; break;                       \
+    case 4: f = (const void*)sweep_kernel<NC_, V_, E_, 16, 1, B_>; break;                       \
+    case 5: f = (const void*)sweep_kernel<NC_, V_, E_, 16, 2, B_>; break;                       \
+    case 6: f = (const void*)sweep_kernel<NC_, V_, E_, 8, 1, B_>; break;                        \
+    default: return -1;                                                                         \
+    }
+#define NMR_PICKB(NC_, V_, E_) if (brick) { NMR_PICK(NC_, V_, E_, true) } else { NMR_PICK(NC_, V_, E_, false) }
+    if (nc == 1) { if (vel) { NMR_PICKB(1, true, false) } else { NMR_PICKB(1, false, false) } }
+    else if (nc == 2 && exact) { if (vel) { NMR_PICKB(2, true, true) } else { NMR_PICKB(2, false, true) } }
+    else if (nc == 2) { if (vel) { NMR_PICKB(2, true, false) } else { NMR_PICKB(2, false, false) } }
+    else if (nc == 3) { if (vel) { NMR_PICKB(3, true, false) } else { NMR_PICKB(3, false, false) } }
+    else return -1;
+#undef NMR_PICKB
+#undef NMR_PICK
+    return resident_blocks(f);
+}
+
+int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick)
+{
+    // the upwind / corrector kernel (the predictor holds fewer registers)
+    const void* f = nullptr;
+#define NMR_PICK(NC_, V_, C_)                                                                   \
+    if (brick) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true>;               \
+    else if (lanes == 64) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 64>;          \
+    else if (lanes == 32) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 32>;          \
+    else return -1;
+#define NMR_PICKC(NC_)                                                                          \
+    if (coef) { if (vel) { NMR_PICK(NC_, true, true) } else { NMR_PICK(NC_, false, true) } }    \
+    else { if (vel) { NMR_PICK(NC_, true, false) } else { NMR_PICK(NC_, false, false) } }
+    if (nc == 1) { NMR_PICKC(1) }
+    else if (nc == 2) { NMR_PICKC(2) }
+    else if (nc == 3) { NMR_PICKC(3) }
+    else return -1;
+#undef NMR_PICKC
+#undef NMR_PICK
+    return resident_blocks(f);
 }
 
 hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
