@@ -184,15 +184,16 @@ int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 /* Device layout of the sweeps (no reference counterpart): 1 (the default)
  * keeps a copy of the state and the velocity in bricks of 128 bytes (8
  * x-voxels by 4 rows of one plane) for runs of 16 steps or more -- the
- * one-step sweep with any one-step tile variant, the general sweep with
- * plain sums; per-voxel coefficients and fields stay in rows -- so that all-solid bricks are never
- * fetched, converting the state in and out once per run; 0 always sweeps
- * the row layout.  The fields are the same bits either way. */
+ * one-step sweep with any one-step tile variant, the general sweep (its
+ * per-voxel coefficients, fields and phase weights stay in rows) -- so
+ * that all-solid bricks are never fetched, converting the state in and out
+ * once per run; 0 always sweeps the row layout.  The fields are the same
+ * bits either way. */
 int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
 /* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = a
  * one-thread-per-voxel kernel with the arithmetic of the sweeps, a
- * cross-check) and z-chunk length
- * (0 = automatic).  nmr_hip_num_variants() gives the variant count;
+ * cross-check) and z-chunk length (0 = automatic: the length in 6..12
+ * that best fills whole rounds of the workgroups the device holds at once).  nmr_hip_num_variants() gives the variant count;
  * nmr_hip_variant_info() describes one: lanes per row, rows per thread, tile
  * width and height, and time steps per pass (2 = temporally blocked sweep,
  * usable when nx <= tile_x).  Every variant gives bit-identical fields. */
