@@ -184,11 +184,11 @@ int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 /* Device layout of the sweeps (no reference counterpart): 1 (the default)
  * keeps a copy of the state and the velocity in bricks of 128 bytes (8
  * x-voxels by 4 rows of one plane) for runs of 16 steps or more -- the
- * one-step sweep with any one-step tile variant, the general sweep (its
- * per-voxel coefficients, fields and phase weights stay in rows) -- so
- * that all-solid bricks are never fetched, converting the state in and out
- * once per run; 0 always sweeps the row layout.  The fields are the same
- * bits either way. */
+ * one-step sweep with any one-step tile variant, the general sweep with
+ * brick copies of the per-voxel coefficients, field and phase weights --
+ * so that all-solid bricks are never fetched, converting the state in and
+ * out once per run (the velocity and the per-voxel arrays once per upload);
+ * 0 always sweeps the row layout.  The fields are the same bits either way. */
 int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
 /* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = a
  * one-thread-per-voxel kernel with the arithmetic of the sweeps, a

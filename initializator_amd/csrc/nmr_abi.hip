@@ -69,6 +69,8 @@ void free_all(nmr_ctx* c)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
     for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
         if (q) (void)hipFree(q);
+    for (float* q : c->pvb)
+        if (q) (void)hipFree(q);
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < 3; q++)
             if (c->mb[b][q]) (void)hipFree(c->mb[b][q]);
@@ -278,6 +280,7 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
 int refresh_inert(nmr_ctx* c);
 bool bricks_fit(const nmr_ctx* c);
 int ensure_bricks(nmr_ctx* c);
+int ensure_pv_bricks(nmr_ctx* c);
 extern const int64_t kBrickMinSteps;
 
 // Steps of the general problems (MacCormack, non-zero-derivative boundaries,
@@ -343,10 +346,10 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // solid values, e.g. through periodic ghosts).
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
         if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
-        // brick layout (long runs; per-voxel coefficients, fields and phase
-        // weights stay in the row layout): the state in both brick buffers --
-        // the output one from the row buffer whose solids were just cleared
-        // -- and, for MacCormack, a brick predictor
+        // brick layout (long runs): the state in both brick buffers -- the
+        // output one from the row buffer whose solids were just cleared --,
+        // for MacCormack a brick predictor, and brick copies of the velocity
+        // and of the per-voxel arrays
         const bool gbrick = ctx->brick_on && nsteps >= kBrickMinSteps && bricks_fit(ctx);
         const int c0 = ctx->cur;
         const int nc4 = ctx->pitch / 4;
@@ -407,13 +410,18 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         a.nby = (p.ny + 3) / 4;
         a.nbyv = (p.ny + 4) / 4;
-        if (gbrick) {
+        if (gbrick) { // velocity and per-voxel arrays in bricks too
+            if ((rc = ensure_pv_bricks(ctx))) return rc;
             a.vx = ctx->vxb;
             a.vy = ctx->vyb;
             a.vz = ctx->vzb;
+            a.dn = ctx->dn ? ctx->pvb[0] : nullptr;
+            a.r1v = ctx->r1v ? ctx->pvb[1] : nullptr;
+            a.r2v = ctx->r2v ? ctx->pvb[2] : nullptr;
+            a.bn = ctx->bn ? ctx->pvb[3] : nullptr;
             if (ctx->weighted) {
-                a.w[0] = ctx->w[0];
-                a.w[1] = ctx->w[1];
+                a.w[0] = ctx->pvb[4];
+                a.w[1] = ctx->pvb[5];
             }
         }
         const bool coef = ctx->hetero;
@@ -514,6 +522,26 @@ int ensure_bricks(nmr_ctx* c)
         HIP_TRY(nmr::launch_brick(c->vz, c->vzb, nbx, p.ny, p.nz + 1, true, c->stream));
         c->brick_vel_valid = true;
     }
+    return NMR_OK;
+}
+
+// Brick copies of the per-voxel arrays the general sweep reads (those
+// uploaded), converted once after each upload.
+int ensure_pv_bricks(nmr_ctx* c)
+{
+    if (c->brick_pv_valid) return NMR_OK;
+    const nmr_problem& p = c->prob;
+    const int nbx = c->pitch / 4;
+    float* const src[6] = {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1]};
+    for (int i = 0; i < 6; i++) {
+        if (!src[i]) continue;
+        if (!c->pvb[i]) {
+            int rc = dev_alloc(c, &c->pvb[i], nmr::brick_floats(nbx, p.ny, p.nz));
+            if (rc) return rc;
+        }
+        HIP_TRY(nmr::launch_brick(src[i], c->pvb[i], nbx, p.ny, p.nz, true, c->stream));
+    }
+    c->brick_pv_valid = true;
     return NMR_OK;
 }
 
@@ -830,6 +858,7 @@ static int put_voxels(nmr_ctx* ctx, float** dst, const std::vector<double>& v, d
         int rc = dev_alloc(ctx, dst, ctx->n_m);
         if (rc) return rc;
     }
+    ctx->brick_pv_valid = false;
     std::vector<float> h(ctx->n_m, 0.f);
     for (int k = 0; k < p.nz; k++)
         for (int j = 0; j < p.ny; j++)

@@ -69,6 +69,10 @@ struct nmr_ctx {
     float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;
     float* pb[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor in bricks
     bool brick_vel_valid = false;
+    // brick copies of the per-voxel arrays, in the order dn, r1v, r2v, bn,
+    // w[0], w[1]; converted after each upload, before the next brick run
+    float* pvb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool brick_pv_valid = false;
     int nsig() const { return weighted ? 2 * nc : nc; }
     uint8_t* mask = nullptr;
     float* vx = nullptr;

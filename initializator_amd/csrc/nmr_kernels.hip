@@ -1479,7 +1479,8 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
     // byte offsets along x, y, z of the float arrays (row layout, or with
     // BRK the 128-byte bricks of 8 x-voxels by 4 rows of the plain sweep);
-    // the per-voxel coefficient arrays always use the row layout
+    // the per-voxel arrays (coefficients, field, phase weights) are laid out
+    // as the state
     const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3;
     auto xo = [&](int x) -> uint32_t {
         return BRK ? ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2) : (uint32_t)x * 4u;
@@ -1497,7 +1498,6 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     auto zov = [&](int z) -> uint32_t { return BRK ? (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u : (uint32_t)z * planey_b; };
     const int yc = iy < ny ? iy : ny - 1; // rows past ny never load
     const uint32_t rowf = yo(yc) + xo(xin ? ix : 0);
-    const uint32_t rowc = (uint32_t)yc * Pb + (uint32_t)(xin ? ix : 0) * 4u; // coefficient arrays
     const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
     const uint32_t rowx = yo1(yc) + xo(xin ? ix : 0);
 
@@ -1576,7 +1576,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     };
     auto load_d = [&](int kk) -> f4 {
         if constexpr (COEF) {
-            if (xin) return ld4_nt(a.dn, (uint32_t)clampz(kk) * plane_b + rowc);
+            if (xin) return ld4_nt(a.dn, zo(clampz(kk)) + rowf);
         }
         return f4{0.f, 0.f, 0.f, 0.f};
     };
@@ -1596,7 +1596,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     };
     auto load_hd = [&](int kk) -> f4 {
         if constexpr (COEF) {
-            if (hrow) return ld4(a.dn, (uint32_t)clampz(kk) * plane_b + (uint32_t)hmrow * Pb + (uint32_t)ix * 4u);
+            if (hrow) return ld4(a.dn, zo(clampz(kk)) + yo(hmrow) + xo(ix));
         }
         return f4{0.f, 0.f, 0.f, 0.f};
     };
@@ -1615,7 +1615,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             if (!xgl) {
                 const uint32_t cz = (uint32_t)clampz(kk);
                 kx = a.mask[cz * plane_m + (uint32_t)iy * (uint32_t)a.pitch + (uint32_t)hcx];
-                if constexpr (COEF) xd = ld1(a.dn, cz * plane_b + (uint32_t)iy * Pb + (uint32_t)hcx * 4u);
+                if constexpr (COEF) xd = ld1(a.dn, zo((int)cz) + yo(iy) + xo(hcx));
             }
         }
         if (gxr && iy < ny) {
@@ -1677,16 +1677,16 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                 wn = ld4(a.vz, zo(kk + 1) + rowf);
             }
             if constexpr (COEF) {
-                const uint32_t pc = (uint32_t)kk * plane_b + rowc;
+                const uint32_t pc = zo(kk) + rowf;
                 r2c = ld4_nt(a.r2v, pc);
                 if constexpr (NC == 3) {
                     if (a.r1v) r1c = ld4_nt(a.r1v, pc);
                 }
             }
-            if (bnax >= 0) bnc = ld4_nt(a.bn, (uint32_t)kk * plane_b + rowc); // row layout, as the coefficients
+            if (bnax >= 0) bnc = ld4_nt(a.bn, zo(kk) + rowf);
             if constexpr (W && ST != GEN_PREDICT) {
-                pw1 = ld4_nt(a.w[0], (uint32_t)kk * plane_b + rowc);
-                pw2 = ld4_nt(a.w[1], (uint32_t)kk * plane_b + rowc);
+                pw1 = ld4_nt(a.w[0], zo(kk) + rowf);
+                pw2 = ld4_nt(a.w[1], zo(kk) + rowf);
             }
             if constexpr (ST == GEN_CORRECT) {
 #pragma unroll
