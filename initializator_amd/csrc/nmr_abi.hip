@@ -115,9 +115,12 @@ void fill_launch(const nmr_problem& p, Launch* L)
 // automatic.  The workgroups of a sweep run in rounds of `resident` (CUs x
 // workgroups per CU), a round lasting about lz + 2 plane-times (the chunk and
 // its two halo planes), so a step takes about ceil(nwg / resident) * (lz + 2)
-// of them: the lz in 6..12 that minimises this, 8 on ties.  At 256^3
-// (64 brick tiles per chunk layer, 768 resident) this picks lz 11, two full
-// rounds: 73.1 us per step against 76.8 us at lz 8 and 85.9 us at lz 16.
+// of them: the lz in 6..64 that minimises this, 8 on ties, else the
+// shortest.  At 256^3 (64 brick tiles per chunk layer, 768 resident) it
+// picks lz 22, one full round: 70.9 us per step against 73.0 at lz 11 (two
+// rounds), 76.8 at lz 8 (2 2/3) and 85.9 at lz 16 (1 1/3); on 512x512x130
+// lz 44 / 22 / 11 / 33 (1, 2, 4, 1 1/3 rounds) measured 138.7 / 144.5 /
+// 148.1 / 168.8 us, the model's order.
 int round_lz(long tiles_xy, int nz, long resident)
 {
     if (resident <= 0 || tiles_xy <= 0 || nz <= 0) return std::max(1, std::min(8, nz));
@@ -127,7 +130,7 @@ int round_lz(long tiles_xy, int nz, long resident)
     };
     int best = 8;
     long bc = cost(8);
-    for (int lz = 6; lz <= 12; lz++) {
+    for (int lz = 6; lz <= 64; lz++) {
         const long c = cost(lz);
         if (c < bc) {
             bc = c;
