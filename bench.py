@@ -35,11 +35,11 @@ BYTES_PER_VOXEL_STEP_MAK = 66.0
 # automatic tile, brick layout): rocprofv3 --pmc FETCH_SIZE (x2.0) +
 # WRITE_SIZE (x1.0) per dispatch, scaled by what the same counters read for a
 # plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
-DEFAULT_TRAFFIC = 246.1e6 + 50.4e6
+DEFAULT_TRAFFIC = 241.2e6 + 50.4e6
 # The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
 # predictor and the corrector launch of one step together (summary under
 # profiles/, gsweep_maccormack_256cube_flow_pgse.md).
-MAK_TRAFFIC = (246.4e6 + 50.4e6) + (314.1e6 + 50.4e6)
+MAK_TRAFFIC = (241.5e6 + 50.4e6) + (308.8e6 + 50.4e6)
 
 
 def cpu_baseline(case, target_s: float, scheme: int = 0):
