@@ -153,3 +153,39 @@ def test_brick_general_sweep_fused_phase_weighted_sums(nc, scheme, buffer):
           _run(case, m0, False, **kw))
     _same(f"weighted sums ncomp {nc} scheme {scheme} buffer {buffer}: brick vs per-voxel", brick,
           _run(case, m0, False, -2, **kw))
+
+
+@pytest.mark.parametrize("scheme", [MAK, 0])
+def test_general_sweep_long_z_chunks(scheme):
+    """The general sweep at the long z-chunks the automatic choice takes on
+    large grids (22, 32, 57 planes, and one chunk longer than the grid) with
+    per-voxel coefficients, a field and two-phase sums: the same bits as
+    8-plane chunks, in bricks and in rows."""
+    shape = (100, 18, 21)
+    case = generators.case_flow(seed=5 + scheme, shape=shape, phi=0.35, nsteps=20, big_delta=6e-4, delta=2e-4)
+    rng = np.random.default_rng(7)
+    m0 = [rng.random(shape) for _ in range(3)]
+    w1 = rng.random(shape)
+    kw = dict(scheme=scheme, coef=(rng.uniform(0.5e-9, 3e-9, shape), rng.uniform(0.2, 40.0, shape), None),
+              field=rng.uniform(-2e-5, 2e-5, shape), weights=(w1, 1.0 - w1))
+
+    def run(brick, lz):
+        n = case.nsteps
+        with HipSolver.from_case(case, scheme=scheme) as s:
+            s.set_layout(brick)
+            s.set_tile(-1, lz)
+            s.upload_magnetization(*m0[: case.ncomp])
+            s.upload_coefficients(kw["coef"][0], kw["coef"][1])
+            s.upload_field(kw["field"])
+            s.upload_phase_weights(*kw["weights"])
+            sig = s.run(n, case.grad_sched[:n])
+            return sig, s.download_magnetization()
+
+    for brick in (True, False):
+        sig8, f8 = run(brick, 8)
+        for lz in (22, 32, 57, 120):
+            sig, f = run(brick, lz)
+            for a, b in zip(f, f8):
+                assert np.array_equal(a, b), (brick, lz)
+            np.testing.assert_allclose(sig, sig8, rtol=1e-12, atol=1e-300)
+    print(f"[brick] general sweep scheme {scheme}: z-chunks 22/32/57/120 bit-identical to 8", flush=True)
