@@ -98,6 +98,8 @@ def lib():
                                            dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_fill_gradient.argtypes = [P, c_double, c_double, c_double, dp, dp, dp]
         L.or_glibc_rand.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.or_set_threads.restype = ctypes.c_int
+        L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_glibc_rand.restype = None
         L.or_fill_noise.argtypes = [P, c_double, dp]
         L.or_fill_noise.restype = None
@@ -116,6 +118,12 @@ def lib():
         L.or_output_2f.argtypes = [P, cp, c_int, c_int, c_int, F, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int = 0) -> int:
+    """Threads of the oracle's OpenMP loops from now on (n > 0); returns the
+    count in force."""
+    return int(lib().or_set_threads(int(n)))
 
 
 def _d(a):

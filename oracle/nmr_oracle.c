@@ -13,20 +13,28 @@
  * the ghost fill per axis (zero derivative, periodic, zero value).
  *
  * Parity status of each function (how it is pinned without a reference
- * build -- the reference's host path needs cuda_runtime.h and MPI, which
- * this image does not provide, so it cannot be compiled here):
+ * build -- the reference's host sources include cuda_runtime.h, which this
+ * image does not provide (an MPI toolchain is present, MPICH under
+ * /opt/conda); building them would need a stand-in for that header, so the
+ * reference counts as unbuildable here):
  *   or_explicit_simple   pinned: bulk T1/T2 closed forms, enclosed-voxel
  *                        surface rate, Neumann cosine eigenmode, 1-D upwind
- *                        brute force (both signs), precession (1-i*theta)^n,
- *                        conservation / fixed-point invariants; the
+ *                        lines along x, y and z (both signs, zero faces),
+ *                        periodic upwind Fourier modes along x, y and z,
+ *                        precession (1-i*theta)^n under x, y and z gradients,
+ *                        invariance under every permutation of the axes of a
+ *                        random anisotropic problem, conservation /
+ *                        fixed-point invariants; the
  *                        advect_mask variant: hand values of a pore next to
  *                        a flowing solid face, and agreement with the plain
  *                        step where no pore-solid face carries velocity
  *   or_maccormack        pinned: Heun factors of the relaxation-only,
- *                        precession-only and diffusion-only (Neumann cosine
- *                        and zero-value sine eigenmodes) problems; the
- *                        Lax-Wendroff amplification factor of periodic 1-D
- *                        advection; uniform fixed point under flow next to
+ *                        precession-only (z and y gradients) and
+ *                        diffusion-only (Neumann cosine and zero-value sine
+ *                        eigenmodes) problems; the Lax-Wendroff amplification
+ *                        factor of periodic 1-D advection along x, y and z;
+ *                        the two-pore channel along x, y and z; axis
+ *                        permutation invariance; uniform fixed point under flow next to
  *                        solids of arbitrary value (the race resolution);
  *                        solids zeroed after a step
  *   or_bc_field          pinned: Neumann invariants + brute copy check;
@@ -47,9 +55,15 @@
  *   or_step_time         pinned: formula arithmetic
  *   or_add_impulse       pinned: Rodrigues rotation closed form (sign
  *                        convention), 2*pi identity, norm preservation,
- *                        gradient phase linear in position
+ *                        gradient phase linear in position (flag 1); flag 3
+ *                        angle g|a||(r - 1e13 d).a_hat| measured from the
+ *                        rotated vectors (distance, its sign, the index
+ *                        offset, the |a| factor); flag 4 angle
+ *                        -g|(r - r_c)*a| measured likewise, and its
+ *                        unnormalised axis
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -704,4 +718,11 @@ void or_add_impulse(const or_params* p, const char* act, int flag, double grad, 
                     or_ort(mx, my, mz, c, nx, ny, nz, angle);
                 }
             }
+}
+
+/* Threads of the OpenMP loops from now on (n > 0), and the count in force. */
+int or_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }

@@ -1,7 +1,9 @@
 """Pins the FP64 oracle (oracle/nmr_oracle.c) without a reference build.
 
-The reference's host path cannot be compiled in this image (it needs
-cuda_runtime.h and an MPI toolchain), so each oracle function is checked
+The reference's host path cannot be compiled in this image: its sources
+include cuda_runtime.h, which the image lacks (an MPI toolchain is present,
+MPICH under /opt/conda), and a stand-in header would not be the reference.
+So each oracle function is checked
 against what the mathematics fixes: closed forms, discrete eigenfunctions,
 brute force on tiny inputs, and invariants.  Each check is chosen so that a
 plausible slip (wrong sign, swapped axis, dropped term, off-by-one index or
