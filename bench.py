@@ -15,6 +15,7 @@ times the FP64 CPU oracle on this host on a bounded number of steps of the
 same grid.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -42,8 +43,35 @@ DEFAULT_TRAFFIC = 241.2e6 + 50.4e6
 MAK_TRAFFIC = (241.5e6 + 50.4e6) + (308.8e6 + 50.4e6)
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(case, p, act, D, t1, t2, vel, m, target_s: float, step0: int):
+    """voxel-steps/s of the FP64 oracle over about target_s seconds of steps."""
+    import oracle
+
+    oracle.run(p, act, D, t1, t2, vel, 1, m, grad_sched=case.grad_sched[step0 : step0 + 1])  # warm
+    t0 = time.perf_counter()
+    oracle.run(p, act, D, t1, t2, vel, 1, m, grad_sched=case.grad_sched[step0 + 1 : step0 + 2])
+    one = time.perf_counter() - t0
+    n = max(1, min(int(target_s / max(one, 1e-6)), len(case.grad_sched) - step0 - 2))
+    t0 = time.perf_counter()
+    oracle.run(p, act, D, t1, t2, vel, n, m, grad_sched=case.grad_sched[step0 + 2 : step0 + 2 + n])
+    dt = time.perf_counter() - t0
+    return case.nvox * n / dt, n, dt
+
+
 def cpu_baseline(case, target_s: float, scheme: int = 0):
-    """FP64 oracle steps of the same grid on the host cores (bounded sample)."""
+    """FP64 oracle steps of the same grid on the host cores (bounded sample):
+    all threads (value, cores) and one thread (single_core_value)."""
     import numpy as np
 
     import oracle
@@ -51,67 +79,48 @@ def cpu_baseline(case, target_s: float, scheme: int = 0):
     p, act, D, t1, t2, vel, m = oracle.setup_case(case, m0=[case.mask.astype(np.float64),
                                                            np.zeros(case.shape), np.zeros(case.shape)],
                                                   scheme=scheme)
-    oracle.run(p, act, D, t1, t2, vel, 1, m, grad_sched=case.grad_sched[:1])  # warm (threads, pages)
-    t0 = time.perf_counter()
-    oracle.run(p, act, D, t1, t2, vel, 2, m, grad_sched=case.grad_sched[1:3])
-    one = (time.perf_counter() - t0) / 2
-    n = max(1, min(int(target_s / max(one, 1e-6)), 400))
-    t0 = time.perf_counter()
-    oracle.run(p, act, D, t1, t2, vel, n, m, grad_sched=case.grad_sched[3 : n + 3])
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    many, n_many, t_many = _oracle_rate(case, p, act, D, t1, t2, vel, m, target_s, 0)
+    oracle.set_threads(1)
+    one, n_one, t_one = _oracle_rate(case, p, act, D, t1, t2, vel, m, target_s / 2, 0)
+    oracle.set_threads(threads)
+    grid = "x".join(str(v) for v in case.shape[::-1])
     return {
-        "value": case.nvox * n / dt,
+        "value": many,
         "unit": "voxel-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle (OpenMP restatement of {'ExplicitMakUpgrade' if scheme else 'ExplicitSimple'}), "
-                  f"{'x'.join(str(v) for v in case.shape[::-1])}, {n} steps, "
-                  f"{dt:.1f} s, {threads} threads",
+                  f"{grid}: {n_many} steps in {t_many:.1f} s on {threads} threads; {n_one} steps in "
+                  f"{t_one:.1f} s on 1 thread",
+        "single_core_value": one,
+        "cpu_model": _cpu_model(),
     }
 
 
+def copy_ceiling(mib: int = 1024, reps: int = 10) -> float:
+    """GB/s of the library's own non-temporal dwordx4 copy kernel (bytes read
+    plus bytes written over its event-timed duration)."""
+    import ctypes
+
+    from initializator_amd import _lib
+
+    lib = _lib.load()
+    ms = ctypes.c_double()
+    nbytes = mib * (1 << 20)
+    rc = lib.nmr_hip_copy_bandwidth(nbytes, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"nmr_hip_copy_bandwidth: {rc}")
+    return 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
 def copy_probe(mib: int) -> int:
-    """Plain copy of a known size: its GB/s is the practical HBM ceiling, and
-    its FETCH_SIZE / WRITE_SIZE counters give the bytes-per-KiB scale."""
-    import torch
-
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 10
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    print(json.dumps({"copy_mib": mib, "ms": ms, "GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+    """The library's copy kernel on mib MiB: its GB/s is the streaming ceiling
+    on this GPU, and its FETCH_SIZE / WRITE_SIZE counters (rocprofv3 --pmc)
+    give the bytes-per-KiB scale of the sweep's counters."""
+    gbs = copy_ceiling(mib, 10)
+    print(json.dumps({"copy_mib": mib, "GBps": gbs, "ms": 2 * mib * (1 << 20) / (gbs * 1e9) * 1e3}), flush=True)
     return 0
-
-
-def _batch_instance(i: int):
-    """Instance i of the batched workload: cube edge U{32..128}, porosity
-    U(0.15, 0.40), rho U(1, 50) um/s, b log-uniform on [1e7, 1e10] s/m^2 with
-    delta 2 ms, Delta 20 ms, divergence-free velocity (max 1 mm/s)."""
-    from initializator_amd import generators
-
-    seed = 0x4E4D52 + 4000 + i
-    r = np.random.default_rng(seed)
-    n = int(r.integers(32, 129))
-    phi = float(r.uniform(0.15, 0.40))
-    rho = float(r.uniform(1e-6, 50e-6))
-    b = float(10.0 ** r.uniform(7.0, 10.0))
-    shape = (n, n, n)
-    mask = generators.porosity(shape, phi, seed)
-    vel = generators.velocity(mask, seed, vmax=1e-3, d=(1e-6, 1e-6, 1e-6))
-    g = generators.gradient_for_b(b, 2e-3, 20e-3)
-    return dict(shape=shape, phi=phi, rho1=rho, rho2=rho, grad_amp=g, delta=2e-3, big_delta=20e-3,
-                mask=mask, vel=vel)
 
 
 def instance_errors(specs, k, nsteps, dx, dt, D):
@@ -152,14 +161,10 @@ def instance_errors(specs, k, nsteps, dx, dt, D):
 
 def batch_main(args) -> int:
     """Batched workload: many uneven instances advanced together."""
-    from concurrent.futures import ProcessPoolExecutor
-
     from initializator_amd import generators
     from initializator_amd.solver import BatchSolver
 
-    workers = max(1, min(16, os.cpu_count() or 1))
-    with ProcessPoolExecutor(workers) as ex:  # before any GPU call in this process
-        specs = list(ex.map(_batch_instance, range(args.instances)))
+    specs = generators.batch_instances(args.instances)
     print(f"[bench] generated {len(specs)} instances", file=sys.stderr, flush=True)
     dx, D = 1e-6, 2.3e-9
     dt = generators.stable_dt(dx, D)
@@ -247,16 +252,12 @@ def learned_main(args) -> int:
     then the time to produce the signal curve with coarse steps of 8 dt
     (corrector, residual check, plain fallback) against the plain solve of
     the same problem, and the curves' relative L2 difference."""
-    from concurrent.futures import ProcessPoolExecutor
-
     from initializator_amd import generators
     from initializator_amd.solver import BatchSolver, Corrector, HipSolver
 
     factor = 8
-    workers = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    with ProcessPoolExecutor(workers) as ex:  # before any GPU call in this process
-        specs = list(ex.map(_batch_instance, range(args.instances)))
+    specs = generators.batch_instances(args.instances)
     case = generators.case_flow(n=args.n, phi=args.phi, nsteps=args.steps)
     gen_s = time.perf_counter() - t0
     print(f"[bench] generated {len(specs)} training instances and the {args.n}^3 case in {gen_s:.1f} s",
@@ -351,7 +352,8 @@ def main() -> int:
     ap.add_argument("--scheme", choices=("upwind", "maccormack"), default="upwind",
                     help="plain workload: ExplicitSimple's upwind step (default) or the reference GPU loop's "
                          "MacCormack step")
-    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU baseline sample: seconds of oracle steps on all threads (half that on one thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("plain", "batch", "learned"), default="plain",
                     help="plain: one 256^3 grid (default); batch: many uneven instances per launch; "
@@ -399,18 +401,25 @@ def main() -> int:
                                 progress=lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
     s = HipSolver.from_case(case, scheme=_lib.NMR_SCHEME_MACCORMACK if mak else _lib.NMR_SCHEME_UPWIND)
     s.set_tile(args.variant, args.lz)
+    # the upload converts the state to the sweep's brick layout and sizes its
+    # buffers; the state then stays on the device, in bricks, from run to run
     s.upload_magnetization(case.mask.astype("float64"), np.zeros(case.shape))
     s.run(args.warmup, case.grad_sched[: args.warmup], want_signal=False)
     t0 = time.perf_counter()
     s.run(args.steps, case.grad_sched[args.warmup :], want_signal=True)
     wall = time.perf_counter() - t0
+    # HIP events on the solver's stream around the timed run's kernels (the
+    # sweep launches and one reduction of their partial sums)
     gpu_ms, nsw = s.last_run_ms()
     s.close()
+    ceiling = copy_ceiling()
 
     nvox = case.nvox
     value = nvox * args.steps / wall
     kernel_s = gpu_ms * 1e-3 / args.steps
     achieved = bpv * nvox / kernel_s / 1e9
+    default = (shape, args.phi, args.variant, args.lz, args.rho) == ((256, 256, 256), 0.20, -1, 0, 10e-6)
+    traffic = (MAK_TRAFFIC if mak else DEFAULT_TRAFFIC) if default else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -437,15 +446,25 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": ((MAK_TRAFFIC if mak else DEFAULT_TRAFFIC)
-                        if (shape, args.phi, args.variant, args.lz, args.rho)
-                        == ((256, 256, 256), 0.20, -1, 0, 10e-6) else None),
+            "traffic": traffic,
+            # traffic is not measured by this run: rocprofv3 --pmc FETCH_SIZE /
+            # WRITE_SIZE of the same command, scaled by a plain copy's counters
+            "traffic_source": (f"profiles/{'gsweep_maccormack' if mak else 'sweep_kernel'}_256cube_flow_pgse.md"
+                               if traffic else None),
+            # measured DRAM bytes over the same kernel time, against 8 TB/s
+            "dram_frac": traffic / kernel_s / (HBM_PEAK_GBS * 1e9) if traffic else None,
             "kernel_us": kernel_s * 1e6,
+            "sweeps_per_step": nsw / args.steps,
+            # the library's own non-temporal 16-byte copy kernel on this GPU
+            "copy_ceiling_GBps": ceiling,
+            "frac_of_copy_ceiling": achieved / ceiling,
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        cb = generators.case_flow(shape=shape, phi=args.phi, nsteps=404, rho=args.rho)
+        # the same grid and physics, with a schedule long enough for the sample
+        cb = dataclasses.replace(case, nsteps=404, grad_sched=generators.pgse_schedule(404, case.dt, 0.3, 2e-3,
+                                                                                       20e-3))
         out["cpu_baseline"] = cpu_baseline(cb, args.cpu_seconds, scheme=1 if mak else 0)
     print(json.dumps(out), flush=True)
     return 0

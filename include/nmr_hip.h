@@ -174,26 +174,36 @@ int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay
 int nmr_hip_device_count(int32_t* count);
 int nmr_hip_set_device(int32_t device);
 
-/* Pore sums of the current state: out[ncomp] (out[2*ncomp] with phase weights). */
+/* Pore sums of the current state: out[ncomp] (out[2*ncomp] with phase
+ * weights); the last row of the last run when nothing has changed the state
+ * since, else a sum over the state (OutputX/Y/Z, cpp:3301-3461). */
 int nmr_hip_signal(nmr_ctx* ctx, double* out);
+
+/* Measurement, no reference counterpart: average GPU time (ms, HIP events)
+ * of `reps` device-to-device copies of `bytes` bytes by the library's own
+ * non-temporal 16-byte-per-lane copy kernel -- the streaming ceiling the
+ * sweeps are compared with.  Allocates and frees its two buffers. */
+int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms);
 
 /* GPU time (ms) between HIP events recorded on the context's stream around
  * the kernels of the last nmr_hip_run, and the number of sweep launches. */
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
 /* Device layout of the sweeps (no reference counterpart): 1 (the default)
- * keeps a copy of the state and the velocity in bricks of 128 bytes (8
- * x-voxels by 4 rows of one plane) for runs of 16 steps or more -- the
- * one-step sweep with any one-step tile variant, the general sweep with
- * brick copies of the per-voxel coefficients, field and phase weights --
- * so that all-solid bricks are never fetched, converting the state in and
- * out once per run (the velocity and the per-voxel arrays once per upload);
- * 0 always sweeps the row layout.  The fields are the same bits either way. */
+ * keeps the state, the velocity and the per-voxel coefficients, field and
+ * phase weights in bricks of 128 bytes (8 x-voxels by 4 rows of one plane)
+ * -- for the one-step sweep with any one-step tile variant and for the
+ * general sweep -- so that all-solid bricks are never fetched.  The uploads
+ * convert their arrays once; the state stays in bricks from run to run
+ * (pulses rotate it there) and is converted back only for a download or a
+ * learned solve.  0 always sweeps the row layout.  The fields are the same
+ * bits either way. */
 int nmr_hip_set_layout(nmr_ctx* ctx, int32_t brick);
 /* Tile shape of the sweep kernel: variant index (-1 = automatic, -2 = a
  * one-thread-per-voxel kernel with the arithmetic of the sweeps, a
- * cross-check) and z-chunk length (0 = automatic: the length in 6..12
- * that best fills whole rounds of the workgroups the device holds at once).  nmr_hip_num_variants() gives the variant count;
+ * cross-check) and z-chunk length (0 = automatic: the length in 6..64
+ * that best fills whole rounds of the workgroups the device holds at
+ * once, 8 on ties).  nmr_hip_num_variants() gives the variant count;
  * nmr_hip_variant_info() describes one: lanes per row, rows per thread, tile
  * width and height, and time steps per pass (2 = temporally blocked sweep,
  * usable when nx <= tile_x).  Every variant gives bit-identical fields. */

@@ -216,9 +216,13 @@ void signal_window(const nmr_problem& p, int w[6])
     w[5] = p.nz - p.buffer[2];
 }
 
+// The partial and signal buffers grow to at least twice their capacity, so
+// that runs of growing length reallocate rarely; uploads pre-size them for
+// the automatic launch (presize_buffers).
 int ensure_partials(nmr_ctx* c, size_t doubles)
 {
     if (doubles <= c->partials_cap) return NMR_OK;
+    doubles = std::max(doubles, 2 * c->partials_cap);
     if (c->partials) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipFree(c->partials));
@@ -235,6 +239,7 @@ int ensure_partials(nmr_ctx* c, size_t doubles)
 int ensure_signal(nmr_ctx* c, size_t doubles)
 {
     if (doubles <= c->dsignal_cap) return NMR_OK;
+    doubles = std::max(doubles, 2 * c->dsignal_cap);
     if (c->dsignal) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipFree(c->dsignal));
@@ -256,7 +261,13 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
         return fail(NMR_ERR_STATE, "signal buffer holds %zu values, row %zu needs %zu", c->dsignal_cap, row,
                     (row + 1) * (size_t)c->nsig());
     nmr::FieldSumArgs a{};
-    for (int q = 0; q < 3; q++) a.f[q] = c->m[c->cur][q];
+    // the state where it lives (rows preferred)
+    const bool rows = (c->home & nmri::kHomeRows) != 0;
+    for (int q = 0; q < 3; q++) a.f[q] = rows ? c->m[c->cur][q] : c->mb[c->bpar][q];
+    if (!rows) {
+        a.nby = (p.ny + 3) / 4;
+        a.nb8 = (c->pitch / 4 + 1) / 2;
+    }
     a.mask = c->mask;
     if (c->weighted) {
         a.w[0] = c->w[0];
@@ -284,7 +295,65 @@ int refresh_inert(nmr_ctx* c);
 bool bricks_fit(const nmr_ctx* c);
 int ensure_bricks(nmr_ctx* c);
 int ensure_pv_bricks(nmr_ctx* c);
-extern const int64_t kBrickMinSteps;
+int state_bricks(nmr_ctx* c);
+
+// Launch geometry of the general sweep: 256 threads, tiles of 4*lanes x
+// (256/lanes) voxels, z-chunks of lz planes (the round model unless set).
+struct GGeom {
+    int lanes, lz;
+    long ntx, nty, ntz, nwg;
+};
+GGeom general_geometry(const nmr_ctx* c, bool gbrick)
+{
+    const nmr_problem& p = c->prob;
+    GGeom g{};
+    g.lanes = gbrick ? 16 : nmr::gsweep_lanes(p.nx);
+    const int tx = 4 * g.lanes, ty = 256 / g.lanes;
+    g.ntx = (p.nx + tx - 1) / tx;
+    g.nty = (p.ny + ty - 1) / ty;
+    int lz = c->lz > 0 ? c->lz
+                       : round_lz(g.ntx * g.nty, p.nz,
+                                  nmr::gsweep_resident_blocks(c->nc, c->vel, c->hetero, g.lanes, gbrick,
+                                                              c->weighted));
+    g.lz = std::min(std::max(lz, 1), p.nz);
+    g.ntz = (p.nz + g.lz - 1) / g.lz;
+    g.nwg = g.ntx * g.nty * g.ntz;
+    return g;
+}
+
+// Launch geometry of the plain sweep: L1 for the run, L2 for an odd last
+// step of a two-step run; *brick: whether the sweep runs on the bricks
+// (one-step tiles, the layout switch on, every brick array addressable).
+void plain_geometry(const nmr_ctx* c, Launch* L1, Launch* L2, bool* brick)
+{
+    const nmr_problem& p = c->prob;
+    auto_launch(c, c->variant, c->lz, L1);
+    const bool want = c->brick_on && bricks_fit(c);
+    if (want) auto_launch(c, c->variant < 0 ? nmr::brick_auto_variant(p.nx) : c->variant, c->lz, L1, true);
+    auto_launch(c, L1->steps == 2 ? 1 : L1->variant, L1->steps == 2 ? 0 : L1->lz, L2);
+    *brick = want && L1->steps == 1 && nmr::brick_variant(L1->variant);
+}
+
+// Copies a run's signal rows to the host -- row 0 from the kept signal of
+// the current state when there was one (sig0) -- and keeps the last row as
+// the signal of the new current state.  Waits for the stream.
+int finish_signal(nmr_ctx* c, int64_t nsteps, bool have0, const double* sig0, double* signal)
+{
+    const size_t ns = (size_t)c->nsig(), first = have0 ? 1 : 0;
+    if (signal && (size_t)nsteps + 1 > first)
+        HIP_TRY(hipMemcpyAsync(signal + first * ns, c->dsignal + first * ns,
+                               sizeof(double) * ns * ((size_t)nsteps + 1 - first), hipMemcpyDeviceToHost, c->stream));
+    if (!signal && (nsteps > 0 || !have0))
+        HIP_TRY(hipMemcpyAsync(c->sig_now, c->dsignal + (size_t)nsteps * ns, sizeof(double) * ns,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (signal) {
+        if (have0) std::copy(sig0, sig0 + ns, signal);
+        std::copy(signal + (size_t)nsteps * ns, signal + ((size_t)nsteps + 1) * ns, c->sig_now);
+    }
+    c->sig_valid = true;
+    return NMR_OK;
+}
 
 // Steps of the general problems (MacCormack, non-zero-derivative boundaries,
 // per-voxel coefficients, two-phase sums): the general z-sweep, or with
@@ -306,8 +375,27 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
             for (int q = 0; q < 3; q++) g[q] = grad_sched[3 * step + q];
         return make_coef(ctx, g);
     };
+    // the general sweep runs on the bricks (state, velocity, per-voxel
+    // arrays and the MacCormack predictor) whenever they are addressable
+    const bool gbrick = !pervoxel && ctx->brick_on && bricks_fit(ctx);
+    GGeom g{};
+    if (!pervoxel) {
+        g = general_geometry(ctx, gbrick);
+        if (g.nwg > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
+    }
+    // each step's signal: fused into the sweep (plain sums; weighted sums
+    // on bricks), else a field sum after the step
+    const bool fused = !pervoxel && (!ctx->weighted || gbrick);
+    if (fused && (rc = ensure_partials(ctx, (size_t)std::min<int64_t>(nsteps, kChunkSteps) * g.nwg * ns))) return rc;
+    if ((rc = gbrick ? state_bricks(ctx) : nmri::state_rows(ctx))) return rc;
+    if (gbrick && (rc = ensure_pv_bricks(ctx))) return rc;
+    const bool have0 = ctx->sig_valid;
+    double sig0[6];
+    std::copy(ctx->sig_now, ctx->sig_now + 6, sig0);
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+    if (!have0 && (rc = enqueue_field_sum(ctx, 0))) return rc;
+    // from here on the sweeps write one layout only
+    ctx->home = gbrick ? nmri::kHomeBricks : nmri::kHomeRows;
     if (pervoxel) {
         nmr::GenArgs a{};
         a.mask = ctx->mask;
@@ -347,94 +435,54 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         // first step's output buffer is cleared at solids before the step and
         // its input buffer after it (the first step still reads the input's
         // solid values, e.g. through periodic ghosts).
+        auto out_buf = [&]() { return gbrick ? ctx->mb[ctx->bpar ^ 1] : ctx->m[ctx->cur ^ 1]; };
+        auto zero_solids = [&](float* const* m) {
+            return gbrick ? nmr::launch_zero_solids_brick(m, nc, ctx->mask, p.ny, p.nz, ctx->pitch, ctx->stream)
+                          : nmr::launch_zero_solids(m, nc, ctx->mask, ctx->n_m, ctx->stream);
+        };
         const bool zero_first = mak && nsteps > 0 && !ctx->solids_zeroed;
-        if (zero_first) HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
-        // brick layout (long runs): the state in both brick buffers -- the
-        // output one from the row buffer whose solids were just cleared --,
-        // for MacCormack a brick predictor, and brick copies of the velocity
-        // and of the per-voxel arrays
-        const bool gbrick = ctx->brick_on && nsteps >= kBrickMinSteps && bricks_fit(ctx);
-        const int c0 = ctx->cur;
-        const int nc4 = ctx->pitch / 4;
-        if (gbrick) {
-            if ((rc = ensure_bricks(ctx))) return rc;
-            if (mak)
-                for (int q = 0; q < nc; q++)
-                    if (!ctx->pb[q] && (rc = dev_alloc(ctx, &ctx->pb[q], nmr::brick_floats(nc4, p.ny, p.nz))))
-                        return rc;
-            for (int q = 0; q < nc; q++) {
-                HIP_TRY(nmr::launch_brick(ctx->m[c0][q], ctx->mb[0][q], nc4, p.ny, p.nz, true, ctx->stream));
-                HIP_TRY(nmr::launch_brick(ctx->m[zero_first ? c0 ^ 1 : c0][q], ctx->mb[1][q], nc4, p.ny, p.nz, true,
-                                          ctx->stream));
-            }
-        }
-        int bpar = 0;
-        // tile geometry: 256 threads, 4*lanes x (256/lanes) voxels, z-chunks
-        // of lz planes (8 unless set)
-        const int lanes = gbrick ? 16 : nmr::gsweep_lanes(p.nx);
-        const int tx = 4 * lanes, ty = 256 / lanes;
-        const long ntx = (p.nx + tx - 1) / tx, nty = (p.ny + ty - 1) / ty;
-        int lz = ctx->lz > 0 ? ctx->lz
-                             : round_lz(ntx * nty, p.nz,
-                                        nmr::gsweep_resident_blocks(nc, ctx->vel, ctx->hetero, lanes, gbrick));
-        lz = std::min(std::max(lz, 1), p.nz);
-        const long ntz = (p.nz + lz - 1) / lz;
-        const long nwg = ntx * nty * ntz;
-        if (nwg > (1L << 30)) return fail(NMR_ERR_INVALID, "too many workgroups");
-        // each step's signal: fused into the sweep (plain sums; weighted sums
-        // on bricks), else a field sum after the step
-        const bool fused = !ctx->weighted || gbrick;
-        if (fused && (rc = ensure_partials(ctx, (size_t)kChunkSteps * nwg * ns))) return rc;
+        if (zero_first) HIP_TRY(zero_solids(out_buf()));
         int w[6];
         signal_window(p, w);
         nmr::GSweepArgs a{};
         a.mask = ctx->mask;
-        a.vx = ctx->vx;
-        a.vy = ctx->vy;
-        a.vz = ctx->vz;
-        a.dn = ctx->dn;
-        a.r1v = ctx->r1v;
-        a.r2v = ctx->r2v;
-        a.bn = ctx->bn;
+        a.vx = gbrick ? ctx->vxb : ctx->vx;
+        a.vy = gbrick ? ctx->vyb : ctx->vy;
+        a.vz = gbrick ? ctx->vzb : ctx->vz;
+        // per-voxel arrays: their brick copies on bricks
+        a.dn = ctx->dn ? (gbrick ? ctx->pvb[0] : ctx->dn) : nullptr;
+        a.r1v = ctx->r1v ? (gbrick ? ctx->pvb[1] : ctx->r1v) : nullptr;
+        a.r2v = ctx->r2v ? (gbrick ? ctx->pvb[2] : ctx->r2v) : nullptr;
+        a.bn = ctx->bn ? (gbrick ? ctx->pvb[3] : ctx->bn) : nullptr;
         a.bnax = ctx->bnax;
+        if (gbrick && ctx->weighted) {
+            a.w[0] = ctx->pvb[4];
+            a.w[1] = ctx->pvb[5];
+        }
         a.nx = p.nx;
         a.ny = p.ny;
         a.nz = p.nz;
         a.pitch = ctx->pitch;
         a.pitch1 = ctx->pitch1;
-        a.ntx = (int)ntx;
-        a.nty = (int)nty;
-        a.ntz = (int)ntz;
-        a.lz = lz;
-        a.nblocks = (int)nwg;
+        a.ntx = (int)g.ntx;
+        a.nty = (int)g.nty;
+        a.ntz = (int)g.ntz;
+        a.lz = g.lz;
+        a.nblocks = (int)g.nwg;
         a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
         for (int q = 0; q < 3; q++) a.bc[q] = p.bc[q];
         a.skip_m = ctx->inert;
         a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         a.nby = (p.ny + 3) / 4;
         a.nbyv = (p.ny + 4) / 4;
-        if (gbrick) { // velocity and per-voxel arrays in bricks too
-            if ((rc = ensure_pv_bricks(ctx))) return rc;
-            a.vx = ctx->vxb;
-            a.vy = ctx->vyb;
-            a.vz = ctx->vzb;
-            a.dn = ctx->dn ? ctx->pvb[0] : nullptr;
-            a.r1v = ctx->r1v ? ctx->pvb[1] : nullptr;
-            a.r2v = ctx->r2v ? ctx->pvb[2] : nullptr;
-            a.bn = ctx->bn ? ctx->pvb[3] : nullptr;
-            if (ctx->weighted) {
-                a.w[0] = ctx->pvb[4];
-                a.w[1] = ctx->pvb[5];
-            }
-        }
         const bool coef = ctx->hetero;
         int64_t done = 0;
         while (done < nsteps) {
             const int chunk = (int)std::min<int64_t>(kChunkSteps, nsteps - done);
             for (int s = 0; s < chunk; s++) {
                 a.c = coef_of(done + s);
-                float* const* cin = gbrick ? ctx->mb[bpar] : ctx->m[ctx->cur];
-                float* const* cout = gbrick ? ctx->mb[bpar ^ 1] : ctx->m[ctx->cur ^ 1];
+                float* const* cin = gbrick ? ctx->mb[ctx->bpar] : ctx->m[ctx->cur];
+                float* const* cout = out_buf();
                 float* const* pred = gbrick ? ctx->pb : ctx->pred;
                 if (mak) {
                     for (int q = 0; q < 3; q++) {
@@ -442,54 +490,38 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
                         a.out[q] = pred[q];
                     }
                     a.partials = nullptr;
-                    HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, nmr::GEN_PREDICT, coef, lanes, a, ctx->stream, gbrick));
+                    HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, nmr::GEN_PREDICT, coef, g.lanes, a, ctx->stream, gbrick));
                 }
                 for (int q = 0; q < 3; q++) {
                     a.f[q] = mak ? pred[q] : cin[q];
                     a.mavg[q] = cin[q];
                     a.out[q] = cout[q];
                 }
-                a.partials = fused ? ctx->partials + (size_t)s * nwg * ns : nullptr;
-                HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, mak ? nmr::GEN_CORRECT : nmr::GEN_UPWIND, coef, lanes, a,
+                a.partials = fused ? ctx->partials + (size_t)s * g.nwg * ns : nullptr;
+                HIP_TRY(nmr::launch_gsweep(nc, ctx->vel, mak ? nmr::GEN_CORRECT : nmr::GEN_UPWIND, coef, g.lanes, a,
                                            ctx->stream, gbrick));
-                ctx->cur ^= 1;
-                bpar ^= 1;
-                if (zero_first && done + s == 0) {
-                    if (gbrick) // the next output buffer: the cleared row buffer's solids
-                        for (int q = 0; q < nc; q++)
-                            HIP_TRY(nmr::launch_brick(ctx->m[c0 ^ 1][q], ctx->mb[bpar ^ 1][q], nc4, p.ny, p.nz, true,
-                                                      ctx->stream));
-                    else
-                        HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
+                if (gbrick) ctx->bpar ^= 1;
+                else ctx->cur ^= 1;
+                if (zero_first && done + s == 0) { // the next output buffer: the first step's input
+                    HIP_TRY(zero_solids(out_buf()));
                     ctx->solids_zeroed = true;
                 }
                 if (!fused && (rc = enqueue_field_sum(ctx, (size_t)(done + s) + 1))) return rc;
             }
             if (fused)
-                HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)nwg, ns,
+                HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, (int)g.nwg, ns,
                                                     ctx->dsignal + (size_t)(done + 1) * ns, ctx->stream));
             done += chunk;
         }
-        if (gbrick) { // the final state back to the current row buffer
-            for (int q = 0; q < nc; q++)
-                HIP_TRY(nmr::launch_brick(ctx->mb[bpar][q], ctx->m[ctx->cur][q], nc4, p.ny, p.nz, false, ctx->stream));
-            if (zero_first) // both row buffers hold 0 at solids from now on
-                HIP_TRY(nmr::launch_zero_solids(ctx->m[ctx->cur ^ 1], nc, ctx->mask, ctx->n_m, ctx->stream));
-        }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-    if (signal)
-        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * ns, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = finish_signal(ctx, nsteps, have0, sig0, signal))) return rc;
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
     ctx->last_sweeps = nsteps * (mak ? 2 : 1);
     return NMR_OK;
 }
-
-extern const int64_t kBrickMinSteps = 16; // shorter runs keep the row layout (3 conversions per run)
 
 // Whether every brick array stays inside the sweep's 32-bit byte offsets.
 bool bricks_fit(const nmr_ctx* c)
@@ -501,8 +533,9 @@ bool bricks_fit(const nmr_ctx* c)
     return big * 4 < ((size_t)1 << 32) - 256;
 }
 
-// Brick copies of the state (both ping-pong buffers) and of the velocity,
-// allocated on the first brick run.
+// Brick buffers of the state (both ping-pong buffers and, for MacCormack,
+// the predictor) and the brick copy of the velocity, allocated and converted
+// by the uploads (or the first brick run after the layout switch).
 int ensure_bricks(nmr_ctx* c)
 {
     const nmr_problem& p = c->prob;
@@ -511,6 +544,12 @@ int ensure_bricks(nmr_ctx* c)
         for (int q = 0; q < c->nc; q++)
             if (!c->mb[b][q]) {
                 int rc = dev_alloc(c, &c->mb[b][q], nmr::brick_floats(nbx, p.ny, p.nz));
+                if (rc) return rc;
+            }
+    if (p.scheme == NMR_SCHEME_MACCORMACK)
+        for (int q = 0; q < c->nc; q++)
+            if (!c->pb[q]) {
+                int rc = dev_alloc(c, &c->pb[q], nmr::brick_floats(nbx, p.ny, p.nz));
                 if (rc) return rc;
             }
     if (c->vel && !c->vxb) {
@@ -545,6 +584,65 @@ int ensure_pv_bricks(nmr_ctx* c)
         HIP_TRY(nmr::launch_brick(src[i], c->pvb[i], nbx, p.ny, p.nz, true, c->stream));
     }
     c->brick_pv_valid = true;
+    return NMR_OK;
+}
+
+// The current state into both brick buffers, from the row buffer m[cur],
+// unless the bricks already hold it.
+int state_bricks(nmr_ctx* c)
+{
+    if (c->home & nmri::kHomeBricks) return NMR_OK;
+    int rc = ensure_bricks(c);
+    if (rc) return rc;
+    const nmr_problem& p = c->prob;
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < c->nc; q++)
+            HIP_TRY(nmr::launch_brick(c->m[c->cur][q], c->mb[b][q], c->pitch / 4, p.ny, p.nz, true, c->stream));
+    c->bpar = 0;
+    c->home |= nmri::kHomeBricks;
+    return NMR_OK;
+}
+
+// Partial-sum and signal buffers sized for the launch the next run takes
+// (chunks of kChunkSteps steps, and a field sum of the current state).
+int presize_buffers(nmr_ctx* c)
+{
+    const nmr_problem& p = c->prob;
+    const size_t ns = (size_t)c->nsig();
+    size_t need = 0;
+    if (c->general) {
+        if (c->variant != -2) need = (size_t)kChunkSteps * general_geometry(c, c->brick_on && bricks_fit(c)).nwg * ns;
+    } else {
+        Launch L1{}, L2{};
+        bool brick = false;
+        plain_geometry(c, &L1, &L2, &brick);
+        if (L1.nwg > 0 && L2.nwg > 0)
+            need = (size_t)kChunkSteps * std::max(L1.nwg, L2.nwg) * ns;
+    }
+    nmr::FieldSumArgs f{};
+    int w[6];
+    signal_window(p, w);
+    f.bx0 = w[0]; f.bx1 = w[1]; f.by0 = w[2]; f.by1 = w[3]; f.bz0 = w[4]; f.bz1 = w[5];
+    need = std::max(need, (size_t)nmr::field_sum_blocks(f) * ns);
+    int rc = ensure_partials(c, need);
+    if (!rc) rc = ensure_signal(c, (size_t)(kChunkSteps + 1) * ns);
+    return rc;
+}
+
+// The one-time work of a solve, done by the uploads so that runs start at
+// once: the inert check, the brick buffers with the velocity and per-voxel
+// arrays converted, the partial-sum and signal buffers.
+int prepare(nmr_ctx* c)
+{
+    if (!c->has_mask) return NMR_OK;
+    int rc = refresh_inert(c);
+    if (rc) return rc;
+    if (c->brick_on && bricks_fit(c)) {
+        if ((rc = ensure_bricks(c))) return rc;
+        if ((rc = ensure_pv_bricks(c))) return rc;
+    }
+    if ((rc = presize_buffers(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return NMR_OK;
 }
 
@@ -747,9 +845,10 @@ int nmr_hip_upload_mask(nmr_ctx* ctx, const uint8_t* mask, int32_t layout)
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->has_mask = true;
     ctx->solids_zeroed = false;
+    ctx->sig_valid = false;
     ctx->mask_version += 1;
     ctx->inert_dirty = true;
-    return NMR_OK;
+    return prepare(ctx);
 }
 
 int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, const double* vz, int32_t layout)
@@ -804,7 +903,7 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
     }
     ctx->inert_dirty = true;
     ctx->brick_vel_valid = false;
-    return NMR_OK;
+    return prepare(ctx);
 }
 
 int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* my, const double* mz, int32_t layout)
@@ -835,6 +934,13 @@ int nmr_hip_upload_magnetization(nmr_ctx* ctx, const double* mx, const double* m
     }
     ctx->has_mag = true;
     ctx->solids_zeroed = false;
+    ctx->sig_valid = false;
+    ctx->home = nmri::kHomeRows;
+    ctx->cur = 0;
+    if ((rc = prepare(ctx))) return rc;
+    // the sweeps keep the state in bricks: converted here, once per upload
+    if (ctx->brick_on && bricks_fit(ctx) && (rc = state_bricks(ctx))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return NMR_OK;
 }
 
@@ -894,7 +1000,7 @@ int nmr_hip_upload_coefficients(nmr_ctx* ctx, const double* D, const double* r1,
     ctx->d_ref = dref;
     ctx->hetero = true;
     ctx->general = true;
-    return NMR_OK;
+    return prepare(ctx);
 }
 
 int nmr_hip_upload_field(nmr_ctx* ctx, const double* b, int32_t layout)
@@ -924,7 +1030,7 @@ int nmr_hip_upload_field(nmr_ctx* ctx, const double* b, int32_t layout)
     if ((rc = put_voxels(ctx, &ctx->bn, v, 1.0))) return rc;
     ctx->bnax = ax;
     ctx->general = true;
-    return NMR_OK;
+    return prepare(ctx);
 }
 
 int nmr_hip_upload_phase_weights(nmr_ctx* ctx, const double* w1, const double* w2, int32_t layout)
@@ -937,7 +1043,8 @@ int nmr_hip_upload_phase_weights(nmr_ctx* ctx, const double* w1, const double* w
     if ((rc = put_voxels(ctx, &ctx->w[0], a, 1.0)) || (rc = put_voxels(ctx, &ctx->w[1], b, 1.0))) return rc;
     ctx->weighted = true;
     ctx->general = true;
-    return NMR_OK;
+    ctx->sig_valid = false; // signal rows now hold 2*ncomp sums
+    return prepare(ctx);
 }
 
 int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double* mz, int32_t layout)
@@ -947,6 +1054,7 @@ int nmr_hip_download_magnetization(nmr_ctx* ctx, double* mx, double* my, double*
     if (rc) return rc;
     double* dst[3] = {mx, my, mz};
     const nmr_problem& p = ctx->prob;
+    if ((rc = nmri::state_rows(ctx))) return rc;
     std::vector<float> h(ctx->n_m);
     for (int q = 0; q < ctx->nc; q++) {
         if (!dst[q]) continue;
@@ -1006,11 +1114,16 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out)
 {
     if (!ctx || !out) return fail(NMR_ERR_NULL, "nmr_hip_signal: null argument");
     if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "signal before mask and magnetization upload");
-    int rc = ensure_signal(ctx, (size_t)ctx->nsig());
-    if (rc) return rc;
-    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
-    HIP_TRY(hipMemcpyAsync(out, ctx->dsignal, sizeof(double) * ctx->nsig(), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int ns = ctx->nsig();
+    if (!ctx->sig_valid) { // a field sum of the state where it lives
+        int rc = ensure_signal(ctx, (size_t)ns);
+        if (rc) return rc;
+        if ((rc = enqueue_field_sum(ctx, 0))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->sig_now, ctx->dsignal, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->sig_valid = true;
+    }
+    std::copy(ctx->sig_now, ctx->sig_now + ns, out);
     return NMR_OK;
 }
 
@@ -1026,7 +1139,14 @@ int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay
         return fail(NMR_ERR_UNSUPPORTED, "ncomp 2 takes rotations about z only");
     const nmr_problem& p = ctx->prob;
     nmr::PulseArgs a{};
-    for (int q = 0; q < 3; q++) a.m[q] = ctx->m[ctx->cur][q];
+    // on the layout that holds the state (bricks preferred); pores only, so
+    // the other buffer of that layout keeps agreeing at all-solid chunks
+    const bool brick = (ctx->home & nmri::kHomeBricks) != 0;
+    for (int q = 0; q < 3; q++) a.m[q] = brick ? ctx->mb[ctx->bpar][q] : ctx->m[ctx->cur][q];
+    if (brick) {
+        a.nby = (p.ny + 3) / 4;
+        a.nb8 = (ctx->pitch / 4 + 1) / 2;
+    }
     a.mask = ctx->mask;
     a.nx = p.nx;
     a.ny = p.ny;
@@ -1044,7 +1164,48 @@ int nmr_hip_pulse(nmr_ctx* ctx, int32_t flag, double param, double ax, double ay
     a.gamma = p.gamma;
     HIP_TRY(nmr::launch_pulse(a, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->home = brick ? nmri::kHomeBricks : nmri::kHomeRows;
+    ctx->sig_valid = false;
     return NMR_OK;
+}
+
+int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms)
+{
+    if (!ms) return fail(NMR_ERR_NULL, "nmr_hip_copy_bandwidth: null argument");
+    if (bytes < 4096 || reps < 1) return fail(NMR_ERR_INVALID, "bytes %lld, reps %d", (long long)bytes, (int)reps);
+    const size_t n = (size_t)bytes / 16;
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(cus, 1) * 16, (n + 1023) / 1024));
+    float *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = NMR_OK;
+    auto check = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == NMR_OK)
+            rc = fail(e == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+        return rc == NMR_OK;
+    };
+    if (check(hipMalloc((void**)&src, n * 16), "hipMalloc") && check(hipMalloc((void**)&dst, n * 16), "hipMalloc") &&
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
+        check(hipEventCreate(&e0), "hipEventCreate") && check(hipEventCreate(&e1), "hipEventCreate") &&
+        check(hipMemsetAsync(src, 0, n * 16, st), "hipMemsetAsync") &&
+        check(nmr::launch_copy(src, dst, n, grid, st), "copy") && check(nmr::launch_copy(src, dst, n, grid, st), "copy") &&
+        check(hipEventRecord(e0, st), "hipEventRecord")) {
+        for (int r = 0; r < reps && rc == NMR_OK; r++) check(nmr::launch_copy(src, dst, n, grid, st), "copy");
+        float t = 0.f;
+        if (check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "hipEventSynchronize") &&
+            check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))
+            *ms = t / reps;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 int nmr_hip_device_count(int32_t* count)
@@ -1088,33 +1249,25 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         }
     }
     if (ctx->general || ctx->variant == -2) return run_general(ctx, nsteps, grad_sched, signal);
-    int rc0 = refresh_inert(ctx);
-    if (rc0) return rc0;
+    int rc = refresh_inert(ctx);
+    if (rc) return rc;
     Launch L1{}, L2{};
-    auto_launch(ctx, ctx->variant, ctx->lz, &L1);
-    const bool want_brick = ctx->brick_on && nsteps >= kBrickMinSteps && bricks_fit(ctx);
-    if (want_brick && ctx->variant < 0) auto_launch(ctx, nmr::brick_auto_variant(p.nx), ctx->lz, &L1, true);
-    else if (want_brick) auto_launch(ctx, ctx->variant, ctx->lz, &L1, true);
-    if (L1.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
+    bool brick = false;
+    plain_geometry(ctx, &L1, &L2, &brick);
+    if (L1.nwg < 0 || L2.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
     if (L1.steps == 2 && p.nx > L1.tx)
         return fail(NMR_ERR_INVALID, "variant %d needs nx <= %d", L1.variant, L1.tx);
-    // one-step launch for an odd last step of a two-step run
-    auto_launch(ctx, L1.steps == 2 ? 1 : L1.variant, L1.steps == 2 ? 0 : L1.lz, &L2);
     const int nc = ctx->nc;
 
     const int64_t rows = nsteps + 1;
-    int rc = ensure_signal(ctx, (size_t)rows * nc);
-    if (rc) return rc;
-    rc = ensure_partials(ctx, std::max({(size_t)kChunkSteps * L1.nwg * nc, (size_t)kChunkSteps * L2.nwg * nc,
-                                        (size_t)p.nz * nc}));
-    if (rc) return rc;
+    if ((rc = ensure_signal(ctx, (size_t)rows * nc))) return rc;
+    if ((rc = ensure_partials(ctx, (size_t)std::min<int64_t>(nsteps, kChunkSteps) * std::max(L1.nwg, L2.nwg) * nc)))
+        return rc;
+    // the one-step sweep keeps the state in 8x4 bricks from run to run
+    if ((rc = brick ? state_bricks(ctx) : nmri::state_rows(ctx))) return rc;
 
     int w[6];
     signal_window(p, w);
-    // the one-step sweep runs on 8x4 bricks when it is long enough to pay
-    // for converting the state in and out (the velocity is converted once)
-    const bool brick = want_brick && L1.steps == 1 && nmr::brick_variant(L1.variant);
-    if (brick && (rc = ensure_bricks(ctx))) return rc;
     nmr::SweepArgs a{};
     a.mask = ctx->mask;
     a.vx = brick ? ctx->vxb : ctx->vx;
@@ -1151,16 +1304,12 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         return make_coef(ctx, g);
     };
 
+    const bool have0 = ctx->sig_valid;
+    double sig0[6];
+    std::copy(ctx->sig_now, ctx->sig_now + 6, sig0);
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    if ((rc = enqueue_field_sum(ctx, 0))) return rc;
-    // brick state: both brick buffers start as the current state (all-solid
-    // chunks are never written, so both must hold their values)
-    int bpar = 0;
-    if (brick)
-        for (int b = 0; b < 2; b++)
-            for (int q = 0; q < nc; q++)
-                HIP_TRY(nmr::launch_brick(ctx->m[ctx->cur][q], ctx->mb[b][q], ctx->pitch / 4, p.ny, p.nz, true,
-                                          ctx->stream));
+    if (!have0 && (rc = enqueue_field_sum(ctx, 0))) return rc;
+    ctx->home = brick ? nmri::kHomeBricks : nmri::kHomeRows; // the sweeps write this layout only
     int64_t done = 0;
     while (done < nsteps) {
         const int64_t left = nsteps - done;
@@ -1194,28 +1343,21 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
             for (int s = 0; s < chunk; s++) {
                 a.c = coef_of(done + s);
                 for (int q = 0; q < 3; q++) {
-                    a.min[q] = brick ? ctx->mb[bpar][q] : ctx->m[ctx->cur][q];
-                    a.mout[q] = brick ? ctx->mb[bpar ^ 1][q] : ctx->m[ctx->cur ^ 1][q];
+                    a.min[q] = brick ? ctx->mb[ctx->bpar][q] : ctx->m[ctx->cur][q];
+                    a.mout[q] = brick ? ctx->mb[ctx->bpar ^ 1][q] : ctx->m[ctx->cur ^ 1][q];
                 }
                 a.partials = ctx->partials + (size_t)s * L.nwg * nc;
                 HIP_TRY(nmr::launch_sweep(nc, ctx->vel, ctx->exact, L.variant, a, ctx->stream, brick));
-                ctx->cur ^= 1;
-                bpar ^= 1;
+                if (brick) ctx->bpar ^= 1;
+                else ctx->cur ^= 1;
             }
         }
         HIP_TRY(nmr::launch_reduce_partials(ctx->partials, chunk, L.nwg, nc, ctx->dsignal + (size_t)(done + 1) * nc,
                                             ctx->stream));
         done += chunk;
     }
-    if (brick) // the final state back to the row layout of the current buffer
-        for (int q = 0; q < nc; q++)
-            HIP_TRY(nmr::launch_brick(ctx->mb[bpar][q], ctx->m[ctx->cur][q], ctx->pitch / 4, p.ny, p.nz, false,
-                                      ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-    if (signal)
-        HIP_TRY(hipMemcpyAsync(signal, ctx->dsignal, sizeof(double) * (size_t)rows * nc, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = finish_signal(ctx, nsteps, have0, sig0, signal))) return rc;
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
@@ -1857,6 +1999,17 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
 // Entry points for the other host files of the library.
 
 namespace nmri {
+
+int state_rows(nmr_ctx* c)
+{
+    if (c->home & kHomeRows) return NMR_OK;
+    const nmr_problem& p = c->prob;
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < c->nc; q++)
+            HIP_TRY(nmr::launch_brick(c->mb[c->bpar][q], c->m[b][q], c->pitch / 4, p.ny, p.nz, false, c->stream));
+    c->home |= kHomeRows;
+    return NMR_OK;
+}
 
 nmr::Coef make_coef(const nmr_ctx* c, const double g[3]) { return ::make_coef(c, g); }
 

@@ -19,6 +19,7 @@
 #include "nmr_kernels.h"
 
 namespace nmri {
+constexpr int kHomeRows = 1, kHomeBricks = 2;
 // Record a status detail for nmr_hip_last_error() and return code.
 int fail(int code, const char* fmt, ...);
 inline int round_up4(int n) { return (n + 3) & ~3; }
@@ -73,6 +74,18 @@ struct nmr_ctx {
     // w[0], w[1]; converted after each upload, before the next brick run
     float* pvb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool brick_pv_valid = false;
+    // Which copies hold the current state: the row buffer m[cur]
+    // (nmri::kHomeRows), the brick buffer mb[bpar] (nmri::kHomeBricks), or
+    // both.  A conversion writes both buffers of its target layout, so the
+    // two buffers of a layout agree wherever a sweep does not write
+    // (all-solid chunks); a sweep or a pulse leaves its own layout as the
+    // only home of the state.
+    int home = 1;
+    int bpar = 0;
+    // signal of the current state: host copy of the last row of the last run
+    // (or of the last nmr_hip_signal); cleared when the state changes otherwise
+    bool sig_valid = false;
+    double sig_now[6] = {0, 0, 0, 0, 0, 0};
     int nsig() const { return weighted ? 2 * nc : nc; }
     uint8_t* mask = nullptr;
     float* vx = nullptr;
@@ -146,4 +159,7 @@ int ensure_partials(nmr_ctx* c, size_t doubles);
 // of the context (the automatic one-step tile), on its stream; the step's
 // signal partials go to the context's partial buffer.
 int sweep_step(nmr_ctx* c, const nmr::Coef& cf, bool exact, int in, int out);
+// Make the row buffers hold the current state (both of them, converted from
+// the bricks when only those hold it).
+int state_rows(nmr_ctx* c);
 } // namespace nmri

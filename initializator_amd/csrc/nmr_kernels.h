@@ -147,21 +147,23 @@ struct GSweepArgs {
 };
 
 struct PulseArgs {
-    float* m[3];
+    float* m[3];            // row layout, or bricks when nb8 > 0
     const uint8_t* mask;
     int nx, ny, nz, pitch, nc;
+    int nby, nb8;           // brick layout: brick rows per plane, bricks per brick row (0: rows)
     int flag;
     double param, ax, ay, az;
     double dx, dy, dz, gamma;
 };
 
 struct FieldSumArgs {
-    const float* f[3];
+    const float* f[3];      // row layout, or bricks when nb8 > 0
     const uint8_t* mask;
     const float* w[2];      // phase weights (null: plain sums)
     double* part;           // [field_sum_blocks][nc] or, with weights, [..][2*nc]
     int nx, ny, nz, pitch;
     int bx0, bx1, by0, by1, bz0, bz1;
+    int nby, nb8;           // brick layout of f (0: rows)
 };
 
 // ---- learned corrector (nmr_learn_kernels.hip) ----
@@ -268,10 +270,11 @@ hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args&
 hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s);
 int batch_resident_blocks(int nc, bool vel, bool exact, int variant);
 // Workgroups of the one-step sweep (variant 0..6, row or brick layout) and of
-// the general sweep's upwind / corrector kernel resident on the device at
+// the general sweep's upwind / corrector kernel (on bricks with fused
+// phase-weighted sums: its weighted instantiation) resident on the device at
 // once (CUs x occupancy); -1 if the query fails.
 int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick);
-int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick);
+int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, bool weighted);
 hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
                                const int* step_base, int row0, int64_t rows, double* signal, int nsteps, int ninst,
                                hipStream_t s);
@@ -286,6 +289,11 @@ hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, cons
                          bool brick = false);
 // m[q][i] = 0 where mask[i] == 0, i < n, for q < nc
 hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s);
+// the same for fields in bricks (nz planes of ny rows of pitch voxels)
+// dst = src over nchunks 16-byte chunks (16-byte aligned), non-temporal
+hipError_t launch_copy(const float* src, float* dst, size_t nchunks, int grid, hipStream_t s);
+hipError_t launch_zero_solids_brick(float* const* m, int nc, const uint8_t* mask, int ny, int nz, int pitch,
+                                    hipStream_t s);
 // workgroups (= partials) of a field sum over the signal window
 int field_sum_blocks(const FieldSumArgs& a);
 hipError_t launch_field_sum(int nc, const FieldSumArgs& a, hipStream_t s);

@@ -80,6 +80,15 @@ __device__ __forceinline__ float hmean(float a, float b)
     return s > 1e-30f ? (2.f * a * b) * __builtin_amdgcn_rcpf(s) : 0.f;
 }
 
+// Float offset of voxel (x, y, z) of a field in bricks of 8 x-voxels by 4
+// rows (brick (z, y/4, x/8); nby brick rows per plane, nb8 bricks per brick
+// row), the layout brick_kernel converts to.
+__device__ __forceinline__ size_t brick_off(int x, int y, int z, int nby, int nb8)
+{
+    return (((size_t)z * (size_t)nby + (size_t)(y >> 2)) * (size_t)nb8 + (size_t)(x >> 3)) * 32 +
+           (size_t)((y & 3) * 8 + (x & 7));
+}
+
 // byte e (0/1) of a packed mask word as a float
 __device__ __forceinline__ float ub(uint32_t w, int e) { return (float)((w >> (8 * e)) & 0xffu); }
 
@@ -1198,6 +1207,7 @@ __global__ void __launch_bounds__(256) pulse_kernel(PulseArgs a)
     if (x >= a.nx) return;
     const size_t o = ((size_t)z * a.ny + y) * a.pitch + x;
     if (a.mask[o] == 0) return;
+    const size_t fo = a.nb8 > 0 ? brick_off(x, y, z, a.nby, a.nb8) : o; // state offset
     double len = sqrt(a.ax * a.ax + a.ay * a.ay + a.az * a.az);
     if (len == 0.0) len = 1.0;
     const double n0x = a.ax / len, n0y = a.ay / len, n0z = a.az / len;
@@ -1224,11 +1234,11 @@ __global__ void __launch_bounds__(256) pulse_kernel(PulseArgs a)
         rx = a.ax; ry = a.ay; rz = a.az;
     }
     const double c = cos(angle), s = sin(angle);
-    const double vx = a.m[0][o], vy = a.m[1][o], vz = a.nc == 3 ? (double)a.m[2][o] : 0.0;
+    const double vx = a.m[0][fo], vy = a.m[1][fo], vz = a.nc == 3 ? (double)a.m[2][fo] : 0.0;
     const double scal = rx * vx + ry * vy + rz * vz;
-    a.m[0][o] = (float)(vx * c + rx * (1.0 - c) * scal + s * (ry * vz - rz * vy));
-    a.m[1][o] = (float)(vy * c + ry * (1.0 - c) * scal + s * (rz * vx - rx * vz));
-    if (a.nc == 3) a.m[2][o] = (float)(vz * c + rz * (1.0 - c) * scal + s * (rx * vy - ry * vx));
+    a.m[0][fo] = (float)(vx * c + rx * (1.0 - c) * scal + s * (ry * vz - rz * vy));
+    a.m[1][fo] = (float)(vy * c + ry * (1.0 - c) * scal + s * (rz * vx - rx * vz));
+    if (a.nc == 3) a.m[2][fo] = (float)(vz * c + rz * (1.0 - c) * scal + s * (rx * vy - ry * vx));
 }
 
 hipError_t launch_pulse(const PulseArgs& a, hipStream_t s)
@@ -1280,17 +1290,18 @@ __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
         const size_t row = ((size_t)z * (size_t)a.ny + (size_t)y) * (size_t)a.pitch;
         for (int x = a.bx0 + lane; x < a.bx1; x += 64) {
             if (a.mask[row + x] == 0) continue;
+            const size_t fo = a.nb8 > 0 ? brick_off(x, y, z, a.nby, a.nb8) : row + x;
             if constexpr (W) {
                 const double w1 = a.w[0][row + x], w2 = a.w[1][row + x];
 #pragma unroll
                 for (int q = 0; q < NC; q++) {
-                    const double v = a.f[q][row + x];
+                    const double v = a.f[q][fo];
                     acc[q] += v * w1;
                     acc[NC + q] += v * w2;
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][row + x];
+                for (int q = 0; q < NC; q++) acc[q] += (double)a.f[q][fo];
             }
         }
     }
@@ -1856,6 +1867,52 @@ __global__ void __launch_bounds__(256) zero_solids_kernel(float* m0, float* m1, 
     }
 }
 
+// zero_solids_kernel for fields in bricks: voxel i of the row order (mask
+// index) is cleared at its brick offset.
+__global__ void __launch_bounds__(256) zero_solids_brick_kernel(float* m0, float* m1, float* m2, int nc,
+                                                                const uint8_t* __restrict__ mask, int ny, int pitch,
+                                                                size_t n)
+{
+    const int nby = (ny + 3) / 4, nb8 = (pitch / 4 + 1) / 2;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        if (mask[i] != 0) continue;
+        const int x = (int)(i % (size_t)pitch);
+        const size_t r = i / (size_t)pitch;
+        const size_t o = brick_off(x, (int)(r % (size_t)ny), (int)(r / (size_t)ny), nby, nb8);
+        m0[o] = 0.f;
+        if (nc > 1) m1[o] = 0.f;
+        if (nc > 2) m2[o] = 0.f;
+    }
+}
+
+// Copy of n 16-byte chunks with non-temporal dwordx4 loads and stores, a
+// grid-stride loop of 4 chunks per thread per pass with all four loads in
+// flight before the stores: the bandwidth a streaming kernel of this project
+// can reach on the device (the ceiling the sweeps are compared with).
+__global__ void __launch_bounds__(256) copy_kernel(const f4* __restrict__ src, f4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const f4 a = __builtin_nontemporal_load(src + i);
+        const f4 b = __builtin_nontemporal_load(src + i + stride);
+        const f4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const f4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+hipError_t launch_copy(const float* src, float* dst, size_t nchunks, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const f4*>(src),
+                       reinterpret_cast<f4*>(dst), nchunks);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 
@@ -2099,12 +2156,15 @@ int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
     return resident_blocks(f);
 }
 
-int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick)
+int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, bool weighted)
 {
-    // the upwind / corrector kernel (the predictor holds fewer registers)
+    // the upwind / corrector kernel (the predictor holds fewer registers);
+    // on bricks with fused phase-weighted sums its instantiation with 2*NC
+    // accumulators
     const void* f = nullptr;
 #define NMR_PICK(NC_, V_, C_)                                                                   \
-    if (brick) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true>;               \
+    if (brick && weighted) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true, true>; \
+    else if (brick) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true>;          \
     else if (lanes == 64) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 64>;          \
     else if (lanes == 32) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 32>;          \
     else return -1;
@@ -2229,7 +2289,7 @@ static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
                          bool brick)
 {
-    if (brick) { // lanes 16; the coefficient arrays stay in the row layout
+    if (brick) { // lanes 16; the per-voxel arrays are read from their brick copies
         if (lanes != 16) return hipErrorInvalidValue;
 #define NMR_GSB(NC_)                                                                         \
     if (coef) return vel ? launch_gs_brick<NC_, true, true>(stage, a, s)                      \
@@ -2254,6 +2314,16 @@ hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, cons
     default: return hipErrorInvalidValue;
     }
 #undef NMR_GSC
+}
+
+hipError_t launch_zero_solids_brick(float* const* m, int nc, const uint8_t* mask, int ny, int nz, int pitch,
+                                    hipStream_t s)
+{
+    const size_t n = (size_t)nz * ny * pitch;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(zero_solids_brick_kernel, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
+                       m[0], m[1], m[2], nc, mask, ny, pitch, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s)

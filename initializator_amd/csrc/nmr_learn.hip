@@ -705,6 +705,23 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
     const int nab = nmr::apply_blocks(p.nx, p.ny, p.nz);
     if ((rc = ensure_dev_doubles(&c->apart, &c->apart_cap, (size_t)nab * 6))) return rc;
     if ((rc = build_rows(c, ctx))) return rc;
+    // The loop works on the row buffers (the coarse step, the net, the
+    // statistics and the fallback runs); it leaves the state there and the
+    // context's layout switch as it found it.
+    if ((rc = nmri::state_rows(ctx))) return rc;
+    struct RowsOnly {
+        nmr_ctx* c;
+        bool keep;
+        ~RowsOnly()
+        {
+            c->brick_on = keep;
+            c->home = nmri::kHomeRows;
+            c->sig_valid = false;
+        }
+    } guard{ctx, ctx->brick_on};
+    ctx->brick_on = false;
+    ctx->home = nmri::kHomeRows;
+    ctx->sig_valid = false;
     nmr::ApplyArgs ap{};
     ap.mask = ctx->mask;
     ap.part = c->apart;
@@ -792,6 +809,7 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
             // plain steps from the kept state (buffer `in` is untouched)
             std::vector<double> gs;
             if (grad_sched) gs.assign(grad_sched + 3 * k * factor, grad_sched + 3 * (k + 1) * factor);
+            ctx->sig_valid = false; // accepted coarse steps changed the state
             if ((rc = nmr_hip_run(ctx, factor, grad_sched ? gs.data() : nullptr, sig8.data()))) return rc;
             if ((rc = stats(ctx->cur, nullptr, cur))) return rc;
         }

@@ -19,6 +19,7 @@ produce inputs of the same meaning:
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -178,3 +179,35 @@ def case_flow(seed: int = 0x4E4D52 + 1000, n: int = 256, phi: float = 0.20, g: f
     dt = stable_dt(dx, D)
     sched = pgse_schedule(nsteps, dt, g, delta, big_delta)
     return Case(shape, mask, vx, vy, vz, dx, dt, D, 2.0, 2.5e10, rho, nsteps, sched, 2)
+
+
+def batch_instance(i: int) -> dict:
+    """Instance i of the batched workload: cube edge U{32..128}, porosity
+    U(0.15, 0.40), rho U(1, 50) um/s, b log-uniform on [1e7, 1e10] s/m^2 with
+    delta 2 ms and Delta 20 ms, divergence-free pore velocity (max 1 mm/s),
+    1 um voxels; seed 0x4E4D52 + 4000 + i."""
+    seed = 0x4E4D52 + 4000 + i
+    r = np.random.default_rng(seed)
+    n = int(r.integers(32, 129))
+    phi = float(r.uniform(0.15, 0.40))
+    rho = float(r.uniform(1e-6, 50e-6))
+    b = float(10.0 ** r.uniform(7.0, 10.0))
+    shape = (n, n, n)
+    mask = porosity(shape, phi, seed)
+    vel = velocity(mask, seed, vmax=1e-3, d=(1e-6, 1e-6, 1e-6))
+    g = gradient_for_b(b, 2e-3, 20e-3)
+    return dict(shape=shape, phi=phi, rho1=rho, rho2=rho, grad_amp=g, delta=2e-3, big_delta=20e-3, b=b,
+                mask=mask, vel=vel)
+
+
+def batch_instances(n: int = 256, workers: int = 16) -> list:
+    """The first n batched-workload instances, generated in a pool of worker
+    processes started fresh (spawn: safe after the GPU is in use)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    workers = max(1, min(workers, os.cpu_count() or 1, n))
+    if workers == 1:
+        return [batch_instance(i) for i in range(n)]
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(batch_instance, range(n)))

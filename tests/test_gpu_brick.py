@@ -189,3 +189,49 @@ def test_general_sweep_long_z_chunks(scheme):
                 assert np.array_equal(a, b), (brick, lz)
             np.testing.assert_allclose(sig, sig8, rtol=1e-12, atol=1e-300)
     print(f"[brick] general sweep scheme {scheme}: z-chunks 22/32/57/120 bit-identical to 8", flush=True)
+
+
+@pytest.mark.parametrize("scheme", [0, MAK])
+def test_resident_state_short_runs_pulses_and_signal(scheme):
+    """The state stays on the device in the sweep's layout from run to run:
+    runs of 3, 7, 1 and 5 steps with RF, gradient and radial pulses between
+    them, signal queries and a download in the middle.  On bricks and on rows
+    the fields are the same bits after every stage and the per-step signals
+    agree to FP64 summation order; row 0 of a run is the signal of the state
+    it starts from (the previous run's last row, or the sum after a pulse)."""
+    shape = (13, 22, 37)
+    case = generators.case_flow(seed=91 + scheme, shape=shape, phi=0.35, nsteps=16, big_delta=6e-4, delta=2e-4)
+    rng = np.random.default_rng(3)
+    m0 = [np.where(case.mask != 0, 1.0, rng.random(shape)), rng.random(shape) - 0.5]
+    g = case.grad_sched
+
+    def stages(brick):
+        out = []
+        with HipSolver.from_case(case, scheme=scheme) as s:
+            s.set_layout(brick)
+            s.upload_magnetization(*m0)
+            a = s.run(3, g[0:3])
+            assert np.array_equal(s.signal(), a[-1])  # kept, not recomputed
+            s.pulse(2, np.pi / 3, (0.0, 0.0, 1.0))
+            p = s.signal()
+            b = s.run(7, g[3:10])
+            assert np.array_equal(b[0], p)
+            out.append(s.download_magnetization())
+            s.pulse(1, 2e-4, (0.0, 0.0, 1.0))
+            c = s.run(1, g[10:11])
+            s.pulse(4, 0.0, (0.0, 0.0, 1.0))
+            d = s.run(5, g[11:16])
+            e = s.run(0)
+            assert np.array_equal(e[0], d[-1])
+            out.append(s.download_magnetization())
+            sigs = [a, b, c, d]
+        return out, sigs
+
+    fb, sb = stages(True)
+    fr, sr = stages(False)
+    for stage_b, stage_r in zip(fb, fr):
+        for x, y in zip(stage_b, stage_r):
+            assert np.array_equal(x, y)
+    for x, y in zip(sb, sr):
+        np.testing.assert_allclose(x, y, rtol=1e-12, atol=1e-300)
+    print(f"[brick] resident state, scheme {scheme}: short runs, pulses, signals: bricks == rows", flush=True)
