@@ -159,6 +159,20 @@ int nmr_hip_upload_phase_weights(nmr_ctx* ctx, const double* w1, const double* w
  * after n updates. */
 int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal);
 
+/* nmr_hip_run without device allocation.  No counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same steps,
+ * fields and signal rows as nmr_hip_run, but the per-workgroup partial sums
+ * and the signal rows live in a fixed 8 MiB device array of the library
+ * (one per GPU, held until the process ends) instead of buffers of the
+ * context: a run is split into sub-runs whose partial sums and rows fit it.
+ * Every other buffer must exist already (the uploads create them; a buffer
+ * that a later layout or scheme switch would need gives NMR_ERR_STATE), and
+ * a grid whose one step needs more partial sums than the array holds gives
+ * NMR_ERR_INVALID.  All contexts on one GPU share the array: calls that
+ * overlap on one GPU (two host threads) race. */
+int nmr_hip_run_static(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal);
+
 /* Instantaneous pulse on every pore voxel, AddImpulse semantics
  * (src/NmrRelaxationSolve.cpp:2552-2608, rotation OrtTransform cpp:2536-2550):
  *   flag 1: gradient pulse, angle = param*g*((i-1)dx*ax + (j-1)dy*ay + (k-1)dz*az)

@@ -41,7 +41,34 @@ constexpr int kChunkSteps = 512; // steps whose partials are held before a reduc
 
 
 
+// Fixed scratch of nmr_hip_run_static: the per-workgroup partial sums and
+// the signal rows of a run in a file-scope device array (kScratchDoubles
+// doubles, 8 MiB on every device the library runs on, held until the process
+// ends), for callers that cannot allocate device memory.
 namespace {
+constexpr size_t kScratchDoubles = (size_t)1 << 20;
+}
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr; // device -> address of its copy of scratch
+
+// Address of `device`'s copy of the scratch array, with that device current;
+// taken once per device (by the first context created there) and kept.
+int scratch_address(int device, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        double* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        it = g_scratch_ptr.emplace(device, ptr).first;
+    }
+    *out = it->second;
+    return NMR_OK;
+}
 
 int check_layout(int32_t layout)
 {
@@ -53,6 +80,7 @@ int check_layout(int32_t layout)
 template <typename T>
 int dev_alloc(nmr_ctx* c, T** p, size_t count)
 {
+    if (c->no_alloc) return fail(NMR_ERR_STATE, "a device buffer is missing and this entry point allocates nothing");
     const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
     HIP_TRY(hipMalloc((void**)p, bytes));
     HIP_TRY(hipMemsetAsync(*p, 0, bytes, c->stream));
@@ -222,6 +250,9 @@ void signal_window(const nmr_problem& p, int w[6])
 int ensure_partials(nmr_ctx* c, size_t doubles)
 {
     if (doubles <= c->partials_cap) return NMR_OK;
+    if (c->no_alloc)
+        return fail(NMR_ERR_INVALID, "partial sums need %zu doubles, the fixed scratch holds %zu", doubles,
+                    c->partials_cap);
     doubles = std::max(doubles, 2 * c->partials_cap);
     if (c->partials) {
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -239,6 +270,9 @@ int ensure_partials(nmr_ctx* c, size_t doubles)
 int ensure_signal(nmr_ctx* c, size_t doubles)
 {
     if (doubles <= c->dsignal_cap) return NMR_OK;
+    if (c->no_alloc)
+        return fail(NMR_ERR_INVALID, "signal rows need %zu doubles, the fixed scratch holds %zu", doubles,
+                    c->dsignal_cap);
     doubles = std::max(doubles, 2 * c->dsignal_cap);
     if (c->dsignal) {
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -777,11 +811,12 @@ int nmr_hip_create(const nmr_problem* prob, nmr_ctx** out)
     };
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) return bail(fail(NMR_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e)));
+    int rc = scratch_address(c->device, &c->scratch);
+    if (rc) return bail(rc);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(NMR_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(NMR_ERR_DEVICE, "hipEventCreate failed"));
-    int rc;
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < c->nc; q++)
             if ((rc = dev_alloc(c, &c->m[b][q], c->n_m))) return bail(rc);
@@ -1363,6 +1398,69 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     ctx->last_ms = ms;
     ctx->last_sweeps = L1.steps == 2 ? nsteps / 2 + (nsteps & 1) : nsteps;
     return NMR_OK;
+}
+
+int nmr_hip_run_static(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* signal)
+{
+    if (!ctx) return fail(NMR_ERR_NULL, "nmr_hip_run_static: null context");
+    if (nsteps < 0) return fail(NMR_ERR_INVALID, "nsteps %lld", (long long)nsteps);
+    if (!ctx->has_mask || !ctx->has_mag) return fail(NMR_ERR_STATE, "run before mask and magnetization upload");
+    if (!ctx->scratch) return fail(NMR_ERR_STATE, "no scratch address on this device");
+    const nmr_problem& p = ctx->prob;
+    const size_t ns = (size_t)ctx->nsig();
+    // partial sums per step of the sweep this run takes (none for the
+    // per-voxel kernel, whose signal is a field sum after each step)
+    size_t per_step = 0;
+    if (ctx->general) {
+        if (ctx->variant != -2) per_step = (size_t)general_geometry(ctx, ctx->brick_on && bricks_fit(ctx)).nwg * ns;
+    } else {
+        Launch L1{}, L2{};
+        bool brick = false;
+        plain_geometry(ctx, &L1, &L2, &brick);
+        if (L1.nwg < 0 || L2.nwg < 0) return fail(NMR_ERR_INVALID, "too many workgroups");
+        per_step = (size_t)std::max(L1.nwg, L2.nwg) * ns;
+    }
+    nmr::FieldSumArgs f{};
+    int w[6];
+    signal_window(p, w);
+    f.bx0 = w[0]; f.bx1 = w[1]; f.by0 = w[2]; f.by1 = w[3]; f.bz0 = w[4]; f.bz1 = w[5];
+    const size_t fsum = (size_t)nmr::field_sum_blocks(f) * ns;
+    // steps per sub-run: their partials (or one field sum's) and their
+    // signal rows must fit the scratch
+    int64_t C = std::min<int64_t>(kChunkSteps, (int64_t)((kScratchDoubles - ns) / (per_step + ns)));
+    while (C >= 1 && std::max((size_t)C * per_step, fsum) + (size_t)(C + 1) * ns > kScratchDoubles) C--;
+    if (C < 1)
+        return fail(NMR_ERR_INVALID,
+                    "grid too large for the fixed scratch: one step needs %zu doubles of partial sums and a field sum "
+                    "%zu, the scratch holds %zu",
+                    per_step + 2 * ns, fsum + 2 * ns, kScratchDoubles);
+    const size_t P = std::max((size_t)C * per_step, fsum);
+    double* const keep_p = ctx->partials;
+    double* const keep_s = ctx->dsignal;
+    const size_t keep_pc = ctx->partials_cap, keep_sc = ctx->dsignal_cap;
+    ctx->partials = ctx->scratch;
+    ctx->partials_cap = P;
+    ctx->dsignal = ctx->scratch + P;
+    ctx->dsignal_cap = (size_t)(C + 1) * ns;
+    ctx->no_alloc = true;
+    double ms = 0.0;
+    int64_t sweeps = 0, done = 0;
+    int rc = NMR_OK;
+    do { // sub-runs of at most C steps; each waits for its stream before the next reuses the scratch
+        const int64_t n = std::min<int64_t>(C, nsteps - done);
+        rc = nmr_hip_run(ctx, n, grad_sched ? grad_sched + 3 * done : nullptr, signal ? signal + done * ns : nullptr);
+        ms += ctx->last_ms;
+        sweeps += ctx->last_sweeps;
+        done += n;
+    } while (rc == NMR_OK && done < nsteps);
+    ctx->partials = keep_p;
+    ctx->partials_cap = keep_pc;
+    ctx->dsignal = keep_s;
+    ctx->dsignal_cap = keep_sc;
+    ctx->no_alloc = false;
+    ctx->last_ms = ms;
+    ctx->last_sweeps = sweeps;
+    return rc;
 }
 
 } // extern "C"

@@ -105,6 +105,10 @@ struct nmr_ctx {
     double last_ms = 0.0;
     int64_t last_sweeps = 0;
     int64_t device_bytes = 0;
+    // set while nmr_hip_run_static runs: every device buffer must exist
+    // already and the partial-sum and signal buffers are the fixed scratch
+    bool no_alloc = false;
+    double* scratch = nullptr; // this device's fixed scratch (nmr_hip_run_static)
 };
 
 struct nmr_batch {
