@@ -195,8 +195,9 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out);
 
 /* Measurement, no reference counterpart: average GPU time (ms, HIP events)
  * of `reps` device-to-device copies of `bytes` bytes by the library's own
- * non-temporal 16-byte-per-lane copy kernel -- the streaming ceiling the
- * sweeps are compared with.  Allocates and frees its two buffers. */
+ * 16-byte-per-lane copy kernels, the fastest of its four shapes (grid-stride
+ * or one pass, plain or non-temporal) -- the streaming ceiling the sweeps are
+ * compared with.  Allocates and frees its two buffers. */
 int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms);
 
 /* GPU time (ms) between HIP events recorded on the context's stream around
@@ -255,6 +256,11 @@ int nmr_hip_batch_destroy(nmr_batch* b);
 int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const double* vx, const double* vy,
                          const double* vz, const double* mx, const double* my, const double* mz);
 int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, double* mz);
+/* Device layout of the batch (no reference counterpart): 1 (the default)
+ * keeps every instance's state and velocity in bricks of 8 x-voxels by 4
+ * rows from run to run, as nmr_hip_set_layout does for one domain; 0 sweeps
+ * the row layout.  The fields are the same bits either way. */
+int nmr_hip_batch_set_layout(nmr_batch* b, int32_t brick);
 /* signal (nullable): [ninst][nsteps+1][ncomp]; step counting continues
  * across calls (the gradient schedule sees steps 0, 1, ... since create). */
 int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal);

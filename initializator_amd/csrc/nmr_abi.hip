@@ -1212,7 +1212,7 @@ int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms)
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int grid = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(cus, 1) * 16, (n + 1023) / 1024));
+    cus = std::max(cus, 1);
     float *src = nullptr, *dst = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1222,18 +1222,23 @@ int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms)
             rc = fail(e == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
         return rc == NMR_OK;
     };
+    double best = 0.0;
     if (check(hipMalloc((void**)&src, n * 16), "hipMalloc") && check(hipMalloc((void**)&dst, n * 16), "hipMalloc") &&
         check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
         check(hipEventCreate(&e0), "hipEventCreate") && check(hipEventCreate(&e1), "hipEventCreate") &&
-        check(hipMemsetAsync(src, 0, n * 16, st), "hipMemsetAsync") &&
-        check(nmr::launch_copy(src, dst, n, grid, st), "copy") && check(nmr::launch_copy(src, dst, n, grid, st), "copy") &&
-        check(hipEventRecord(e0, st), "hipEventRecord")) {
-        for (int r = 0; r < reps && rc == NMR_OK; r++) check(nmr::launch_copy(src, dst, n, grid, st), "copy");
-        float t = 0.f;
-        if (check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "hipEventSynchronize") &&
-            check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))
-            *ms = t / reps;
+        check(hipMemsetAsync(src, 0, n * 16, st), "hipMemsetAsync")) {
+        // every copy shape: two untimed copies, then reps timed ones; the fastest shape's time
+        for (int kind = 0; kind < nmr::copy_kinds() && rc == NMR_OK; kind++) {
+            for (int r = 0; r < 2 && rc == NMR_OK; r++) check(nmr::launch_copy(kind, src, dst, n, cus, st), "copy");
+            check(hipEventRecord(e0, st), "hipEventRecord");
+            for (int r = 0; r < reps && rc == NMR_OK; r++) check(nmr::launch_copy(kind, src, dst, n, cus, st), "copy");
+            float t = 0.f;
+            if (check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "hipEventSynchronize") &&
+                check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))
+                if (best == 0.0 || t / reps < best) best = t / reps;
+        }
     }
+    if (rc == NMR_OK) *ms = best;
     if (st) (void)hipStreamSynchronize(st);
     if (src) (void)hipFree(src);
     if (dst) (void)hipFree(dst);
@@ -1480,6 +1485,11 @@ void batch_free(nmr_batch* b)
         for (int p = 0; p < 2; p++)
             for (int q = 0; q < 3; q++)
                 if (I.m[p][q]) (void)hipFree(I.m[p][q]);
+        for (int p = 0; p < 2; p++)
+            for (int q = 0; q < 3; q++)
+                if (I.mb[p][q]) (void)hipFree(I.mb[p][q]);
+        for (float* q : {I.vxb, I.vyb, I.vzb})
+            if (q) (void)hipFree(q);
         if (I.mask) (void)hipFree(I.mask);
         if (I.vx) (void)hipFree(I.vx);
         if (I.vy) (void)hipFree(I.vy);
@@ -1502,6 +1512,70 @@ int balloc(T** p, size_t count)
 }
 
 constexpr int kMixedVariant = -3; // every instance takes its own tile shape
+
+// Whether every brick array of every instance stays inside the sweep's
+// 32-bit byte offsets.
+bool batch_bricks_fit(const nmr_batch* b)
+{
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        const auto& s = b->spec[i];
+        const int nbx = b->inst[i].pitch / 4, nbx1 = b->inst[i].pitch1 / 4;
+        const size_t big = std::max({nmr::brick_floats(nbx, s.ny, s.nz), nmr::brick_floats(nbx1, s.ny, s.nz),
+                                     nmr::brick_floats(nbx, s.ny + 1, s.nz), nmr::brick_floats(nbx, s.ny, s.nz + 1)});
+        if (big * 4 >= ((size_t)1 << 32) - 256) return false;
+    }
+    return true;
+}
+
+// Brick buffers of instance i (allocated once) with its velocity converted.
+int batch_inst_bricks(nmr_batch* b, int i)
+{
+    const auto& s = b->spec[i];
+    auto& I = b->inst[i];
+    const int nbx = I.pitch / 4, nbx1 = I.pitch1 / 4;
+    int rc;
+    for (int p = 0; p < 2; p++)
+        for (int q = 0; q < b->nc; q++)
+            if (!I.mb[p][q] && (rc = balloc(&I.mb[p][q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
+    if (b->vel) {
+        if (!I.vxb && ((rc = balloc(&I.vxb, nmr::brick_floats(nbx1, s.ny, s.nz))) ||
+                       (rc = balloc(&I.vyb, nmr::brick_floats(nbx, s.ny + 1, s.nz))) ||
+                       (rc = balloc(&I.vzb, nmr::brick_floats(nbx, s.ny, s.nz + 1)))))
+            return rc;
+        if (!I.vel_bricks) {
+            HIP_TRY(nmr::launch_brick(I.vx, I.vxb, nbx1, s.ny, s.nz, true, b->stream));
+            HIP_TRY(nmr::launch_brick(I.vy, I.vyb, nbx, s.ny + 1, s.nz, true, b->stream));
+            HIP_TRY(nmr::launch_brick(I.vz, I.vzb, nbx, s.ny, s.nz + 1, true, b->stream));
+            I.vel_bricks = true;
+        }
+    }
+    return NMR_OK;
+}
+
+// Instance i's current row state into both of its brick buffers.
+int batch_inst_to_bricks(nmr_batch* b, int i)
+{
+    int rc = batch_inst_bricks(b, i);
+    if (rc) return rc;
+    const auto& s = b->spec[i];
+    auto& I = b->inst[i];
+    for (int p = 0; p < 2; p++)
+        for (int q = 0; q < b->nc; q++)
+            HIP_TRY(nmr::launch_brick(I.m[b->parity][q], I.mb[p][q], I.pitch / 4, s.ny, s.nz, true, b->stream));
+    return NMR_OK;
+}
+
+// Every instance's state into the bricks unless they hold it already.
+int batch_bricks(nmr_batch* b)
+{
+    if (b->home & nmri::kHomeBricks) return NMR_OK;
+    for (int i = 0; i < b->ninst; i++) {
+        int rc = batch_inst_to_bricks(b, i);
+        if (rc) return rc;
+    }
+    b->home |= nmri::kHomeBricks;
+    return NMR_OK;
+}
 
 // One-step tile shape for an nx x ny instance under variant -3: the least
 // padded tile area weighted by each shape's measured cost per voxel on a
@@ -1527,9 +1601,10 @@ int mixed_variant(int nx, int ny, const double* cost)
 
 // Tile list, descriptors and grid for (variant, lz, grid kind); variant -3
 // gives each instance its own tile shape and groups the tiles by shape.
-int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
+int batch_prepare(nmr_batch* b, int variant, int lz, int persistent, bool brick)
 {
-    if (variant == b->cur_variant && lz == b->cur_lz && persistent == b->cur_persistent) return NMR_OK;
+    if (variant == b->cur_variant && lz == b->cur_lz && persistent == b->cur_persistent && brick == b->cur_brick)
+        return NMR_OK;
     const bool mixed = variant == kMixedVariant;
     int lx, rpt, tx, ty, st;
     if (!mixed && (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
@@ -1545,13 +1620,15 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
         const int vi = mixed ? mixed_variant(s.nx, s.ny, b->tile_cost.data()) : variant;
         nmr::variant_shape(vi, &lx, &rpt, &tx, &ty, &st);
         for (int q = 0; q < 3; q++) {
-            d.m[0][q] = I.m[0][q];
-            d.m[1][q] = I.m[1][q];
+            d.m[0][q] = brick ? I.mb[0][q] : I.m[0][q];
+            d.m[1][q] = brick ? I.mb[1][q] : I.m[1][q];
         }
         d.mask = I.mask;
-        d.vx = I.vx;
-        d.vy = I.vy;
-        d.vz = I.vz;
+        d.vx = brick ? I.vxb : I.vx;
+        d.vy = brick ? I.vyb : I.vy;
+        d.vz = brick ? I.vzb : I.vz;
+        d.nby = (s.ny + 3) / 4;
+        d.nbyv = (s.ny + 4) / 4;
         d.nx = s.nx;
         d.ny = s.ny;
         d.nz = s.nz;
@@ -1624,7 +1701,7 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
     b->ntiles = (int)tiles.size();
     for (auto& gr : b->groups) {
         if (persistent) {
-            const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, gr.variant);
+            const int res = nmr::batch_resident_blocks(b->nc, b->vel, b->exact, gr.variant, brick);
             if (res <= 0) return fail(NMR_ERR_DEVICE, "occupancy query failed");
             gr.grid = std::min(res, gr.count);
         } else {
@@ -1635,6 +1712,7 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent)
     b->cur_variant = variant;
     b->cur_lz = lz;
     b->cur_persistent = persistent;
+    b->cur_brick = brick;
     return NMR_OK;
 }
 
@@ -1661,7 +1739,7 @@ int batch_launch_step(nmr_batch* b, int local, int parity, double* part)
         a.local_step = local;
         a.step_base = b->d_step;
         a.partials = part + (size_t)gr.first * b->nc;
-        HIP_TRY(nmr::launch_batch(b->nc, b->vel, b->exact, gr.variant, gr.grid, a, b->stream));
+        HIP_TRY(nmr::launch_batch(b->nc, b->vel, b->exact, gr.variant, gr.grid, a, b->stream, b->cur_brick));
     }
     return NMR_OK;
 }
@@ -1781,6 +1859,7 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
     const nmr_batch_instance& s = b->spec[i];
     auto& I = b->inst[i];
     const int P = I.pitch, P1 = I.pitch1;
+    int rc = NMR_OK;
     {
         std::vector<uint8_t> h(I.n_m, 0);
         int64_t np = 0;
@@ -1853,6 +1932,12 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
         I.inert = f == 0;
     }
     I.uploaded = true;
+    I.vel_bricks = false;
+    // the rows hold the new state (both buffers); so do the bricks if they
+    // are the home of the other instances
+    if ((b->home & nmri::kHomeBricks) && (rc = batch_inst_to_bricks(b, i))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((size_t)i < b->sig_ok.size()) b->sig_ok[i] = 0;
     b->cur_variant = -2; // descriptors carry the inert flag: rebuild
     return NMR_OK;
 }
@@ -1864,6 +1949,11 @@ int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, doub
     double* dst[3] = {mx, my, mz};
     const nmr_batch_instance& s = b->spec[i];
     auto& I = b->inst[i];
+    if (!(b->home & nmri::kHomeRows)) // the instance's current buffer from the bricks
+        for (int q = 0; q < b->nc; q++)
+            HIP_TRY(nmr::launch_brick(I.mb[b->parity][q], I.m[b->parity][q], I.pitch / 4, s.ny, s.nz, false,
+                                      b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     std::vector<float> h(I.n_m);
     for (int q = 0; q < b->nc; q++) {
         if (!dst[q]) continue;
@@ -1874,6 +1964,14 @@ int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, doub
                     dst[q][(size_t)x + (size_t)s.nx * ((size_t)j + (size_t)s.ny * k)] =
                         h[(size_t)k * s.ny * I.pitch + (size_t)j * I.pitch + x];
     }
+    return NMR_OK;
+}
+
+int nmr_hip_batch_set_layout(nmr_batch* b, int32_t brick)
+{
+    if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_set_layout: null batch");
+    if (brick != 0 && brick != 1) return fail(NMR_ERR_INVALID, "layout %d", (int)brick);
+    b->brick_on = brick != 0;
     return NMR_OK;
 }
 
@@ -1914,6 +2012,12 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
             return NMR_OK;
         }
     }
+    // trials on the layout the runs take
+    const bool brick = b->brick_on && batch_bricks_fit(b);
+    {
+        int rc = brick ? batch_bricks(b) : nmri::batch_rows(b);
+        if (rc) return rc;
+    }
     const int trials = std::max(1, (int)trial_steps);
     double best = 1e300;
     int bv = 0, bl = 8, bp = 1;
@@ -1946,7 +2050,7 @@ int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, 
         }
         for (int cfg = 0; cfg < 10; cfg++) {
             const int l = 4 << (cfg >> 1), per = cfg & 1; // lz 4..64, grid kind
-            int rc = batch_prepare(b, v, l, per);
+            int rc = batch_prepare(b, v, l, per, brick);
             if (rc) return rc;
             if ((rc = batch_partials(b, 1))) return rc;
             // warm-up, then timed: every trial reads the current buffer and
@@ -2002,8 +2106,11 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
         if (v < 0) v = av;
         if (l <= 0) l = al;
     }
-    int rc = batch_prepare(b, v, l, b->persistent);
+    // the state stays in bricks from run to run when the layout allows
+    const bool brick = b->brick_on && batch_bricks_fit(b);
+    int rc = brick ? batch_bricks(b) : nmri::batch_rows(b);
     if (rc) return rc;
+    if ((rc = batch_prepare(b, v, l, b->persistent, brick))) return rc;
     const int nc = b->nc;
     const int64_t rows = nsteps + 1;
     const int S = b->graph_steps;
@@ -2015,11 +2122,21 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
         if ((rc = balloc(&b->d_signal, need))) return rc;
         b->signal_cap = need;
     }
-    // row 0: signal of every instance before the run
+    // row 0: signal of every instance before the run (kept from the last
+    // run, else summed over the state where it lives)
+    b->sig_ok.resize(b->ninst, 0);
+    b->sig_now.resize((size_t)b->ninst * nc, 0.0);
+    const std::vector<uint8_t> have0 = b->sig_ok;
+    const std::vector<double> sig0 = b->sig_now;
     for (int i = 0; i < b->ninst; i++) {
+        if (have0[i]) continue;
         const nmr_batch_instance& s = b->spec[i];
         nmr::FieldSumArgs a{};
-        for (int q = 0; q < 3; q++) a.f[q] = b->inst[i].m[b->parity][q];
+        for (int q = 0; q < 3; q++) a.f[q] = brick ? b->inst[i].mb[b->parity][q] : b->inst[i].m[b->parity][q];
+        if (brick) {
+            a.nby = (s.ny + 3) / 4;
+            a.nb8 = (b->inst[i].pitch / 4 + 1) / 2;
+        }
         a.mask = b->inst[i].mask;
         a.part = b->d_fpart;
         a.nx = s.nx;
@@ -2076,11 +2193,24 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
         done += n;
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    b->home = brick ? nmri::kHomeBricks : nmri::kHomeRows;
     if (signal)
         HIP_TRY(hipMemcpyAsync(signal, b->d_signal, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
+    // the last row of every instance: the signal of its new state (row 0 of
+    // an instance that kept its signal is not written on the device)
+    std::vector<double> last((size_t)b->ninst * nc);
+    HIP_TRY(hipMemcpy2DAsync(last.data(), sizeof(double) * nc, b->d_signal + (size_t)nsteps * nc,
+                             sizeof(double) * rows * nc, sizeof(double) * nc, b->ninst, hipMemcpyDeviceToHost,
+                             b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i < b->ninst; i++) {
+        const size_t o = (size_t)i * nc;
+        if (nsteps > 0 || !have0[i]) std::copy(last.begin() + o, last.begin() + o + nc, b->sig_now.begin() + o);
+        if (signal && have0[i]) std::copy(sig0.begin() + o, sig0.begin() + o + nc, signal + (size_t)i * rows * nc);
+    }
+    std::fill(b->sig_ok.begin(), b->sig_ok.end(), 1);
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
@@ -2097,6 +2227,20 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
 // Entry points for the other host files of the library.
 
 namespace nmri {
+
+int batch_rows(nmr_batch* b)
+{
+    if (b->home & kHomeRows) return NMR_OK;
+    for (int i = 0; i < b->ninst; i++) {
+        const auto& s = b->spec[i];
+        auto& I = b->inst[i];
+        for (int p = 0; p < 2; p++)
+            for (int q = 0; q < b->nc; q++)
+                HIP_TRY(nmr::launch_brick(I.mb[b->parity][q], I.m[p][q], I.pitch / 4, s.ny, s.nz, false, b->stream));
+    }
+    b->home |= kHomeRows;
+    return NMR_OK;
+}
 
 int state_rows(nmr_ctx* c)
 {

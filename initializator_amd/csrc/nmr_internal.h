@@ -125,15 +125,30 @@ struct nmr_batch {
         bool uploaded = false;
         int inert = 0;
         int64_t npore = 0;
+        // brick copies (8 x-voxels by 4 rows) of the state and the velocity
+        float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+        float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;
+        bool vel_bricks = false; // the velocity bricks hold the uploaded velocity
     };
     std::vector<Inst> inst;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int variant = -1, lz = 0, graph_steps = 64, persistent = 1;
     int parity = 0;
+    // the sweeps on bricks (nmr_hip_batch_set_layout); which layout holds
+    // every instance's current state (nmri::kHomeRows / kHomeBricks bits; a
+    // conversion writes both buffers of its target layout)
+    bool brick_on = true;
+    int home = 1;
+    // signal of every instance's current state ([ninst][nc], host copy of the
+    // last rows of the last run) and, per instance, whether it is still the
+    // signal of that state (an upload clears it)
+    std::vector<double> sig_now;
+    std::vector<uint8_t> sig_ok;
     int64_t steps_done = 0;
     // launch state for the current variant / lz / grid kind
     int cur_variant = -2, cur_lz = -1, cur_persistent = -1, grid = 0, ntiles = 0;
+    bool cur_brick = false;
     // per-instance tile shapes (variant -3): one launch per shape, over the
     // tile range [first, first + count) of the tile list
     struct Group { int variant, first, count, grid; };
@@ -166,4 +181,6 @@ int sweep_step(nmr_ctx* c, const nmr::Coef& cf, bool exact, int in, int out);
 // Make the row buffers hold the current state (both of them, converted from
 // the bricks when only those hold it).
 int state_rows(nmr_ctx* c);
+// The same for every instance of a batch.
+int batch_rows(nmr_batch* b);
 } // namespace nmri

@@ -39,7 +39,7 @@ struct SweepArgs {
 
 // One instance of a batch (device memory).
 struct BatchInst {
-    float* m[2][3];         // ping-pong magnetization buffers
+    float* m[2][3];         // ping-pong magnetization buffers (rows, or bricks with the velocity in bricks too)
     const uint8_t* mask;
     const float* vx;
     const float* vy;
@@ -48,6 +48,7 @@ struct BatchInst {
     int ntx, nty, ntz, lz;
     int bx0, bx1, by0, by1, bz0, bz1;
     int inert;
+    int nby, nbyv;          // brick layout: brick rows per plane of M / Vx / Vz and of Vy
     int nd, nD;             // PGSE lobe length and start of the second lobe (steps)
     float bgz;              // dt*g*G*dz of the +G lobe, rounded once on the host
                             // (the -G lobe negates it, which is exact)
@@ -267,8 +268,9 @@ size_t brick_floats(int nc4, int Y, int Z);
 // row layout [Z][Y][4*nc4] <-> bricks (to_brick: row -> brick)
 hipError_t launch_brick(const float* src, float* dst, int nc4, int Y, int Z, bool to_brick, hipStream_t s);
 hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args& a, hipStream_t s);
-hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s);
-int batch_resident_blocks(int nc, bool vel, bool exact, int variant);
+hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s,
+                        bool brick = false);
+int batch_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick = false);
 // Workgroups of the one-step sweep (variant 0..6, row or brick layout) and of
 // the general sweep's upwind / corrector kernel (on bricks with fused
 // phase-weighted sums: its weighted instantiation) resident on the device at
@@ -290,8 +292,10 @@ hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, cons
 // m[q][i] = 0 where mask[i] == 0, i < n, for q < nc
 hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s);
 // the same for fields in bricks (nz planes of ny rows of pitch voxels)
-// dst = src over nchunks 16-byte chunks (16-byte aligned), non-temporal
-hipError_t launch_copy(const float* src, float* dst, size_t nchunks, int grid, hipStream_t s);
+// dst = src over nchunks 16-byte chunks (16-byte aligned) by copy kernel
+// shape `kind` (0 .. copy_kinds() - 1) on a device of `cus` compute units
+int copy_kinds();
+hipError_t launch_copy(int kind, const float* src, float* dst, size_t nchunks, int cus, hipStream_t s);
 hipError_t launch_zero_solids_brick(float* const* m, int nc, const uint8_t* mask, int ny, int nz, int pitch,
                                     hipStream_t s);
 // workgroups (= partials) of a field sum over the signal window

@@ -823,7 +823,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 // tiles w, w + gridDim.x, ... of a list sorted by cost, each tile naming its
 // instance.  Instance descriptors and the step counter live in device memory
 // so that a chunk of launches can be captured in a graph and replayed.
-template <int NC, bool VEL, bool EXACT, int LX, int RPT>
+template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
 __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
 {
     __shared__ double sred[4][NC];
@@ -854,6 +854,8 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
         a.nblocks = 0;
         a.bx0 = I->bx0; a.bx1 = I->bx1; a.by0 = I->by0; a.by1 = I->by1; a.bz0 = I->bz0; a.bz1 = I->bz1;
         a.inert = I->inert;
+        a.nby = I->nby;
+        a.nbyv = I->nbyv;
         a.c = I->c;
         // pulsed gradient along z: +G on [0, nd), -G on [nD, nD + nd) (the
         // second lobe wins where they overlap)
@@ -861,7 +863,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
         double acc[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++) acc[q] = 0.0;
-        sweep_tile<NC, VEL, EXACT, LX, RPT>(a, tl.y, acc);
+        sweep_tile<NC, VEL, EXACT, LX, RPT, BRK>(a, tl.y, acc);
         block_reduce<NC, 256>(acc, sred, b.partials + (size_t)t * NC);
         __syncthreads();
     }
@@ -1885,31 +1887,69 @@ __global__ void __launch_bounds__(256) zero_solids_brick_kernel(float* m0, float
     }
 }
 
-// Copy of n 16-byte chunks with non-temporal dwordx4 loads and stores, a
-// grid-stride loop of 4 chunks per thread per pass with all four loads in
-// flight before the stores: the bandwidth a streaming kernel of this project
-// can reach on the device (the ceiling the sweeps are compared with).
+// Copies of n 16-byte chunks (dwordx4 loads and stores): the streaming
+// ceiling the sweeps are compared with is the fastest of these shapes.
+//   KIND 0: grid-stride loop, four chunks a grid-stride apart in flight per
+//           thread, non-temporal;
+//   KIND 1: one chunk per thread, one pass;
+//   KIND 2: one chunk per thread, non-temporal;
+//   KIND 3: four consecutive chunks per thread (64 B; a wavefront moves
+//           4 KiB of consecutive bytes), non-temporal.
+template <int KIND>
 __global__ void __launch_bounds__(256) copy_kernel(const f4* __restrict__ src, f4* __restrict__ dst, size_t n)
 {
-    const size_t stride = (size_t)gridDim.x * 256;
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const f4 a = __builtin_nontemporal_load(src + i);
-        const f4 b = __builtin_nontemporal_load(src + i + stride);
-        const f4 c = __builtin_nontemporal_load(src + i + 2 * stride);
-        const f4 d = __builtin_nontemporal_load(src + i + 3 * stride);
-        __builtin_nontemporal_store(a, dst + i);
-        __builtin_nontemporal_store(b, dst + i + stride);
-        __builtin_nontemporal_store(c, dst + i + 2 * stride);
-        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    if constexpr (KIND == 0) {
+        const size_t stride = (size_t)gridDim.x * 256;
+        size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            const f4 a = __builtin_nontemporal_load(src + i);
+            const f4 b = __builtin_nontemporal_load(src + i + stride);
+            const f4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+            const f4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+            __builtin_nontemporal_store(a, dst + i);
+            __builtin_nontemporal_store(b, dst + i + stride);
+            __builtin_nontemporal_store(c, dst + i + 2 * stride);
+            __builtin_nontemporal_store(d, dst + i + 3 * stride);
+        }
+        for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    } else if constexpr (KIND == 1 || KIND == 2) {
+        const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+        if (i >= n) return;
+        if constexpr (KIND == 1) dst[i] = src[i];
+        else __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    } else {
+        // wavefront w of the grid covers chunks [256 w, 256 w + 256); load j
+        // of lane l takes chunk 256 w + 64 j + l, so each load instruction of
+        // the wavefront reads 1 KiB of consecutive bytes
+        const size_t base = ((size_t)blockIdx.x * 256 + (threadIdx.x & ~63u)) * 4 + (threadIdx.x & 63u);
+        f4 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (base + 64 * j < n) v[j] = __builtin_nontemporal_load(src + base + 64 * j);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (base + 64 * j < n) __builtin_nontemporal_store(v[j], dst + base + 64 * j);
     }
-    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
-hipError_t launch_copy(const float* src, float* dst, size_t nchunks, int grid, hipStream_t s)
+int copy_kinds() { return 4; }
+
+hipError_t launch_copy(int kind, const float* src, float* dst, size_t nchunks, int cus, hipStream_t s)
 {
-    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const f4*>(src),
-                       reinterpret_cast<f4*>(dst), nchunks);
+    const f4* a = reinterpret_cast<const f4*>(src);
+    f4* b = reinterpret_cast<f4*>(dst);
+    const size_t one = (nchunks + 255) / 256, four = (nchunks + 1023) / 1024;
+    if (one > 0x7fffffffULL) return hipErrorInvalidValue;
+    switch (kind) {
+    case 0:
+        hipLaunchKernelGGL(copy_kernel<0>, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((size_t)cus * 16, four))),
+                           dim3(256), 0, s, a, b, nchunks);
+        break;
+    case 1: hipLaunchKernelGGL(copy_kernel<1>, dim3((unsigned)one), dim3(256), 0, s, a, b, nchunks); break;
+    case 2: hipLaunchKernelGGL(copy_kernel<2>, dim3((unsigned)one), dim3(256), 0, s, a, b, nchunks); break;
+    case 3: hipLaunchKernelGGL(copy_kernel<3>, dim3((unsigned)four), dim3(256), 0, s, a, b, nchunks); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -2070,30 +2110,42 @@ hipError_t launch_sweep2(int nc, bool vel, bool exact, int lx, const Sweep2Args&
 }
 
 template <int NC, bool VEL, bool EXACT>
-static hipError_t launchb_nc(int v, int grid, const BatchArgs& b, hipStream_t s)
+static hipError_t launchb_nc(int v, int grid, const BatchArgs& b, hipStream_t s, bool brick)
 {
     const dim3 g(grid), block(256);
+#define NMR_BV(LX_, RPT_)                                                                                    \
+    if (brick) hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, LX_, RPT_, true>), g, block, 0, s, b);      \
+    else hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, LX_, RPT_>), g, block, 0, s, b);
     switch (v) {
-    case 0: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 64, 1>), g, block, 0, s, b); break;
-    case 1: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 64, 2>), g, block, 0, s, b); break;
-    case 2: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 32, 1>), g, block, 0, s, b); break;
-    case 3: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 32, 2>), g, block, 0, s, b); break;
-    case 4: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 16, 1>), g, block, 0, s, b); break;
-    case 5: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 16, 2>), g, block, 0, s, b); break;
-    case 6: hipLaunchKernelGGL((batch_kernel<NC, VEL, EXACT, 8, 1>), g, block, 0, s, b); break;
+    case 0: NMR_BV(64, 1) break;
+    case 1: NMR_BV(64, 2) break;
+    case 2: NMR_BV(32, 1) break;
+    case 3: NMR_BV(32, 2) break;
+    case 4: NMR_BV(16, 1) break;
+    case 5: NMR_BV(16, 2) break;
+    case 6: NMR_BV(8, 1) break;
     default: return hipErrorInvalidValue;
     }
+#undef NMR_BV
     return hipGetLastError();
 }
 
-hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s)
+hipError_t launch_batch(int nc, bool vel, bool exact, int variant, int grid, const BatchArgs& b, hipStream_t s,
+                        bool brick)
 {
-    if (nc == 1) return vel ? launchb_nc<1, true, false>(variant, grid, b, s) : launchb_nc<1, false, false>(variant, grid, b, s);
+    if (nc == 1)
+        return vel ? launchb_nc<1, true, false>(variant, grid, b, s, brick)
+                   : launchb_nc<1, false, false>(variant, grid, b, s, brick);
     if (nc == 2) {
-        if (exact) return vel ? launchb_nc<2, true, true>(variant, grid, b, s) : launchb_nc<2, false, true>(variant, grid, b, s);
-        return vel ? launchb_nc<2, true, false>(variant, grid, b, s) : launchb_nc<2, false, false>(variant, grid, b, s);
+        if (exact)
+            return vel ? launchb_nc<2, true, true>(variant, grid, b, s, brick)
+                       : launchb_nc<2, false, true>(variant, grid, b, s, brick);
+        return vel ? launchb_nc<2, true, false>(variant, grid, b, s, brick)
+                   : launchb_nc<2, false, false>(variant, grid, b, s, brick);
     }
-    if (nc == 3) return vel ? launchb_nc<3, true, false>(variant, grid, b, s) : launchb_nc<3, false, false>(variant, grid, b, s);
+    if (nc == 3)
+        return vel ? launchb_nc<3, true, false>(variant, grid, b, s, brick)
+                   : launchb_nc<3, false, false>(variant, grid, b, s, brick);
     return hipErrorInvalidValue;
 }
 
@@ -2108,25 +2160,27 @@ static int resident_blocks(const void* f)
     return per_cu * ncu;
 }
 
-int batch_resident_blocks(int nc, bool vel, bool exact, int variant)
+int batch_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
 {
     const void* f = nullptr;
-#define NMR_PICK(NC_, V_, E_)                                                                   \
+#define NMR_PICK(NC_, V_, E_, B_)                                                               \
     switch (variant) {                                                                          \
-    case 0: f = (const void*)batch_kernel<NC_, V_, E_, 64, 1>; break;                           \
-    case 1: f = (const void*)batch_kernel<NC_, V_, E_, 64, 2>; break;                           \
-    case 2: f = (const void*)batch_kernel<NC_, V_, E_, 32, 1>; break;                           \
-    case 3: f = (const void*)batch_kernel<NC_, V_, E_, 32, 2>; break;                           \
-    case 4: f = (const void*)batch_kernel<NC_, V_, E_, 16, 1>; break;                           \
-    case 5: f = (const void*)batch_kernel<NC_, V_, E_, 16, 2>; break;                           \
-    case 6: f = (const void*)batch_kernel<NC_, V_, E_, 8, 1>; break;                            \
+    case 0: f = (const void*)batch_kernel<NC_, V_, E_, 64, 1, B_>; break;                       \
+    case 1: f = (const void*)batch_kernel<NC_, V_, E_, 64, 2, B_>; break;                       \
+    case 2: f = (const void*)batch_kernel<NC_, V_, E_, 32, 1, B_>; break;                       \
+    case 3: f = (const void*)batch_kernel<NC_, V_, E_, 32, 2, B_>; break;                       \
+    case 4: f = (const void*)batch_kernel<NC_, V_, E_, 16, 1, B_>; break;                       \
+    case 5: f = (const void*)batch_kernel<NC_, V_, E_, 16, 2, B_>; break;                       \
+    case 6: f = (const void*)batch_kernel<NC_, V_, E_, 8, 1, B_>; break;                        \
     default: return -1;                                                                         \
     }
-    if (nc == 1) { if (vel) { NMR_PICK(1, true, false) } else { NMR_PICK(1, false, false) } }
-    else if (nc == 2 && exact) { if (vel) { NMR_PICK(2, true, true) } else { NMR_PICK(2, false, true) } }
-    else if (nc == 2) { if (vel) { NMR_PICK(2, true, false) } else { NMR_PICK(2, false, false) } }
-    else if (nc == 3) { if (vel) { NMR_PICK(3, true, false) } else { NMR_PICK(3, false, false) } }
+#define NMR_PICKB(NC_, V_, E_) if (brick) { NMR_PICK(NC_, V_, E_, true) } else { NMR_PICK(NC_, V_, E_, false) }
+    if (nc == 1) { if (vel) { NMR_PICKB(1, true, false) } else { NMR_PICKB(1, false, false) } }
+    else if (nc == 2 && exact) { if (vel) { NMR_PICKB(2, true, true) } else { NMR_PICKB(2, false, true) } }
+    else if (nc == 2) { if (vel) { NMR_PICKB(2, true, false) } else { NMR_PICKB(2, false, false) } }
+    else if (nc == 3) { if (vel) { NMR_PICKB(3, true, false) } else { NMR_PICKB(3, false, false) } }
     else return -1;
+#undef NMR_PICKB
 #undef NMR_PICK
     return resident_blocks(f);
 }

@@ -524,6 +524,20 @@ int nmr_hip_corrector_train(nmr_corrector* c, nmr_batch* b, const nmr_train_opts
         return fail(NMR_ERR_INVALID, "training options");
     for (const auto& I : b->inst)
         if (!I.uploaded) return fail(NMR_ERR_STATE, "train before every instance is uploaded");
+    // the samples are read from the instances' row buffers: the batch runs
+    // on rows while training and leaves its state there
+    if (int rc0 = nmri::batch_rows(b)) return rc0;
+    struct RowsOnly {
+        nmr_batch* b;
+        bool keep;
+        ~RowsOnly()
+        {
+            b->brick_on = keep;
+            b->home = nmri::kHomeRows;
+        }
+    } guard{b, b->brick_on};
+    b->brick_on = false;
+    b->home = nmri::kHomeRows;
     const nmr_problem& P = b->prob;
     size_t nmax = 0;
     int mx = 1, my = 1, mz = 1;

@@ -18,12 +18,15 @@
 // there is no CPU solver behind them.  The whole domain lives on one GPU, so
 // MPI_size must be 1.  Two-phase mixtures (NCOMP=2, data/n1_init.dat) and
 // face-wise relaxivity files run with per-voxel coefficients and, for
-// NCOMP=2, the 7-column two-phase output.  Configurations the GPU path does
-// not implement (random noise field, NB flag combinations whose ghosts the
-// reference never refreshes) are reported and end the program with
-// exit(1), where the reference would run on with uninitialised data.  The NB flags select zero-derivative,
-// periodic or zero-value ghosts in both loops (the reference's host loop
-// ignores them).
+// NCOMP=2, the 7-column two-phase output.  The random noise field
+// (AMPL_NOISE with NDIRECT_FIELD 1-3, cpp:1973-1978) is drawn as the
+// reference draws it (one rand() per padded cell from glibc's unseeded state)
+// and runs as a per-voxel field.  Configurations the GPU path does not
+// implement (NB flag combinations whose ghosts the reference never
+// refreshes) are reported and end the program with exit(1), where the
+// reference would run on with uninitialised data.  The NB flags select
+// zero-derivative, periodic or zero-value ghosts in both loops (the
+// reference's host loop ignores them).
 #ifndef NMR_RELAXATION_SOLVE_H
 #define NMR_RELAXATION_SOLVE_H
 

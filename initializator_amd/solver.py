@@ -272,6 +272,10 @@ class BatchSolver:
         check(self.lib.nmr_hip_batch_download(self.h, int(i), *[_dptr(a) for a in outs]), "nmr_hip_batch_download")
         return outs[: self.ncomp]
 
+    def set_layout(self, brick: bool = True) -> None:
+        """Keep the instances' state in 128-byte bricks of 8 x 4 voxels (default) or in rows."""
+        check(self.lib.nmr_hip_batch_set_layout(self.h, int(bool(brick))), "nmr_hip_batch_set_layout")
+
     def set_tile(self, variant: int = -1, lz: int = 0, graph_steps: int = 64, persistent: bool = True) -> None:
         check(self.lib.nmr_hip_batch_set_tile(self.h, int(variant), int(lz), int(graph_steps), int(persistent)),
               "nmr_hip_batch_set_tile")

@@ -200,3 +200,36 @@ def test_batch_state_errors():
     with pytest.raises(Exception):
         b.set_tile(8, 0, 0)  # a two-step variant
     b.close()
+
+
+def test_batch_bricks_equal_rows():
+    """The batch on bricks (the default: state and velocity of every
+    instance in 8 x 4 bricks from run to run) against the row layout: the
+    same fields bit for bit and the same signal rows, across runs of odd
+    length (graph replays and direct steps), a download in the middle and a
+    re-upload of one instance between runs."""
+    specs = _instances()
+    res = {}
+    for brick in (True, False):
+        b = _batch(specs, graph_steps=16)
+        b.set_layout(brick)
+        s1 = b.run(37)
+        f1 = [b.download(i) for i in range(len(specs))]
+        b.upload(2, specs[2]["mask"], specs[2]["m0"], specs[2]["vel"])
+        s2 = b.run(20)
+        f2 = [b.download(i) for i in range(len(specs))]
+        res[brick] = (s1, f1, s2, f2)
+        b.close()
+    a, r = res[True], res[False]
+    for fa, fr in ((a[1], r[1]), (a[3], r[3])):
+        for i in range(len(specs)):
+            for q in range(2):
+                assert np.array_equal(fa[i][q], fr[i][q]), (i, q)
+    np.testing.assert_allclose(a[0], r[0], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(a[2], r[2], rtol=1e-12, atol=1e-300)
+    # row 0 of the second run: instance 2 restarted from its upload, the
+    # others continue from the first run's last row
+    for i in range(len(specs)):
+        if i != 2:
+            assert np.array_equal(a[2][i][0], a[0][i][-1])
+    print("[batch] bricks == rows across runs, downloads and a re-upload", flush=True)
