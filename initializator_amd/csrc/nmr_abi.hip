@@ -1980,8 +1980,8 @@ int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t gr
     if (!b) return fail(NMR_ERR_NULL, "nmr_hip_batch_set_tile: null batch");
     int lx, rpt, tx, ty, st;
     if (variant != -1 && variant != kMixedVariant &&
-        (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1))
-        return fail(NMR_ERR_INVALID, "batch variant %d", (int)variant);
+        (!nmr::variant_shape(variant, &lx, &rpt, &tx, &ty, &st) || st != 1 || variant > 6))
+        return fail(NMR_ERR_INVALID, "batch variant %d (one-step tiles 0..6)", (int)variant);
     if (lz < 0 || graph_steps < 0 || (graph_steps & 1) || (persistent != 0 && persistent != 1))
         return fail(NMR_ERR_INVALID, "lz %d / graph_steps %d / persistent %d", lz, graph_steps, persistent);
     b->variant = variant;

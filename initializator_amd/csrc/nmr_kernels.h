@@ -255,6 +255,8 @@ hipError_t launch_stats_reduce(const double* part, int nblocks, double* out, hip
 hipError_t launch_row_flags(const uint8_t* mask, int nx, int ny, int nz, int pitch, uint8_t* flag, hipStream_t s);
 
 int num_variants();
+// the compacted brick sweep's variant index (the last one)
+int compact_variant();
 // steps: time steps per pass (1, or 2 for the temporally blocked sweep)
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);
 hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s,

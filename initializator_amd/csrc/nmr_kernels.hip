@@ -807,6 +807,336 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     }
 }
 
+// Compacted brick sweep: tile 64 x 16 (one 4-voxel chunk of a row per
+// thread), the arithmetic of update_chunk on exactly the values the tile
+// sweep passes it, so the fields are the same bits.
+//
+// In the porous media here about 28 % of the 4-voxel chunks hold a pore,
+// but 97 % of the wavefronts of the tile sweep hold at least one, so the
+// update runs with about 28 % of its lanes busy.  Here every thread stages
+// its own chunk of each plane into a ring of four LDS planes (k-1, k, k+1
+// and the next), the chunks of a plane that hold a pore are listed (wave
+// ballots, a prefix over the waves) and the list is handed out to as few
+// wavefronts as it needs, 64 chunks each; a worker lane reads its chunk and
+// the six neighbour chunks from LDS.  The list of plane k+2 is built while
+// plane k is updated, so a worker knows its chunk of plane k+1 when the
+// update of plane k ends and loads that chunk's velocity faces right after
+// it, across the next plane's staging.  Which wavefront takes
+// the first 64 chunks rotates with the workgroup's slot on its CU, spreading
+// the update over the SIMDs.  One barrier per plane: the ring has a slot more than the
+// three planes an update reads, and the lists and counts three slots.
+template <int NC, bool VEL, bool EXACT>
+__device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, double (&acc)[NC])
+{
+    constexpr int D = 1; // planes loaded ahead of the one staged next
+    constexpr int LX = 16, TX = 64, TY = 16, NT = 256;
+    constexpr int LW = TX + 8; // LDS row: col 3 = x0-1, 4..4+TX-1 tile, 4+TX = x0+TX
+    constexpr int LH = TY + 2; // row 0 = y0-1, 1..TY tile, TY+1 = y0+TY
+    __shared__ __attribute__((aligned(16))) float sm[4][NC][LH][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t sk[4][LH][LW];
+    __shared__ uint8_t slist[3][NT];
+    __shared__ int scnt[3][4];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lx = tid % LX, ry = tid / LX;
+    const int tx = t % a.ntx;
+    const int tyz = t / a.ntx;
+    const int ty = tyz % a.nty;
+    const int tz = tyz / a.nty;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int k0 = tz * a.lz;
+    const int k1 = min(k0 + a.lz, a.nz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+    const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3;
+    auto xo = [&](int x) -> uint32_t { return ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2); };
+    auto yo = [&](int y) -> uint32_t { return (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5); };
+    auto yo1 = [&](int y) -> uint32_t { return (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5); };
+    auto zo = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nby * nbx * 128u; };
+    auto zo1 = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u; };
+    auto zov = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u; };
+    auto clampz = [&](int kk) { return kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk); };
+    auto slot = [](int p) { return (p + 4) & 3; }; // p >= -1
+    auto lst = [](int p) { return p % 3; };        // p >= 0
+
+    // ---- the owner's chunk (staging) ----
+    const int ix = x0 + 4 * lx, iy = y0 + ry;
+    const bool rin = ix < nx && iy < ny;
+    const int xlast = nx - 1 - ix;
+    const bool xend = rin && xlast <= 3;
+    const bool xpart = rin && xlast < 3;
+    const bool xbeg = ix == 0;
+    const int c = 4 + 4 * lx, lr = 1 + ry;
+    const uint32_t rowf = yo(iy < ny ? iy : 0) + xo(ix);
+    const uint32_t rowm = (uint32_t)(iy < ny ? iy : 0) * (uint32_t)a.pitch + (uint32_t)ix;
+    const bool top = ry == 0, bot = ry == TY - 1;
+    const int hy = top ? (y0 - 1) : (y0 + TY);
+    const bool hrow = (top || bot) && ix < nx && hy >= 0 && hy < ny;
+    const uint32_t hrowf = yo(hrow ? hy : 0) + xo(ix);
+    const uint32_t hrowm = (uint32_t)(hrow ? hy : 0) * (uint32_t)a.pitch + (uint32_t)ix;
+    const bool hcl = lx == 0 && x0 > 0;
+    const bool hcr = lx == LX - 1 && x0 + TX < nx;
+    const int hcx = lx == 0 ? x0 - 1 : x0 + TX;
+    const bool skip_m = a.inert != 0;
+
+    auto load_mask = [&](int kk) -> uint32_t {
+        return rin ? ldw_nt(a.mask, (uint32_t)clampz(kk) * plane_m + rowm) : 0u;
+    };
+    auto load_hmask = [&](int kk) -> uint32_t {
+        return hrow ? *reinterpret_cast<const uint32_t*>(a.mask + (uint32_t)clampz(kk) * plane_m + hrowm) : 0u;
+    };
+    auto load_m = [&](int kk, uint32_t kw, f4 (&m)[NC]) {
+        const uint32_t pf = zo(clampz(kk));
+#pragma unroll
+        for (int q = 0; q < NC; q++) m[q] = (rin && (!skip_m || kw != 0)) ? ld4_nt(a.min[q], pf + rowf) : f4{0.f, 0.f, 0.f, 0.f};
+    };
+    auto load_hm = [&](int kk, uint32_t hk, f4 (&h)[NC]) {
+        const uint32_t pf = zo(clampz(kk));
+#pragma unroll
+        for (int q = 0; q < NC; q++) h[q] = (hrow && (!skip_m || hk != 0)) ? ld4(a.min[q], pf + hrowf) : f4{0.f, 0.f, 0.f, 0.f};
+    };
+    auto load_xhalo = [&](int kk, float (&xv)[NC], uint32_t& kx) {
+        if ((hcl || hcr) && iy < ny) {
+            const int zz = clampz(kk);
+            const uint32_t of = zo(zz) + yo(iy) + xo(hcx);
+#pragma unroll
+            for (int q = 0; q < NC; q++) xv[q] = ld1(a.min[q], of);
+            kx = a.mask[(uint32_t)zz * plane_m + (uint32_t)iy * (uint32_t)a.pitch + (uint32_t)hcx];
+        }
+    };
+    // Elements past nx take the ghost value (the last voxel's).
+    auto fix_w = [&](uint32_t w) { return xpart ? clamp_tail_w(w, xlast) : w; };
+    auto fix_v = [&](f4 (&m)[NC]) {
+#pragma unroll
+        for (int q = 0; q < NC; q++)
+            if (xpart) m[q] = clamp_tail(m[q], xlast);
+    };
+    // Plane p (values v, mask w, halo row h / hk, halo column xv / kx) into
+    // its ring slot with the ghost cells of the domain boundary.
+    auto stage = [&](int p, const f4 (&v)[NC], uint32_t w, const f4 (&h)[NC], uint32_t hk, const float (&xv)[NC],
+                     uint32_t kx) {
+        const int s = slot(p);
+        if (rin) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[s][q][lr][c]) = v[q];
+            *reinterpret_cast<uint32_t*>(&sk[s][lr][c]) = w;
+            if (xend) { // ghost column nx (inside this chunk or the next column)
+                const int xl = xlast < 3 ? xlast : 3;
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[s][q][lr][c + 4] = v[q][xl];
+                sk[s][lr][c + 4] = (uint8_t)((w >> (8 * xl)) & 0xffu);
+            }
+            if (xbeg) { // ghost column -1
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[s][q][lr][c - 1] = v[q].x;
+                sk[s][lr][c - 1] = (uint8_t)(w & 0xffu);
+            }
+            if (hcl || hcr) {
+                const int col = hcl ? c - 1 : c + 4;
+#pragma unroll
+                for (int q = 0; q < NC; q++) sm[s][q][lr][col] = xv[q];
+                sk[s][lr][col] = (uint8_t)kx;
+            }
+            if (iy == 0) { // ghost row -1
+#pragma unroll
+                for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[s][q][lr - 1][c]) = v[q];
+                *reinterpret_cast<uint32_t*>(&sk[s][lr - 1][c]) = w;
+            }
+            if (iy == ny - 1) { // ghost row ny
+#pragma unroll
+                for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[s][q][lr + 1][c]) = v[q];
+                *reinterpret_cast<uint32_t*>(&sk[s][lr + 1][c]) = w;
+            }
+        }
+        if (hrow) {
+            const int hr = top ? 0 : TY + 1;
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[s][q][hr][c]) = h[q];
+            *reinterpret_cast<uint32_t*>(&sk[s][hr][c]) = hk;
+        }
+    };
+    // Count of plane p's chunks that hold a pore (p inside the z-chunk), per wave.
+    auto ballot_count = [&](int p, uint32_t w) -> unsigned long long {
+        const bool act = p < k1 && rin && w != 0u;
+        const unsigned long long b = __ballot(act);
+        if (lane == 0) scnt[lst(p)][wave] = (int)__popcll(b);
+        return b;
+    };
+    // After the barrier: this owner's place in plane p's list.
+    auto list_write = [&](int p, unsigned long long b) {
+        if ((b >> lane) & 1ull) {
+            int base = 0;
+            for (int w2 = 0; w2 < wave; w2++) base += scnt[lst(p)][w2];
+            const int pos = base + (int)__popcll(b & ((1ull << lane) - 1ull));
+            slist[lst(p)][pos] = (uint8_t)tid;
+        }
+    };
+
+    // ---- the worker's chunk ----
+    // The workgroups resident on one CU share its four SIMDs; the one that
+    // takes a workgroup's first 64 chunks rotates with the workgroup's slot
+    // on the CU (TG_ID, bits 19:16 of the HW_ID register).
+    const int rot = (int)(__builtin_amdgcn_s_getreg(4 | (16 << 6) | (3 << 11)) & 3u);
+    const int wslot = ((wave - rot) & 3) * 64 + lane;
+    auto assign = [&](int p) -> int { // chunk of plane p this lane updates, or -1
+        if (p >= k1) return -1;
+        const int n = scnt[lst(p)][0] + scnt[lst(p)][1] + scnt[lst(p)][2] + scnt[lst(p)][3];
+        return wslot < n ? (int)slist[lst(p)][wslot] : -1;
+    };
+    struct Faces {
+        f4 vx4, vy4, vy4n, wc, wn;
+        float vxe;
+    };
+    auto load_faces = [&](int p, int cid, Faces& f) {
+        f.vx4 = f.vy4 = f.vy4n = f.wc = f.wn = f4{0.f, 0.f, 0.f, 0.f};
+        f.vxe = 0.f;
+        if (VEL && cid >= 0) {
+            const int wx = x0 + 4 * (cid % LX), wy = y0 + cid / LX;
+            const uint32_t rf = yo(wy) + xo(wx);
+            f.vx4 = ld4_nt(a.vx, zo1(p) + yo1(wy) + xo(wx));
+            f.vxe = (nx - 1 - wx >= 3) ? ld1(a.vx, zo1(p) + yo1(wy) + xo(wx + 4)) : 0.f; // face wx+4 <= nx
+            f.vy4 = ld4_nt(a.vy, zov(p) + rf);
+            f.vy4n = ld4(a.vy, zov(p) + yo(wy + 1) + xo(wx));
+            f.wc = ld4(a.vz, zo(p) + rf);
+            f.wn = ld4(a.vz, zo(p + 1) + rf);
+        }
+    };
+
+    // ---- prologue: planes k0-1 and k0 staged; planes k0+1 .. k0+D in
+    // registers (the ring of loads in flight); the lists of planes k0 and
+    // k0+1; the faces of plane k0 ----
+    struct Plane {
+        f4 m[NC], h[NC];
+        float x[NC];
+        uint32_t w, hk, kx;
+    };
+    auto load_plane = [&](int p, uint32_t w, uint32_t hk, Plane& P) {
+        P.w = w;
+        P.hk = hk;
+        P.kx = 0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) P.x[q] = 0.f;
+        load_m(p, w, P.m);
+        load_hm(p, hk, P.h);
+        load_xhalo(p, P.x, P.kx);
+    };
+    auto fix_plane = [&](Plane& P) {
+        P.w = fix_w(P.w);
+        P.hk = fix_w(P.hk);
+        fix_v(P.m);
+        fix_v(P.h);
+    };
+    auto stage_plane = [&](int p, const Plane& P) { stage(p, P.m, P.w, P.h, P.hk, P.x, P.kx); };
+    Plane pl[D + 1]; // pl[0]: plane k+1 (arrived), pl[1..D]: planes k+2 .. k+D+1 (in flight)
+    uint32_t wnext = load_mask(k0 + D + 2), hnext = load_hmask(k0 + D + 2); // masks D+2 planes ahead
+    {
+        Plane pa, pb;
+        load_plane(k0 - 1, load_mask(k0 - 1), load_hmask(k0 - 1), pa);
+        load_plane(k0, load_mask(k0), load_hmask(k0), pb);
+#pragma unroll
+        for (int d = 0; d <= D; d++) load_plane(k0 + 1 + d, load_mask(k0 + 1 + d), load_hmask(k0 + 1 + d), pl[d]);
+        fix_plane(pa);
+        fix_plane(pb);
+        fix_plane(pl[0]);
+        stage_plane(k0 - 1, pa);
+        stage_plane(k0, pb);
+        const unsigned long long b0 = ballot_count(k0, pb.w);
+        const unsigned long long b1 = ballot_count(k0 + 1, pl[0].w);
+        __syncthreads();
+        list_write(k0, b0);
+        list_write(k0 + 1, b1);
+        __syncthreads();
+    }
+    int cid = assign(k0);
+    Faces f0;
+    load_faces(k0, cid, f0);
+
+    const Coef cf = a.c;
+    for (int k = k0; k < k1; k++) {
+        // 1. stage plane k+1; 2. count plane k+2's pore chunks (its mask is
+        // the raw mask word of pl[1])
+        stage_plane(k + 1, pl[0]);
+        const unsigned long long b2 = ballot_count(k + 2, pl[1].w);
+        // 3. loads: the masks D+3 planes ahead first (the next iteration's
+        // loads wait for them, and so for nothing issued after them), then
+        // plane k+D+2 into the ring
+        const uint32_t wn2 = load_mask(k + D + 3), hn2 = load_hmask(k + D + 3);
+        Plane pn;
+        load_plane(k + D + 2, wnext, hnext, pn);
+        wnext = wn2;
+        hnext = hn2;
+        __syncthreads();
+        // 5. plane k+2's list; 6. this lane's chunk of plane k+1 (its faces
+        // are loaded after the update, so they do not hold registers during it)
+        list_write(k + 2, b2);
+        const int cid1 = assign(k + 1);
+
+        // 7. update of plane k
+        if (cid >= 0) {
+            const int wlx = cid % LX, wry = cid / LX;
+            const int wx = x0 + 4 * wlx, wy = y0 + wry;
+            const int wc_ = 4 + 4 * wlx, wlr = 1 + wry;
+            const int s0 = slot(k), sp = slot(k - 1), sn_ = slot(k + 1);
+            f4 mc[NC], mp[NC], mn[NC], ym[NC], yp[NC];
+            float xl[NC], xr[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                mc[q] = *reinterpret_cast<const f4*>(&sm[s0][q][wlr][wc_]);
+                mp[q] = *reinterpret_cast<const f4*>(&sm[sp][q][wlr][wc_]);
+                mn[q] = *reinterpret_cast<const f4*>(&sm[sn_][q][wlr][wc_]);
+                ym[q] = *reinterpret_cast<const f4*>(&sm[s0][q][wlr - 1][wc_]);
+                yp[q] = *reinterpret_cast<const f4*>(&sm[s0][q][wlr + 1][wc_]);
+                xl[q] = sm[s0][q][wlr][wc_ - 1];
+                xr[q] = sm[s0][q][wlr][wc_ + 4];
+            }
+            const uint32_t kc = *reinterpret_cast<const uint32_t*>(&sk[s0][wlr][wc_]);
+            const uint32_t kp = *reinterpret_cast<const uint32_t*>(&sk[sp][wlr][wc_]);
+            const uint32_t kn = *reinterpret_cast<const uint32_t*>(&sk[sn_][wlr][wc_]);
+            const uint32_t kym = *reinterpret_cast<const uint32_t*>(&sk[s0][wlr - 1][wc_]);
+            const uint32_t kyp = *reinterpret_cast<const uint32_t*>(&sk[s0][wlr + 1][wc_]);
+            const uint32_t kxm = (kc << 8) | (uint32_t)sk[s0][wlr][wc_ - 1];
+            const uint32_t kxp = (kc >> 8) | ((uint32_t)sk[s0][wlr][wc_ + 4] << 24);
+            const float thz = cf.bgz * ((float)k + 0.5f);
+            float sn, cm1;
+            plane_rotation<EXACT>(thz, sn, cm1);
+            const float thy = cf.bgy * ((float)wy + 0.5f);
+            f4 out[NC];
+            update_chunk<NC, VEL, EXACT>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, f0.vx4,
+                                         f0.vxe, f0.vy4, f0.vy4n, f0.wc, f0.wn, wx, thy, thz, sn, cm1, out);
+            if (k >= a.bz0 && k < a.bz1 && wy >= a.by0 && wy < a.by1) {
+                bool trim_x[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) trim_x[e] = (wx + e >= a.bx0) && (wx + e < a.bx1);
+                accumulate<NC>(acc, out, kc, trim_x);
+            }
+            const uint32_t of = zo(k) + yo(wy) + xo(wx);
+#pragma unroll
+            for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
+        }
+
+        // 8. the next plane's faces; rotate the ring of loads
+        load_faces(k + 1, cid1, f0);
+        cid = cid1;
+#pragma unroll
+        for (int d = 0; d < D; d++) pl[d] = pl[d + 1];
+        pl[D] = pn;
+        fix_plane(pl[0]);
+    }
+}
+
+template <int NC, bool VEL, bool EXACT>
+__global__ void __launch_bounds__(256) csweep_kernel(SweepArgs a)
+{
+    __shared__ double sred[4][NC];
+    double acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) acc[q] = 0.0;
+    csweep_tile<NC, VEL, EXACT>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
+}
+
 template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
@@ -1973,11 +2303,19 @@ static const int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 static const int kVariants2[] = {64, 32, 16};
 static const int kNumVariants2 = (int)(sizeof(kVariants2) / sizeof(kVariants2[0]));
 
-int num_variants() { return kNumVariants + kNumVariants2; }
+// After the two-step variants: the compacted brick sweep (tile 64 x 16; in
+// the row layout it runs as variant 4, the same tile).  Opt-in: at 256^3 it
+// issues 34 % fewer VALU instructions than the tile sweep (lanes busy 73 %
+// against 42 %) but its waves wait longer, and it measured 73.8 against
+// 70.6 us per step.
+static const int kCompactVariant = kNumVariants + kNumVariants2;
+int num_variants() { return kCompactVariant + 1; }
+int compact_variant() { return kCompactVariant; }
 
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps)
 {
     if (v < 0 || v >= num_variants()) return false;
+    if (v == kCompactVariant) v = 4;
     if (v < kNumVariants) {
         *lx = kVariants[v].lx;
         *rpt = kVariants[v].rpt;
@@ -2025,13 +2363,18 @@ hipError_t launch_brick(const float* src, float* dst, int nbx, int Y, int Z, boo
     return hipGetLastError();
 }
 
-bool brick_variant(int v) { return v >= 0 && v <= 6; }
+bool brick_variant(int v) { return (v >= 0 && v <= 6) || v == kCompactVariant; }
+// (the compacted sweep is opt-in: it measured slower than the 64 x 16 tile)
 int brick_auto_variant(int nx) { return nx > 32 ? 4 : 6; }
 
 template <int NC, bool VEL, bool EXACT>
 static hipError_t launch_nc_brick(int v, const SweepArgs& a, hipStream_t s)
 {
     const dim3 grid(a.nblocks), block(256);
+    if (v == kCompactVariant) {
+        hipLaunchKernelGGL((csweep_kernel<NC, VEL, EXACT>), grid, block, 0, s, a);
+        return hipGetLastError();
+    }
     switch (v) {
     case 0: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 64, 1, true>), grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 64, 2, true>), grid, block, 0, s, a); break;
@@ -2064,6 +2407,7 @@ static hipError_t launch_nc(int v, const SweepArgs& a, hipStream_t s)
 
 hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s, bool brick)
 {
+    if (!brick && variant == kCompactVariant) variant = 4; // the same tile in rows
     if (brick) {
         if (nc == 1)
             return vel ? launch_nc_brick<1, true, false>(variant, a, s) : launch_nc_brick<1, false, false>(variant, a, s);
@@ -2188,6 +2532,15 @@ int batch_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
 int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
 {
     const void* f = nullptr;
+    if (variant == kCompactVariant && brick) {
+        if (nc == 1) f = vel ? (const void*)csweep_kernel<1, true, false> : (const void*)csweep_kernel<1, false, false>;
+        else if (nc == 2 && exact)
+            f = vel ? (const void*)csweep_kernel<2, true, true> : (const void*)csweep_kernel<2, false, true>;
+        else if (nc == 2) f = vel ? (const void*)csweep_kernel<2, true, false> : (const void*)csweep_kernel<2, false, false>;
+        else if (nc == 3) f = vel ? (const void*)csweep_kernel<3, true, false> : (const void*)csweep_kernel<3, false, false>;
+        return resident_blocks(f);
+    }
+    if (variant == kCompactVariant) variant = 4;
 #define NMR_PICK(NC_, V_, E_, B_)                                                               \
     switch (variant) {                                                                          \
     case 0: f = (const void*)sweep_kernel<NC_, V_, E_, 64, 1, B_>; break;                       \
