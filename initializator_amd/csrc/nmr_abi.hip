@@ -97,6 +97,9 @@ void free_all(nmr_ctx* c)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
     for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
         if (q) (void)hipFree(q);
+    for (int q = 0; q < 3; q++)
+        for (float* f : {c->adv[q], c->advb[q]})
+            if (f) (void)hipFree(f);
     for (float* q : c->pvb)
         if (q) (void)hipFree(q);
     for (int b = 0; b < 2; b++)
@@ -568,8 +571,9 @@ bool bricks_fit(const nmr_ctx* c)
 }
 
 // Brick buffers of the state (both ping-pong buffers and, for MacCormack,
-// the predictor) and the brick copy of the velocity, allocated and converted
-// by the uploads (or the first brick run after the layout switch).
+// the predictor) and, for the general sweep, the brick copy of the face
+// velocities, allocated and converted by the uploads (or the first brick run
+// after the layout switch).
 int ensure_bricks(nmr_ctx* c)
 {
     const nmr_problem& p = c->prob;
@@ -586,6 +590,7 @@ int ensure_bricks(nmr_ctx* c)
                 int rc = dev_alloc(c, &c->pb[q], nmr::brick_floats(nbx, p.ny, p.nz));
                 if (rc) return rc;
             }
+    if (!c->general) return NMR_OK; // the face velocities in bricks serve the general sweep
     if (c->vel && !c->vxb) {
         int rc = dev_alloc(c, &c->vxb, nmr::brick_floats(nbx1, p.ny, p.nz));
         if (!rc) rc = dev_alloc(c, &c->vyb, nmr::brick_floats(nbx, p.ny + 1, p.nz));
@@ -597,6 +602,43 @@ int ensure_bricks(nmr_ctx* c)
         HIP_TRY(nmr::launch_brick(c->vy, c->vyb, nbx, p.ny + 1, p.nz, true, c->stream));
         HIP_TRY(nmr::launch_brick(c->vz, c->vzb, nbx, p.ny, p.nz + 1, true, c->stream));
         c->brick_vel_valid = true;
+    }
+    return NMR_OK;
+}
+
+// Upwind velocity sums of the plain sweep: rows (also the source of the
+// brick copy) and, for brick sweeps, bricks; computed once per velocity
+// upload.
+int ensure_adv(nmr_ctx* c, bool brick)
+{
+    if (!c->vel) return NMR_OK;
+    const nmr_problem& p = c->prob;
+    int rc;
+    if (!c->adv_valid) {
+        nmr::AdvArgs a{};
+        for (int q = 0; q < 3; q++) {
+            if (!c->adv[q] && (rc = dev_alloc(c, &c->adv[q], c->n_m))) return rc;
+            a.out[q] = c->adv[q];
+        }
+        a.vx = c->vx;
+        a.vy = c->vy;
+        a.vz = c->vz;
+        a.nx = p.nx;
+        a.ny = p.ny;
+        a.nz = p.nz;
+        a.pitch = c->pitch;
+        a.pitch1 = c->pitch1;
+        HIP_TRY(nmr::launch_adv(a, c->stream));
+        c->adv_valid = true;
+        c->advb_valid = false;
+    }
+    if (brick && !c->advb_valid) {
+        const int nbx = c->pitch / 4;
+        for (int q = 0; q < 3; q++) {
+            if (!c->advb[q] && (rc = dev_alloc(c, &c->advb[q], nmr::brick_floats(nbx, p.ny, p.nz)))) return rc;
+            HIP_TRY(nmr::launch_brick(c->adv[q], c->advb[q], nbx, p.ny, p.nz, true, c->stream));
+        }
+        c->advb_valid = true;
     }
     return NMR_OK;
 }
@@ -675,6 +717,7 @@ int prepare(nmr_ctx* c)
         if ((rc = ensure_bricks(c))) return rc;
         if ((rc = ensure_pv_bricks(c))) return rc;
     }
+    if (!c->general && (rc = ensure_adv(c, c->brick_on && bricks_fit(c)))) return rc;
     if ((rc = presize_buffers(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return NMR_OK;
@@ -938,6 +981,7 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
     }
     ctx->inert_dirty = true;
     ctx->brick_vel_valid = false;
+    ctx->adv_valid = ctx->advb_valid = false;
     return prepare(ctx);
 }
 
@@ -1305,14 +1349,13 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         return rc;
     // the one-step sweep keeps the state in 8x4 bricks from run to run
     if ((rc = brick ? state_bricks(ctx) : nmri::state_rows(ctx))) return rc;
+    if ((rc = ensure_adv(ctx, brick))) return rc;
 
     int w[6];
     signal_window(p, w);
     nmr::SweepArgs a{};
     a.mask = ctx->mask;
-    a.vx = brick ? ctx->vxb : ctx->vx;
-    a.vy = brick ? ctx->vyb : ctx->vy;
-    a.vz = brick ? ctx->vzb : ctx->vz;
+    for (int q = 0; q < 3; q++) a.adv[q] = brick ? ctx->advb[q] : ctx->adv[q];
     a.nx = p.nx;
     a.ny = p.ny;
     a.nz = p.nz;
@@ -1321,12 +1364,9 @@ int nmr_hip_run(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     a.bx0 = w[0]; a.bx1 = w[1]; a.by0 = w[2]; a.by1 = w[3]; a.bz0 = w[4]; a.bz1 = w[5];
     a.inert = ctx->inert;
     a.nby = (p.ny + 3) / 4;
-    a.nbyv = (p.ny + 4) / 4;
     nmr::Sweep2Args b{};
     b.mask = ctx->mask;
-    b.vx = ctx->vx;
-    b.vy = ctx->vy;
-    b.vz = ctx->vz;
+    for (int q = 0; q < 3; q++) b.adv[q] = ctx->adv[q];
     b.nx = p.nx;
     b.ny = p.ny;
     b.nz = p.nz;
@@ -1488,8 +1528,9 @@ void batch_free(nmr_batch* b)
         for (int p = 0; p < 2; p++)
             for (int q = 0; q < 3; q++)
                 if (I.mb[p][q]) (void)hipFree(I.mb[p][q]);
-        for (float* q : {I.vxb, I.vyb, I.vzb})
-            if (q) (void)hipFree(q);
+        for (int q = 0; q < 3; q++)
+            for (float* f : {I.adv[q], I.advb[q]})
+                if (f) (void)hipFree(f);
         if (I.mask) (void)hipFree(I.mask);
         if (I.vx) (void)hipFree(I.vx);
         if (I.vy) (void)hipFree(I.vy);
@@ -1519,35 +1560,28 @@ bool batch_bricks_fit(const nmr_batch* b)
 {
     for (size_t i = 0; i < b->inst.size(); i++) {
         const auto& s = b->spec[i];
-        const int nbx = b->inst[i].pitch / 4, nbx1 = b->inst[i].pitch1 / 4;
-        const size_t big = std::max({nmr::brick_floats(nbx, s.ny, s.nz), nmr::brick_floats(nbx1, s.ny, s.nz),
-                                     nmr::brick_floats(nbx, s.ny + 1, s.nz), nmr::brick_floats(nbx, s.ny, s.nz + 1)});
+        const size_t big = nmr::brick_floats(b->inst[i].pitch / 4, s.ny, s.nz);
         if (big * 4 >= ((size_t)1 << 32) - 256) return false;
     }
     return true;
 }
 
-// Brick buffers of instance i (allocated once) with its velocity converted.
+// Brick buffers of instance i (allocated once) with its velocity sums converted.
 int batch_inst_bricks(nmr_batch* b, int i)
 {
     const auto& s = b->spec[i];
     auto& I = b->inst[i];
-    const int nbx = I.pitch / 4, nbx1 = I.pitch1 / 4;
+    const int nbx = I.pitch / 4;
     int rc;
     for (int p = 0; p < 2; p++)
         for (int q = 0; q < b->nc; q++)
             if (!I.mb[p][q] && (rc = balloc(&I.mb[p][q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
-    if (b->vel) {
-        if (!I.vxb && ((rc = balloc(&I.vxb, nmr::brick_floats(nbx1, s.ny, s.nz))) ||
-                       (rc = balloc(&I.vyb, nmr::brick_floats(nbx, s.ny + 1, s.nz))) ||
-                       (rc = balloc(&I.vzb, nmr::brick_floats(nbx, s.ny, s.nz + 1)))))
-            return rc;
-        if (!I.vel_bricks) {
-            HIP_TRY(nmr::launch_brick(I.vx, I.vxb, nbx1, s.ny, s.nz, true, b->stream));
-            HIP_TRY(nmr::launch_brick(I.vy, I.vyb, nbx, s.ny + 1, s.nz, true, b->stream));
-            HIP_TRY(nmr::launch_brick(I.vz, I.vzb, nbx, s.ny, s.nz + 1, true, b->stream));
-            I.vel_bricks = true;
+    if (b->vel && !I.adv_bricks) {
+        for (int q = 0; q < 3; q++) {
+            if (!I.advb[q] && (rc = balloc(&I.advb[q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
+            HIP_TRY(nmr::launch_brick(I.adv[q], I.advb[q], nbx, s.ny, s.nz, true, b->stream));
         }
+        I.adv_bricks = true;
     }
     return NMR_OK;
 }
@@ -1624,11 +1658,8 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent, bool brick)
             d.m[1][q] = brick ? I.mb[1][q] : I.m[1][q];
         }
         d.mask = I.mask;
-        d.vx = brick ? I.vxb : I.vx;
-        d.vy = brick ? I.vyb : I.vy;
-        d.vz = brick ? I.vzb : I.vz;
+        for (int q = 0; q < 3; q++) d.adv[q] = brick ? I.advb[q] : I.adv[q];
         d.nby = (s.ny + 3) / 4;
-        d.nbyv = (s.ny + 4) / 4;
         d.nx = s.nx;
         d.ny = s.ny;
         d.nz = s.nz;
@@ -1829,7 +1860,8 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
             I.n_vx = (size_t)s.nz * s.ny * I.pitch1;
             I.n_vy = (size_t)s.nz * (s.ny + 1) * I.pitch;
             I.n_vz = (size_t)(s.nz + 1) * s.ny * I.pitch;
-            if ((rc = balloc(&I.vx, I.n_vx)) || (rc = balloc(&I.vy, I.n_vy)) || (rc = balloc(&I.vz, I.n_vz)))
+            if ((rc = balloc(&I.vx, I.n_vx)) || (rc = balloc(&I.vy, I.n_vy)) || (rc = balloc(&I.vz, I.n_vz)) ||
+                (rc = balloc(&I.adv[0], I.n_m)) || (rc = balloc(&I.adv[1], I.n_m)) || (rc = balloc(&I.adv[2], I.n_m)))
                 return bail(rc);
         }
     }
@@ -1926,13 +1958,24 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
         a.pitch = P;
         a.pitch1 = P1;
         HIP_TRY(nmr::launch_inert_check(a, b->stream));
+        nmr::AdvArgs v{};
+        v.vx = I.vx;
+        v.vy = I.vy;
+        v.vz = I.vz;
+        for (int q = 0; q < 3; q++) v.out[q] = I.adv[q];
+        v.nx = s.nx;
+        v.ny = s.ny;
+        v.nz = s.nz;
+        v.pitch = P;
+        v.pitch1 = P1;
+        HIP_TRY(nmr::launch_adv(v, b->stream));
         int f = 0;
         HIP_TRY(hipMemcpyAsync(&f, b->d_flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         I.inert = f == 0;
     }
     I.uploaded = true;
-    I.vel_bricks = false;
+    I.adv_bricks = false;
     // the rows hold the new state (both buffers); so do the bricks if they
     // are the home of the other instances
     if ((b->home & nmri::kHomeBricks) && (rc = batch_inst_to_bricks(b, i))) return rc;
@@ -2266,17 +2309,16 @@ int sweep_step(nmr_ctx* c, const nmr::Coef& cf, bool exact, int in, int out)
     int rc = refresh_inert(c);
     if (rc) return rc;
     if ((rc = ::ensure_partials(c, (size_t)L.nwg * c->nc))) return rc;
+    if ((rc = ensure_adv(c, false))) return rc;
     int w[6];
     signal_window(p, w);
     nmr::SweepArgs a{};
     for (int q = 0; q < 3; q++) {
         a.min[q] = c->m[in][q];
         a.mout[q] = c->m[out][q];
+        a.adv[q] = c->adv[q];
     }
     a.mask = c->mask;
-    a.vx = c->vx;
-    a.vy = c->vy;
-    a.vz = c->vz;
     a.partials = c->partials;
     a.nx = p.nx;
     a.ny = p.ny;

@@ -67,9 +67,14 @@ struct nmr_ctx {
     // the sweeps in 128-byte bricks (8 x-voxels by 4 rows): state copies and velocity
     bool brick_on = true;       // nmr_hip_set_layout
     float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;
+    float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr; // face velocities (general sweep)
     float* pb[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor in bricks
     bool brick_vel_valid = false;
+    // upwind velocity sums of the plain sweep (nmr::launch_adv), in rows
+    // and, for brick sweeps, in bricks; computed once per velocity upload
+    float* adv[3] = {nullptr, nullptr, nullptr};
+    float* advb[3] = {nullptr, nullptr, nullptr};
+    bool adv_valid = false, advb_valid = false;
     // brick copies of the per-voxel arrays, in the order dn, r1v, r2v, bn,
     // w[0], w[1]; converted after each upload, before the next brick run
     float* pvb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -125,10 +130,12 @@ struct nmr_batch {
         bool uploaded = false;
         int inert = 0;
         int64_t npore = 0;
-        // brick copies (8 x-voxels by 4 rows) of the state and the velocity
+        // upwind velocity sums of the plain sweep (rows), computed at upload
+        float* adv[3] = {nullptr, nullptr, nullptr};
+        // brick copies (8 x-voxels by 4 rows) of the state and the velocity sums
         float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-        float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr;
-        bool vel_bricks = false; // the velocity bricks hold the uploaded velocity
+        float* advb[3] = {nullptr, nullptr, nullptr};
+        bool adv_bricks = false; // the brick sums hold the uploaded velocity's
     };
     std::vector<Inst> inst;
     hipStream_t stream = nullptr;

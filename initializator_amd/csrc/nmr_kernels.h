@@ -24,31 +24,27 @@ struct SweepArgs {
     const float* min[3];
     float* mout[3];
     const uint8_t* mask;
-    const float* vx;        // [nz][ny][pitch1]  face i-1|i at index i, i = 0..nx
-    const float* vy;        // [nz][ny+1][pitch] face j-1|j at row j
-    const float* vz;        // [nz+1][ny][pitch] face k-1|k at plane k
+    const float* adv[3];    // upwind velocity sums per axis (adv_kernel), laid out as the state
     double* partials;       // [nblocks][nc] signal partials of this step
     int nx, ny, nz;
     int pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
     int inert;              // no pore-solid face carries velocity
-    int nby, nbyv;          // brick layout: brick rows per plane of M / Vx / Vz and of Vy
+    int nby;                // brick layout: brick rows per plane
     Coef c;
 };
 
 // One instance of a batch (device memory).
 struct BatchInst {
-    float* m[2][3];         // ping-pong magnetization buffers (rows, or bricks with the velocity in bricks too)
+    float* m[2][3];         // ping-pong magnetization buffers (rows, or bricks with the velocity sums in bricks too)
     const uint8_t* mask;
-    const float* vx;
-    const float* vy;
-    const float* vz;
+    const float* adv[3];    // upwind velocity sums, laid out as the state
     int nx, ny, nz, pitch, pitch1;
     int ntx, nty, ntz, lz;
     int bx0, bx1, by0, by1, bz0, bz1;
     int inert;
-    int nby, nbyv;          // brick layout: brick rows per plane of M / Vx / Vz and of Vy
+    int nby;                // brick layout: brick rows per plane
     int nd, nD;             // PGSE lobe length and start of the second lobe (steps)
     float bgz;              // dt*g*G*dz of the +G lobe, rounded once on the host
                             // (the -G lobe negates it, which is exact)
@@ -63,6 +59,17 @@ struct BatchArgs {
     int local_step;         // step within the captured chunk
     const int* step_base;   // device step counter (first step of the chunk)
     double* partials;       // [ntiles][nc] of this step
+};
+
+// Upwind velocity sums of the plain sweep from the face velocities (row
+// layout: vx [nz][ny][pitch1], vy [nz][ny+1][pitch], vz [nz+1][ny][pitch]),
+// written as the state's rows [nz][ny][pitch].
+struct AdvArgs {
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    float* out[3];
+    int nx, ny, nz, pitch, pitch1;
 };
 
 struct InertArgs {
@@ -80,9 +87,7 @@ struct Sweep2Args {
     const float* min[3];
     float* mout[3];
     const uint8_t* mask;
-    const float* vx;
-    const float* vy;
-    const float* vz;
+    const float* adv[3];    // upwind velocity sums (row layout)
     double* partials1;      // [nblocks][nc] signal after the first step
     double* partials2;      // [nblocks][nc] signal after the second step
     int nx, ny, nz;
@@ -285,6 +290,7 @@ hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int
 hipError_t launch_step_advance(int* step_base, int n, hipStream_t s);
 hipError_t launch_reduce_partials(const double* part, int nsteps, int nwg, int nc, double* out, hipStream_t s);
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s);
+hipError_t launch_adv(const AdvArgs& a, hipStream_t s);
 hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
 // lanes per tile row of the general sweep for a grid of width nx (tile 4*lanes wide)

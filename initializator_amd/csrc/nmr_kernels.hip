@@ -124,16 +124,18 @@ __device__ __forceinline__ uint32_t clamp_tail_w(uint32_t w, int last)
 // Inputs: centre values mc, z/y neighbour chunks, x neighbours xl (x-1 of
 // element 0) and xr (x+1 of element 3), packed 0/1 mask words of the chunk and
 // of its six neighbour chunks (kxm/kxp already shifted by one voxel), and the
-// staggered face velocities.  Ghost cells hold zero-derivative copies, so no
-// bounds tests are needed.  Solid voxels keep their value.
+// upwind velocity sums of the chunk's voxels (adv_kernel: per axis the sum of
+// the voxel's two face velocities, 0 where the upwind face carries none).
+// Ghost cells hold zero-derivative copies, so no bounds tests are needed.
+// Solid voxels keep their value.
 template <int NC, bool VEL, bool EXACT>
 __device__ __forceinline__ void update_chunk(const Coef& cf, const f4 (&mc)[NC], const f4 (&fzm)[NC],
                                              const f4 (&fzp)[NC], const f4 (&fym)[NC], const f4 (&fyp)[NC],
                                              const float (&xl)[NC], const float (&xr)[NC], uint32_t kc,
                                              uint32_t kxm, uint32_t kxp, uint32_t kym, uint32_t kyp,
-                                             uint32_t kzm, uint32_t kzp, const f4& vx4, float vxe, const f4& vy4,
-                                             const f4& vy4n, const f4& vz0, const f4& vz1, int ix, float thy,
-                                             float thz, float sn, float cm1, f4 (&out)[NC])
+                                             uint32_t kzm, uint32_t kzp, const f4& ax4, const f4& ay4,
+                                             const f4& az4, int ix, float thy, float thz, float sn, float cm1,
+                                             f4 (&out)[NC])
 {
     const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
     // solid neighbours per axis, one byte per voxel
@@ -150,16 +152,14 @@ __device__ __forceinline__ void update_chunk(const Coef& cf, const f4 (&mc)[NC],
         float wym = cf.cdy * ub(kym, e), wyp = cf.cdy * ub(kyp, e);
         float wzm = cf.cdz * ub(kzm, e), wzp = cf.cdz * ub(kzp, e);
         if constexpr (VEL) {
-            const float um = vx4[e], up = (e < 3) ? vx4[e + 1] : vxe;
-            const float vm = vy4[e], vp = vy4n[e];
-            const float wm = vz0[e], wp = vz1[e];
-            const float su = um + up, sv = vm + vp, sw = wm + wp;
-            wxm = (um != 0.f) ? fmaf(hxh, fmaxf(su, 0.f), wxm) : wxm;
-            wxp = (up != 0.f) ? fmaf(-hxh, fminf(su, 0.f), wxp) : wxp;
-            wym = (vm != 0.f) ? fmaf(hyh, fmaxf(sv, 0.f), wym) : wym;
-            wyp = (vp != 0.f) ? fmaf(-hyh, fminf(sv, 0.f), wyp) : wyp;
-            wzm = (wm != 0.f) ? fmaf(hzh, fmaxf(sw, 0.f), wzm) : wzm;
-            wzp = (wp != 0.f) ? fmaf(-hzh, fminf(sw, 0.f), wzp) : wzp;
+            // inflow across the minus face when the sum is positive, across
+            // the plus face when negative; a zero sum adds h*0 (exact)
+            wxm = fmaf(hxh, fmaxf(ax4[e], 0.f), wxm);
+            wxp = fmaf(-hxh, fminf(ax4[e], 0.f), wxp);
+            wym = fmaf(hyh, fmaxf(ay4[e], 0.f), wym);
+            wyp = fmaf(-hyh, fminf(ay4[e], 0.f), wyp);
+            wzm = fmaf(hzh, fmaxf(az4[e], 0.f), wzm);
+            wzp = fmaf(-hzh, fminf(az4[e], 0.f), wzp);
         }
         const float nsx = ub(solx, e), nsy = ub(soly, e), nsz = ub(solz, e);
         const float rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
@@ -485,41 +485,28 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     const uint32_t Pb = (uint32_t)a.pitch * 4u;      // bytes per row (float arrays)
     const uint32_t plane_b = (uint32_t)ny * Pb;      // bytes per plane
     const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
-    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
-    const uint32_t planex_b = (uint32_t)ny * P1b;
-    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
-    // byte offsets of the float arrays along x, y, z (they add up): M and
-    // Vz (rows of pitch floats), Vx (pitch1), Vy (ny+1 rows per plane)
-    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3; // bricks per row
+    // byte offsets of the float arrays along x, y, z (they add up): rows of
+    // pitch floats, or 128-byte bricks of 8 x-voxels by 4 rows
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3; // bricks per row
     auto xo = [&](int x) -> uint32_t {
         return BRK ? ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2) : (uint32_t)x * 4u;
     };
     auto yo = [&](int y) -> uint32_t {
         return BRK ? (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * Pb;
     };
-    auto yo1 = [&](int y) -> uint32_t {
-        return BRK ? (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * P1b;
-    };
     auto zo = [&](int z) -> uint32_t {
         return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx * 128u : (uint32_t)z * plane_b;
-    };
-    auto zo1 = [&](int z) -> uint32_t {
-        return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u : (uint32_t)z * planex_b;
-    };
-    auto zov = [&](int z) -> uint32_t {
-        return BRK ? (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u : (uint32_t)z * planey_b;
     };
 
     int iy[RPT];
     bool rin[RPT];
-    uint32_t rowf[RPT], rowm[RPT], rowx[RPT];
+    uint32_t rowf[RPT], rowm[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         iy[r] = y0 + ry + r * RY;
         rin[r] = xin && (iy[r] < ny);
         rowf[r] = yo(iy[r]) + xo(ix);
         rowm[r] = (uint32_t)iy[r] * (uint32_t)a.pitch + (uint32_t)ix;
-        rowx[r] = yo1(iy[r]) + xo(ix);
     }
     const bool top = (ry == 0);
     const bool bot = (ry == RY - 1);
@@ -629,28 +616,21 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
         }
     };
 
-    // velocity faces of a plane: Vx (faces i..i+4), Vy (rows y, y+1), Vz (k, k+1)
-    auto load_vel = [&](int kk, const uint32_t (&kw)[RPT], f4 (&vx4)[RPT], float (&vxe)[RPT], f4 (&vy4)[RPT],
-                        f4 (&vy4n)[RPT], f4 (&wc)[RPT], f4 (&wn)[RPT]) {
-        const bool inz = kk < nz;
-        const uint32_t px = zo1(kk), py = zov(kk), pz = zo(kk), pz1 = zo(kk + 1);
+    // upwind velocity sums of a plane's chunks
+    auto load_vel = [&](int kk, const uint32_t (&kw)[RPT], f4 (&ax)[RPT], f4 (&ay)[RPT], f4 (&az)[RPT]) {
+        const uint32_t pz = zo(kk);
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
-            if (inz && rin[r] && kw[r] != 0) {
-                vx4[r] = ld4_nt(a.vx, px + rowx[r]);
-                vxe[r] = (xlast >= 3) ? ld1(a.vx, px + yo1(iy[r]) + xo(ix + 4)) : 0.f; // face ix+4 <= nx
-                vy4[r] = ld4_nt(a.vy, py + rowf[r]);
-                vy4n[r] = ld4(a.vy, py + yo(iy[r] + 1) + xo(ix));
-                wc[r] = ld4(a.vz, pz + rowf[r]);
-                wn[r] = ld4(a.vz, pz1 + rowf[r]);
+            if (rin[r] && kw[r] != 0) {
+                ax[r] = ld4_nt(a.adv[0], pz + rowf[r]);
+                ay[r] = ld4_nt(a.adv[1], pz + rowf[r]);
+                az[r] = ld4_nt(a.adv[2], pz + rowf[r]);
             } else {
-                vx4[r] = vy4[r] = vy4n[r] = wc[r] = wn[r] = f4{0.f, 0.f, 0.f, 0.f};
-                vxe[r] = 0.f;
+                ax[r] = ay[r] = az[r] = f4{0.f, 0.f, 0.f, 0.f};
             }
         }
     };
-    f4 vx4[RPT], vy4[RPT], vy4n[RPT], wc[RPT], wn[RPT];
-    float vxe[RPT];
+    f4 ax4[RPT], ay4[RPT], az4[RPT];
 
     // prologue: masks of planes k0-1, k0, k0+1; magnetization of k0-1, k0
     load_mask(k0 - 1, kp);
@@ -695,7 +675,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
         load_hm(k + 1, hkn, hn);
         load_xhalo(k + 1, xn, kxn);
         // velocity faces of plane k (only for chunks holding a pore)
-        if constexpr (VEL) load_vel(k, kc, vx4, vxe, vy4, vy4n, wc, wn);
+        if constexpr (VEL) load_vel(k, kc, ax4, ay4, az4);
 
         // stage plane k (with ghosts) into LDS
 #pragma unroll
@@ -775,7 +755,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             const uint32_t kxp = (kc[r] >> 8) | ((uint32_t)sk[buf][lr][c + 4] << 24);
             f4 out[NC];
             update_chunk<NC, VEL, EXACT>(cf, mc[r], mp[r], mn[r], ym, yp, xl, xr, kc[r], kxm, kxp, kym, kyp, kp[r],
-                                         kn[r], vx4[r], vxe[r], vy4[r], vy4n[r], wc[r], wn[r], ix, thy, thz, sn,
+                                         kn[r], ax4[r], ay4[r], az4[r], ix, thy, thz, sn,
                                          cm1, out);
             if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
             const uint32_t of = zo(k) + rowf[r];
@@ -828,7 +808,6 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
 template <int NC, bool VEL, bool EXACT>
 __device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, double (&acc)[NC])
 {
-    constexpr int D = 1; // planes loaded ahead of the one staged next
     constexpr int LX = 16, TX = 64, TY = 16, NT = 256;
     constexpr int LW = TX + 8; // LDS row: col 3 = x0-1, 4..4+TX-1 tile, 4+TX = x0+TX
     constexpr int LH = TY + 2; // row 0 = y0-1, 1..TY tile, TY+1 = y0+TY
@@ -850,13 +829,10 @@ __device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, dou
     const int nx = a.nx, ny = a.ny, nz = a.nz;
 
     const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
-    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3;
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3;
     auto xo = [&](int x) -> uint32_t { return ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2); };
     auto yo = [&](int y) -> uint32_t { return (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5); };
-    auto yo1 = [&](int y) -> uint32_t { return (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5); };
     auto zo = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nby * nbx * 128u; };
-    auto zo1 = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u; };
-    auto zov = [&](int z) -> uint32_t { return (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u; };
     auto clampz = [&](int kk) { return kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk); };
     auto slot = [](int p) { return (p + 4) & 3; }; // p >= -1
     auto lst = [](int p) { return p % 3; };        // p >= 0
@@ -986,87 +962,71 @@ __device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, dou
         return wslot < n ? (int)slist[lst(p)][wslot] : -1;
     };
     struct Faces {
-        f4 vx4, vy4, vy4n, wc, wn;
-        float vxe;
+        f4 ax, ay, az; // upwind velocity sums of the worker's chunk
     };
     auto load_faces = [&](int p, int cid, Faces& f) {
-        f.vx4 = f.vy4 = f.vy4n = f.wc = f.wn = f4{0.f, 0.f, 0.f, 0.f};
-        f.vxe = 0.f;
+        f.ax = f.ay = f.az = f4{0.f, 0.f, 0.f, 0.f};
         if (VEL && cid >= 0) {
             const int wx = x0 + 4 * (cid % LX), wy = y0 + cid / LX;
-            const uint32_t rf = yo(wy) + xo(wx);
-            f.vx4 = ld4_nt(a.vx, zo1(p) + yo1(wy) + xo(wx));
-            f.vxe = (nx - 1 - wx >= 3) ? ld1(a.vx, zo1(p) + yo1(wy) + xo(wx + 4)) : 0.f; // face wx+4 <= nx
-            f.vy4 = ld4_nt(a.vy, zov(p) + rf);
-            f.vy4n = ld4(a.vy, zov(p) + yo(wy + 1) + xo(wx));
-            f.wc = ld4(a.vz, zo(p) + rf);
-            f.wn = ld4(a.vz, zo(p + 1) + rf);
+            const uint32_t of = zo(p) + yo(wy) + xo(wx);
+            f.ax = ld4_nt(a.adv[0], of);
+            f.ay = ld4_nt(a.adv[1], of);
+            f.az = ld4_nt(a.adv[2], of);
         }
     };
 
-    // ---- prologue: planes k0-1 and k0 staged; planes k0+1 .. k0+D in
-    // registers (the ring of loads in flight); the lists of planes k0 and
-    // k0+1; the faces of plane k0 ----
-    struct Plane {
-        f4 m[NC], h[NC];
-        float x[NC];
-        uint32_t w, hk, kx;
-    };
-    auto load_plane = [&](int p, uint32_t w, uint32_t hk, Plane& P) {
-        P.w = w;
-        P.hk = hk;
-        P.kx = 0;
+    // ---- prologue: planes k0-1 and k0 staged, plane k0+1 in registers,
+    // the lists of planes k0 and k0+1, the faces of plane k0 ----
+    f4 m1[NC], m2[NC], h1[NC], h2[NC], mt[NC], ht[NC];
+    float x1[NC], x2[NC], xt[NC];
+    uint32_t kx1 = 0, kx2 = 0, kxt = 0;
 #pragma unroll
-        for (int q = 0; q < NC; q++) P.x[q] = 0.f;
-        load_m(p, w, P.m);
-        load_hm(p, hk, P.h);
-        load_xhalo(p, P.x, P.kx);
-    };
-    auto fix_plane = [&](Plane& P) {
-        P.w = fix_w(P.w);
-        P.hk = fix_w(P.hk);
-        fix_v(P.m);
-        fix_v(P.h);
-    };
-    auto stage_plane = [&](int p, const Plane& P) { stage(p, P.m, P.w, P.h, P.hk, P.x, P.kx); };
-    Plane pl[D + 1]; // pl[0]: plane k+1 (arrived), pl[1..D]: planes k+2 .. k+D+1 (in flight)
-    uint32_t wnext = load_mask(k0 + D + 2), hnext = load_hmask(k0 + D + 2); // masks D+2 planes ahead
-    {
-        Plane pa, pb;
-        load_plane(k0 - 1, load_mask(k0 - 1), load_hmask(k0 - 1), pa);
-        load_plane(k0, load_mask(k0), load_hmask(k0), pb);
-#pragma unroll
-        for (int d = 0; d <= D; d++) load_plane(k0 + 1 + d, load_mask(k0 + 1 + d), load_hmask(k0 + 1 + d), pl[d]);
-        fix_plane(pa);
-        fix_plane(pb);
-        fix_plane(pl[0]);
-        stage_plane(k0 - 1, pa);
-        stage_plane(k0, pb);
-        const unsigned long long b0 = ballot_count(k0, pb.w);
-        const unsigned long long b1 = ballot_count(k0 + 1, pl[0].w);
-        __syncthreads();
-        list_write(k0, b0);
-        list_write(k0 + 1, b1);
-        __syncthreads();
-    }
+    for (int q = 0; q < NC; q++) x1[q] = x2[q] = xt[q] = 0.f;
+    uint32_t wa = load_mask(k0 - 1), wb = load_mask(k0), k1w = load_mask(k0 + 1), k2w = load_mask(k0 + 2);
+    uint32_t hka = load_hmask(k0 - 1), hkb = load_hmask(k0), hk1 = load_hmask(k0 + 1), hk2 = load_hmask(k0 + 2);
+    load_m(k0 - 1, wa, mt);
+    load_hm(k0 - 1, hka, ht);
+    load_xhalo(k0 - 1, xt, kxt);
+    load_m(k0, wb, m2);
+    load_hm(k0, hkb, h2);
+    load_xhalo(k0, x2, kx2);
+    load_m(k0 + 1, k1w, m1);
+    load_hm(k0 + 1, hk1, h1);
+    load_xhalo(k0 + 1, x1, kx1);
+    wa = fix_w(wa);
+    wb = fix_w(wb);
+    k1w = fix_w(k1w);
+    hka = fix_w(hka);
+    hkb = fix_w(hkb);
+    hk1 = fix_w(hk1);
+    fix_v(mt);
+    fix_v(ht);
+    fix_v(m2);
+    fix_v(h2);
+    fix_v(m1);
+    fix_v(h1);
+    stage(k0 - 1, mt, wa, ht, hka, xt, kxt);
+    stage(k0, m2, wb, h2, hkb, x2, kx2);
+    const unsigned long long b0 = ballot_count(k0, wb);
+    const unsigned long long b1 = ballot_count(k0 + 1, k1w);
+    __syncthreads();
+    list_write(k0, b0);
+    list_write(k0 + 1, b1);
+    __syncthreads();
     int cid = assign(k0);
     Faces f0;
     load_faces(k0, cid, f0);
 
     const Coef cf = a.c;
     for (int k = k0; k < k1; k++) {
-        // 1. stage plane k+1; 2. count plane k+2's pore chunks (its mask is
-        // the raw mask word of pl[1])
-        stage_plane(k + 1, pl[0]);
-        const unsigned long long b2 = ballot_count(k + 2, pl[1].w);
-        // 3. loads: the masks D+3 planes ahead first (the next iteration's
-        // loads wait for them, and so for nothing issued after them), then
-        // plane k+D+2 into the ring
-        const uint32_t wn2 = load_mask(k + D + 3), hn2 = load_hmask(k + D + 3);
-        Plane pn;
-        load_plane(k + D + 2, wnext, hnext, pn);
-        wnext = wn2;
-        hnext = hn2;
+        // 1. stage plane k+1; 2. count plane k+2's pore chunks
+        stage(k + 1, m1, k1w, h1, hk1, x1, kx1);
+        const unsigned long long b2 = ballot_count(k + 2, k2w);
+        // 3. loads: mask of plane k+3, plane k+2
+        const uint32_t k3w = load_mask(k + 3), hk3 = load_hmask(k + 3);
+        load_m(k + 2, k2w, m2);
+        load_hm(k + 2, hk2, h2);
+        load_xhalo(k + 2, x2, kx2);
         __syncthreads();
         // 5. plane k+2's list; 6. this lane's chunk of plane k+1 (its faces
         // are loaded after the update, so they do not hold registers during it)
@@ -1103,8 +1063,8 @@ __device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, dou
             plane_rotation<EXACT>(thz, sn, cm1);
             const float thy = cf.bgy * ((float)wy + 0.5f);
             f4 out[NC];
-            update_chunk<NC, VEL, EXACT>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, f0.vx4,
-                                         f0.vxe, f0.vy4, f0.vy4n, f0.wc, f0.wn, wx, thy, thz, sn, cm1, out);
+            update_chunk<NC, VEL, EXACT>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, f0.ax,
+                                         f0.ay, f0.az, wx, thy, thz, sn, cm1, out);
             if (k >= a.bz0 && k < a.bz1 && wy >= a.by0 && wy < a.by1) {
                 bool trim_x[4];
 #pragma unroll
@@ -1116,13 +1076,22 @@ __device__ __forceinline__ void csweep_tile(const SweepArgs& a, const int t, dou
             for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
         }
 
-        // 8. the next plane's faces; rotate the ring of loads
+        // 8. the next plane's faces; rotate
         load_faces(k + 1, cid1, f0);
         cid = cid1;
 #pragma unroll
-        for (int d = 0; d < D; d++) pl[d] = pl[d + 1];
-        pl[D] = pn;
-        fix_plane(pl[0]);
+        for (int q = 0; q < NC; q++) {
+            m1[q] = m2[q];
+            h1[q] = h2[q];
+            x1[q] = x2[q];
+        }
+        fix_v(m1);
+        fix_v(h1);
+        kx1 = kx2;
+        k1w = fix_w(k2w);
+        hk1 = fix_w(hk2);
+        k2w = k3w;
+        hk2 = hk3;
     }
 }
 
@@ -1168,9 +1137,8 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
             a.mout[q] = I->m[b.parity ^ 1][q];
         }
         a.mask = I->mask;
-        a.vx = I->vx;
-        a.vy = I->vy;
-        a.vz = I->vz;
+#pragma unroll
+        for (int q = 0; q < 3; q++) a.adv[q] = I->adv[q];
         a.partials = nullptr;
         a.nx = I->nx;
         a.ny = I->ny;
@@ -1185,7 +1153,6 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
         a.bx0 = I->bx0; a.bx1 = I->bx1; a.by0 = I->by0; a.by1 = I->by1; a.bz0 = I->bz0; a.bz1 = I->bz1;
         a.inert = I->inert;
         a.nby = I->nby;
-        a.nbyv = I->nbyv;
         a.c = I->c;
         // pulsed gradient along z: +G on [0, nd), -G on [nD, nD + nd) (the
         // second lobe wins where they overlap)
@@ -1288,13 +1255,9 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
     const uint32_t Pb = (uint32_t)a.pitch * 4u;
     const uint32_t plane_b = (uint32_t)ny * Pb;
     const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
-    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
-    const uint32_t planex_b = (uint32_t)ny * P1b;
-    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
     const int yc = act ? y : 0, yec = eact ? ye : 0;
     const uint32_t rowf = (uint32_t)yc * Pb + (uint32_t)ix * 4u;
     const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)ix;
-    const uint32_t rowx = (uint32_t)yc * P1b + (uint32_t)ix * 4u;
     const uint32_t erowf = (uint32_t)yec * Pb + (uint32_t)ix * 4u;
     const uint32_t erowm = (uint32_t)yec * (uint32_t)a.pitch + (uint32_t)ix;
 
@@ -1371,10 +1334,9 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
     uint32_t kn0, kn1, kn2, kn3, ke1, ke2, kq = 0;
     // level-1 results of planes p-2, p-1 and p (r0, r1, rn)
     f4 r0[NC], r1[NC], rn[NC];
-    // velocities of the current level-1 plane and of the previous one
-    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, vz0 = vx4, vz1 = vx4;
-    f4 pvx4 = vx4, pvy4 = vx4, pvy4n = vx4, pvz0 = vx4, pvz1 = vx4;
-    float vxe = 0.f, pvxe = 0.f;
+    // upwind velocity sums of the current level-1 plane and of the previous one
+    f4 ax4 = f4{0.f, 0.f, 0.f, 0.f}, ay4 = ax4, az4 = ax4;
+    f4 pax4 = ax4, pay4 = ax4, paz4 = ax4;
 #pragma unroll
     for (int q = 0; q < NC; q++) r0[q] = r1[q] = rn[q] = f4{0.f, 0.f, 0.f, 0.f};
 
@@ -1383,9 +1345,6 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
     load_row(act, p0, rowf, rowm, n1, kn1);
     load_row(act, p0 + 1, rowf, rowm, n2, kn2);
     load_row(eact, p0, erowf, erowm, e1, ke1);
-    if constexpr (VEL) { // Vz face of the first level-1 plane (face 0 when the chunk starts at z = 0)
-        if (act) vz1 = ld4_nt(a.vz, (uint32_t)(p0 > 0 ? p0 : 0) * plane_b + rowf);
-    }
 
     double acc1[NC], acc2[NC];
 #pragma unroll
@@ -1404,14 +1363,11 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
         load_row(act, p + 2, rowf, rowm, n3, kn3);
         load_row(eact, p + 1, erowf, erowm, e2, ke2);
         if constexpr (VEL) {
-            vz0 = vz1;
             if (act && l1) {
-                const uint32_t px = (uint32_t)p * planex_b, py = (uint32_t)p * planey_b;
-                vx4 = ld4(a.vx, px + rowx);
-                vxe = (xlast >= 3) ? ld1(a.vx, px + rowx + 16u) : 0.f;
-                vy4 = ld4(a.vy, py + rowf);
-                vy4n = ld4(a.vy, py + rowf + Pb);
-                vz1 = ld4_nt(a.vz, (uint32_t)(p + 1) * plane_b + rowf);
+                const uint32_t of = (uint32_t)p * plane_b + rowf;
+                ax4 = ld4(a.adv[0], of);
+                ay4 = ld4(a.adv[1], of);
+                az4 = ld4(a.adv[2], of);
             }
         }
         // 1. stage M^n(p)
@@ -1429,8 +1385,8 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
             const float thz = c1.bgz * ((float)p + 0.5f);
             float sn, cm1;
             plane_rotation<EXACT>(thz, sn, cm1);
-            update_chunk<NC, VEL, EXACT>(c1, n1, n0, n2, ym, yp, xl, xr, kn1, kxm, kxp, kym, kyp, kn0, kn2, vx4, vxe,
-                                         vy4, vy4n, vz0, vz1, ix, thy1, thz, sn, cm1, rn);
+            update_chunk<NC, VEL, EXACT>(c1, n1, n0, n2, ym, yp, xl, xr, kn1, kxm, kxp, kym, kyp, kn0, kn2, ax4, ay4,
+                                         az4, ix, thy1, thz, sn, cm1, rn);
             if (xend) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) rn[q] = clamp_tail(rn[q], xlast);
@@ -1453,8 +1409,8 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
             float sn, cm1;
             plane_rotation<EXACT>(thz, sn, cm1);
             f4 out[NC];
-            update_chunk<NC, VEL, EXACT>(c2, r1, zm, zp, ym, yp, xl, xr, kn0, kxm, kxp, kym, kyp, kq, kn1, pvx4,
-                                         pvxe, pvy4, pvy4n, pvz0, pvz1, ix, thy2, thz, sn, cm1, out);
+            update_chunk<NC, VEL, EXACT>(c2, r1, zm, zp, ym, yp, xl, xr, kn0, kxm, kxp, kym, kyp, kq, kn1, pax4,
+                                         pay4, paz4, ix, thy2, thz, sn, cm1, out);
             const uint32_t of = (uint32_t)pm * plane_b + rowf;
 #pragma unroll
             for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
@@ -1479,12 +1435,9 @@ __global__ void __launch_bounds__(512) sweep2_kernel(Sweep2Args a)
         kn2 = kn3;
         ke1 = ke2;
         if constexpr (VEL) {
-            pvx4 = vx4;
-            pvxe = vxe;
-            pvy4 = vy4;
-            pvy4n = vy4n;
-            pvz0 = vz0;
-            pvz1 = vz1;
+            pax4 = ax4;
+            pay4 = ay4;
+            paz4 = az4;
         }
     }
     block_reduce<NC, NT>(acc1, sred, a.partials1 + (size_t)blockIdx.x * NC);
@@ -1523,6 +1476,50 @@ __global__ void __launch_bounds__(256) inert_check_kernel(InertArgs a)
 hipError_t launch_inert_check(const InertArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(inert_check_kernel, dim3((a.nx + 255) / 256, a.ny, a.nz), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Upwind velocity sums of one voxel per thread, for the plain sweep: per
+// axis the sum of the voxel's two face velocities, kept where the face on
+// the inflow side carries velocity (the minus face for a positive sum, the
+// plus face for a negative one) and 0 elsewhere -- ExplicitSimple
+// (cpp:3236-3247) advects across a face only when its velocity is nonzero.
+// The sweep then adds h*max(s, 0) to the minus face's weight and
+// -h*min(s, 0) to the plus face's, the same values as the face-by-face
+// test.  Written in the state's row layout [nz][ny][pitch], 0 past nx.
+__device__ __forceinline__ float upwind_sum(float m, float p)
+{
+    const float s = m + p;
+    const bool keep = s > 0.f ? m != 0.f : (s < 0.f && p != 0.f);
+    return keep ? s : 0.f;
+}
+
+__global__ void __launch_bounds__(256) adv_kernel(AdvArgs a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.pitch) return;
+    const size_t row = (size_t)z * a.ny + y;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if (x < a.nx) {
+        const size_t rx = row * a.pitch1 + x;
+        const size_t ry = ((size_t)z * (a.ny + 1) + y) * a.pitch + x;
+        const size_t rz = row * a.pitch + x;
+        sx = upwind_sum(a.vx[rx], a.vx[rx + 1]);
+        sy = upwind_sum(a.vy[ry], a.vy[ry + a.pitch]);
+        sz = upwind_sum(a.vz[rz], a.vz[rz + (size_t)a.ny * a.pitch]);
+    }
+    const size_t o = row * a.pitch + x;
+    a.out[0][o] = sx;
+    a.out[1][o] = sy;
+    a.out[2][o] = sz;
+}
+
+hipError_t launch_adv(const AdvArgs& a, hipStream_t s)
+{
+    if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0 || a.pitch < a.nx || a.pitch1 <= a.nx) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.pitch + 255) / 256), (unsigned)a.ny, (unsigned)a.nz);
+    hipLaunchKernelGGL(adv_kernel, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -612,9 +612,7 @@ int nmr_hip_corrector_train(nmr_corrector* c, nmr_batch* b, const nmr_train_opts
             a.mout[0] = coarse[0];
             a.mout[1] = coarse[1];
             a.mask = I.mask;
-            a.vx = I.vx;
-            a.vy = I.vy;
-            a.vz = I.vz;
+            for (int qq = 0; qq < 3; qq++) a.adv[qq] = I.adv[qq];
             a.partials = spart;
             a.nx = s.nx;
             a.ny = s.ny;
