@@ -98,7 +98,7 @@ void free_all(nmr_ctx* c)
     for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
         if (q) (void)hipFree(q);
     for (int q = 0; q < 3; q++)
-        for (float* f : {c->adv[q], c->advb[q]})
+        for (float* f : {c->adv[q], c->advb[q], c->csum[q], c->csumb[q]})
             if (f) (void)hipFree(f);
     for (float* q : c->pvb)
         if (q) (void)hipFree(q);
@@ -329,6 +329,7 @@ int enqueue_field_sum(nmr_ctx* c, size_t row)
 }
 
 int refresh_inert(nmr_ctx* c);
+int velocity_sums(nmr_ctx* c, bool brick, bool centred);
 bool bricks_fit(const nmr_ctx* c);
 int ensure_bricks(nmr_ctx* c);
 int ensure_pv_bricks(nmr_ctx* c);
@@ -426,6 +427,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     if (fused && (rc = ensure_partials(ctx, (size_t)std::min<int64_t>(nsteps, kChunkSteps) * g.nwg * ns))) return rc;
     if ((rc = gbrick ? state_bricks(ctx) : nmri::state_rows(ctx))) return rc;
     if (gbrick && (rc = ensure_pv_bricks(ctx))) return rc;
+    if (mak && !pervoxel && (rc = velocity_sums(ctx, gbrick, true))) return rc;
     const bool have0 = ctx->sig_valid;
     double sig0[6];
     std::copy(ctx->sig_now, ctx->sig_now + 6, sig0);
@@ -512,6 +514,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         a.nby = (p.ny + 3) / 4;
         a.nbyv = (p.ny + 4) / 4;
+        for (int q = 0; q < 3; q++) a.cs[q] = gbrick ? ctx->csumb[q] : ctx->csum[q];
         const bool coef = ctx->hetero;
         int64_t done = 0;
         while (done < nsteps) {
@@ -590,7 +593,8 @@ int ensure_bricks(nmr_ctx* c)
                 int rc = dev_alloc(c, &c->pb[q], nmr::brick_floats(nbx, p.ny, p.nz));
                 if (rc) return rc;
             }
-    if (!c->general) return NMR_OK; // the face velocities in bricks serve the general sweep
+    // the face velocities in bricks serve the general sweep's upwind stage
+    if (!c->general || p.scheme == NMR_SCHEME_MACCORMACK) return NMR_OK;
     if (c->vel && !c->vxb) {
         int rc = dev_alloc(c, &c->vxb, nmr::brick_floats(nbx1, p.ny, p.nz));
         if (!rc) rc = dev_alloc(c, &c->vyb, nmr::brick_floats(nbx, p.ny + 1, p.nz));
@@ -606,19 +610,23 @@ int ensure_bricks(nmr_ctx* c)
     return NMR_OK;
 }
 
-// Upwind velocity sums of the plain sweep: rows (also the source of the
-// brick copy) and, for brick sweeps, bricks; computed once per velocity
-// upload.
-int ensure_adv(nmr_ctx* c, bool brick)
+// Per-voxel face-velocity sums (upwind: those of the plain sweep; centred:
+// those of the MacCormack stages) in rows (also the source of the brick
+// copy) and, for brick sweeps, in bricks; computed once per velocity upload.
+int velocity_sums(nmr_ctx* c, bool brick, bool centred)
 {
     if (!c->vel) return NMR_OK;
     const nmr_problem& p = c->prob;
+    float** rows = centred ? c->csum : c->adv;
+    float** bricks = centred ? c->csumb : c->advb;
+    bool& rows_ok = centred ? c->csum_valid : c->adv_valid;
+    bool& bricks_ok = centred ? c->csumb_valid : c->advb_valid;
     int rc;
-    if (!c->adv_valid) {
+    if (!rows_ok) {
         nmr::AdvArgs a{};
         for (int q = 0; q < 3; q++) {
-            if (!c->adv[q] && (rc = dev_alloc(c, &c->adv[q], c->n_m))) return rc;
-            a.out[q] = c->adv[q];
+            if (!rows[q] && (rc = dev_alloc(c, &rows[q], c->n_m))) return rc;
+            a.out[q] = rows[q];
         }
         a.vx = c->vx;
         a.vy = c->vy;
@@ -628,20 +636,23 @@ int ensure_adv(nmr_ctx* c, bool brick)
         a.nz = p.nz;
         a.pitch = c->pitch;
         a.pitch1 = c->pitch1;
+        a.centred = centred ? 1 : 0;
         HIP_TRY(nmr::launch_adv(a, c->stream));
-        c->adv_valid = true;
-        c->advb_valid = false;
+        rows_ok = true;
+        bricks_ok = false;
     }
-    if (brick && !c->advb_valid) {
+    if (brick && !bricks_ok) {
         const int nbx = c->pitch / 4;
         for (int q = 0; q < 3; q++) {
-            if (!c->advb[q] && (rc = dev_alloc(c, &c->advb[q], nmr::brick_floats(nbx, p.ny, p.nz)))) return rc;
-            HIP_TRY(nmr::launch_brick(c->adv[q], c->advb[q], nbx, p.ny, p.nz, true, c->stream));
+            if (!bricks[q] && (rc = dev_alloc(c, &bricks[q], nmr::brick_floats(nbx, p.ny, p.nz)))) return rc;
+            HIP_TRY(nmr::launch_brick(rows[q], bricks[q], nbx, p.ny, p.nz, true, c->stream));
         }
-        c->advb_valid = true;
+        bricks_ok = true;
     }
     return NMR_OK;
 }
+
+int ensure_adv(nmr_ctx* c, bool brick) { return velocity_sums(c, brick, false); }
 
 // Brick copies of the per-voxel arrays the general sweep reads (those
 // uploaded), converted once after each upload.
@@ -718,6 +729,9 @@ int prepare(nmr_ctx* c)
         if ((rc = ensure_pv_bricks(c))) return rc;
     }
     if (!c->general && (rc = ensure_adv(c, c->brick_on && bricks_fit(c)))) return rc;
+    if (c->general && c->prob.scheme == NMR_SCHEME_MACCORMACK &&
+        (rc = velocity_sums(c, c->brick_on && bricks_fit(c), true)))
+        return rc;
     if ((rc = presize_buffers(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return NMR_OK;
@@ -982,6 +996,7 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
     ctx->inert_dirty = true;
     ctx->brick_vel_valid = false;
     ctx->adv_valid = ctx->advb_valid = false;
+    ctx->csum_valid = ctx->csumb_valid = false;
     return prepare(ctx);
 }
 

@@ -75,6 +75,10 @@ struct nmr_ctx {
     float* adv[3] = {nullptr, nullptr, nullptr};
     float* advb[3] = {nullptr, nullptr, nullptr};
     bool adv_valid = false, advb_valid = false;
+    // the same for the plain face-velocity sums of the MacCormack stages
+    float* csum[3] = {nullptr, nullptr, nullptr};
+    float* csumb[3] = {nullptr, nullptr, nullptr};
+    bool csum_valid = false, csumb_valid = false;
     // brick copies of the per-voxel arrays, in the order dn, r1v, r2v, bn,
     // w[0], w[1]; converted after each upload, before the next brick run
     float* pvb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

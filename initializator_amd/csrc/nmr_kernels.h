@@ -61,15 +61,18 @@ struct BatchArgs {
     double* partials;       // [ntiles][nc] of this step
 };
 
-// Upwind velocity sums of the plain sweep from the face velocities (row
-// layout: vx [nz][ny][pitch1], vy [nz][ny+1][pitch], vz [nz+1][ny][pitch]),
-// written as the state's rows [nz][ny][pitch].
+// Per-voxel sums of the two face velocities of each axis from the face
+// velocities (row layout: vx [nz][ny][pitch1], vy [nz][ny+1][pitch], vz
+// [nz+1][ny][pitch]), written as the state's rows [nz][ny][pitch]: the
+// upwind sums of the plain sweep (0 where the inflow face carries no
+// velocity) or, with centred, the plain sums of the MacCormack stages.
 struct AdvArgs {
     const float* vx;
     const float* vy;
     const float* vz;
     float* out[3];
     int nx, ny, nz, pitch, pitch1;
+    int centred;
 };
 
 struct InertArgs {
@@ -142,6 +145,8 @@ struct GSweepArgs {
     double* partials;       // [nblocks][nc] pore sums of the result, or null
     const float* w[2];      // brick sweep: phase weights (row layout) of fused weighted sums,
                             // partials then [nblocks][2*nc]; or null
+    const float* cs[3];     // MacCormack stages: face-velocity sums per voxel (AdvArgs centred),
+                            // laid out as the state; the upwind stage reads vx, vy, vz
     int nx, ny, nz, pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
