@@ -1559,11 +1559,17 @@ void batch_free(nmr_batch* b)
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
+// A cleared device buffer.  The clear runs on the batch's stream and is
+// waited for: that stream is non-blocking, so a clear on the null stream
+// (hipMemset) would not be ordered before the batch's own work on the
+// buffer and could land after it.
 template <typename T>
-int balloc(T** p, size_t count)
+int balloc(hipStream_t s, T** p, size_t count)
 {
-    HIP_TRY(hipMalloc((void**)p, std::max<size_t>(count * sizeof(T), 256)));
-    HIP_TRY(hipMemset(*p, 0, std::max<size_t>(count * sizeof(T), 256)));
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIP_TRY(hipMalloc((void**)p, bytes));
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return NMR_OK;
 }
 
@@ -1590,10 +1596,10 @@ int batch_inst_bricks(nmr_batch* b, int i)
     int rc;
     for (int p = 0; p < 2; p++)
         for (int q = 0; q < b->nc; q++)
-            if (!I.mb[p][q] && (rc = balloc(&I.mb[p][q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
+            if (!I.mb[p][q] && (rc = balloc(b->stream, &I.mb[p][q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
     if (b->vel && !I.adv_bricks) {
         for (int q = 0; q < 3; q++) {
-            if (!I.advb[q] && (rc = balloc(&I.advb[q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
+            if (!I.advb[q] && (rc = balloc(b->stream, &I.advb[q], nmr::brick_floats(nbx, s.ny, s.nz)))) return rc;
             HIP_TRY(nmr::launch_brick(I.adv[q], I.advb[q], nbx, s.ny, s.nz, true, b->stream));
         }
         I.adv_bricks = true;
@@ -1737,8 +1743,8 @@ int batch_prepare(nmr_batch* b, int variant, int lz, int persistent, bool brick)
     b->d_off = nullptr;
     b->d_ids = nullptr;
     int rc;
-    if ((rc = balloc(&b->d_inst, hi.size())) || (rc = balloc(&b->d_tiles, ht.size())) ||
-        (rc = balloc(&b->d_off, off.size())) || (rc = balloc(&b->d_ids, ids.size())))
+    if ((rc = balloc(b->stream, &b->d_inst, hi.size())) || (rc = balloc(b->stream, &b->d_tiles, ht.size())) ||
+        (rc = balloc(b->stream, &b->d_off, off.size())) || (rc = balloc(b->stream, &b->d_ids, ids.size())))
         return rc;
     HIP_TRY(hipMemcpy(b->d_inst, hi.data(), hi.size() * sizeof(nmr::BatchInst), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_tiles, ht.data(), ht.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1768,7 +1774,7 @@ int batch_partials(nmr_batch* b, size_t steps)
     if (need <= b->part_cap) return NMR_OK;
     if (b->d_part) HIP_TRY(hipFree(b->d_part));
     b->d_part = nullptr;
-    int rc = balloc(&b->d_part, need);
+    int rc = balloc(b->stream, &b->d_part, need);
     if (rc) return rc;
     b->part_cap = need;
     return NMR_OK;
@@ -1869,18 +1875,21 @@ int nmr_hip_batch_create(const nmr_problem* common, int32_t ninst, const nmr_bat
         maxrows = std::max(maxrows, (size_t)s.ny * (size_t)s.nz);
         for (int par = 0; par < 2; par++)
             for (int q = 0; q < b->nc; q++)
-                if ((rc = balloc(&I.m[par][q], I.n_m))) return bail(rc);
-        if ((rc = balloc(&I.mask, I.n_m))) return bail(rc);
+                if ((rc = balloc(b->stream, &I.m[par][q], I.n_m))) return bail(rc);
+        if ((rc = balloc(b->stream, &I.mask, I.n_m))) return bail(rc);
         if (b->vel) {
             I.n_vx = (size_t)s.nz * s.ny * I.pitch1;
             I.n_vy = (size_t)s.nz * (s.ny + 1) * I.pitch;
             I.n_vz = (size_t)(s.nz + 1) * s.ny * I.pitch;
-            if ((rc = balloc(&I.vx, I.n_vx)) || (rc = balloc(&I.vy, I.n_vy)) || (rc = balloc(&I.vz, I.n_vz)) ||
-                (rc = balloc(&I.adv[0], I.n_m)) || (rc = balloc(&I.adv[1], I.n_m)) || (rc = balloc(&I.adv[2], I.n_m)))
+            hipStream_t st = b->stream;
+            if ((rc = balloc(st, &I.vx, I.n_vx)) || (rc = balloc(st, &I.vy, I.n_vy)) || (rc = balloc(st, &I.vz, I.n_vz)) ||
+                (rc = balloc(st, &I.adv[0], I.n_m)) || (rc = balloc(st, &I.adv[1], I.n_m)) ||
+                (rc = balloc(st, &I.adv[2], I.n_m)))
                 return bail(rc);
         }
     }
-    if ((rc = balloc(&b->d_step, 1)) || (rc = balloc(&b->d_fpart, (maxrows / 16 + 2) * 3)) || (rc = balloc(&b->d_flag, 16)))
+    if ((rc = balloc(b->stream, &b->d_step, 1)) || (rc = balloc(b->stream, &b->d_fpart, (maxrows / 16 + 2) * 3)) ||
+        (rc = balloc(b->stream, &b->d_flag, 16)))
         return bail(rc);
     *out = b;
     return NMR_OK;
@@ -1960,7 +1969,7 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
     if (!b->vel) {
         I.inert = 1;
     } else {
-        HIP_TRY(hipMemset(b->d_flag, 0, sizeof(int)));
+        HIP_TRY(hipMemsetAsync(b->d_flag, 0, sizeof(int), b->stream));
         nmr::InertArgs a{};
         a.mask = I.mask;
         a.vx = I.vx;
@@ -2177,7 +2186,7 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
     if (need > b->signal_cap) {
         if (b->d_signal) HIP_TRY(hipFree(b->d_signal));
         b->d_signal = nullptr;
-        if ((rc = balloc(&b->d_signal, need))) return rc;
+        if ((rc = balloc(b->stream, &b->d_signal, need))) return rc;
         b->signal_cap = need;
     }
     // row 0: signal of every instance before the run (kept from the last

@@ -456,11 +456,14 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     constexpr int LW = TX + 8;   // LDS row: col 3 = x0-1, 4..4+TX-1 tile, 4+TX = x0+TX
     constexpr int LH = TY + 2;   // row 0 = y0-1, 1..TY tile, TY+1 = y0+TY
 
-    // Plane k of the tile plus its one-voxel halo; ghost cells of the domain
-    // boundary hold the zero-derivative copy, so the update reads neighbours
-    // without bounds tests.
-    __shared__ __attribute__((aligned(16))) float sm[2][NC][LH][LW];
-    __shared__ __attribute__((aligned(16))) uint8_t sk[2][LH][LW];
+    // Planes k-1 and k of the tile plus their one-voxel halo, in a ring of
+    // three slots; ghost cells of the domain boundary hold the
+    // zero-derivative copy, so the update reads neighbours without bounds
+    // tests.  Three slots let the update read planes k-1 and k from LDS
+    // (only the plane in flight, k+1, stays in registers) with one barrier
+    // per plane: the slot written for plane k was last read two planes ago.
+    __shared__ __attribute__((aligned(16))) float sm[3][NC][LH][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t sk[3][LH][LW];
 
     const int tid = threadIdx.x;
     const int lx = tid % LX;
@@ -660,14 +663,22 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
     // staging, where it would also wait for the loads just issued for the
     // next plane.
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    // plane k0-1 into slot 0 (its chunks only: the update reads it as the z
+    // neighbour at the same position)
+#pragma unroll
+    for (int r = 0; r < RPT; r++)
+        if (rin[r]) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) *reinterpret_cast<f4*>(&sm[0][q][1 + ry + r * RY][c]) = mp[r][q];
+        }
 
     const Coef cf = a.c;
     bool trim_x[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) trim_x[e] = (ix + e >= a.bx0) && (ix + e < a.bx1);
 
+    int pbuf = 0, buf = 1; // slots of planes k-1 and k
     for (int k = k0; k < k1; k++) {
-        const int buf = (k - k0) & 1;
         // prefetch: masks of plane k+2, magnetization of plane k+1
         load_mask(k + 2, kq);
         hkq = load_hmask(k + 2);
@@ -753,8 +764,13 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             const uint32_t kyp = *reinterpret_cast<const uint32_t*>(&sk[buf][lr + 1][c]);
             const uint32_t kxm = (kc[r] << 8) | (uint32_t)sk[buf][lr][c - 1];
             const uint32_t kxp = (kc[r] >> 8) | ((uint32_t)sk[buf][lr][c + 4] << 24);
-            f4 out[NC];
-            update_chunk<NC, VEL, EXACT>(cf, mc[r], mp[r], mn[r], ym, yp, xl, xr, kc[r], kxm, kxp, kym, kyp, kp[r],
+            f4 out[NC], mcl[NC], mpl[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                mcl[q] = *reinterpret_cast<const f4*>(&sm[buf][q][lr][c]);
+                mpl[q] = *reinterpret_cast<const f4*>(&sm[pbuf][q][lr][c]);
+            }
+            update_chunk<NC, VEL, EXACT>(cf, mcl, mpl, mn[r], ym, yp, xl, xr, kc[r], kxm, kxp, kym, kyp, kp[r],
                                          kn[r], ax4[r], ay4[r], az4[r], ix, thy, thz, sn,
                                          cm1, out);
             if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
@@ -763,12 +779,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
         }
 
-        // rotate the register planes
+        // rotate the register planes and the LDS slots
 #pragma unroll
         for (int r = 0; r < RPT; r++) {
 #pragma unroll
             for (int q = 0; q < NC; q++) {
-                mp[r][q] = mc[r][q];
                 mc[r][q] = mn[r][q];
                 xc[r][q] = xn[r][q];
             }
@@ -784,6 +799,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
         for (int q = 0; q < NC; q++) hc[q] = hn[q];
         hkc = hkn;
         hkn = hkq;
+        pbuf = buf;
+        buf = buf == 2 ? 0 : buf + 1;
     }
 }
 

@@ -69,11 +69,17 @@ struct nmr_corrector {
 
 namespace {
 
+// A cleared device buffer.  The clear runs on the corrector's stream and is
+// waited for: that stream is non-blocking, so a clear on the null stream
+// (hipMemset) would not be ordered before the corrector's own work on the
+// buffer and could land after it.
 template <typename T>
-int dalloc(T** p, size_t count)
+int dalloc(hipStream_t s, T** p, size_t count)
 {
-    HIP_TRY(hipMalloc((void**)p, std::max<size_t>(count * sizeof(T), 256)));
-    HIP_TRY(hipMemset(*p, 0, std::max<size_t>(count * sizeof(T), 256)));
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIP_TRY(hipMalloc((void**)p, bytes));
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return NMR_OK;
 }
 
@@ -95,10 +101,11 @@ int ensure_tensors(nmr_corrector* c, int nx, int ny, int nz, bool train)
         const size_t cap = std::max(nv, t.cap);
         tfree(t);
         int rc;
-        if ((rc = dalloc(&t.x, cap * 8)) || (rc = dalloc(&t.a1, cap * 16)) || (rc = dalloc(&t.a2, cap * 16)) ||
-            (rc = dalloc(&t.p, cap * 2)))
+        if ((rc = dalloc(c->stream, &t.x, cap * 8)) || (rc = dalloc(c->stream, &t.a1, cap * 16)) ||
+            (rc = dalloc(c->stream, &t.a2, cap * 16)) || (rc = dalloc(c->stream, &t.p, cap * 2)))
             return rc;
-        if (train && ((rc = dalloc(&t.g3, cap * 16)) || (rc = dalloc(&t.g2, cap * 16)) || (rc = dalloc(&t.g1, cap * 16))))
+        if (train && ((rc = dalloc(c->stream, &t.g3, cap * 16)) || (rc = dalloc(c->stream, &t.g2, cap * 16)) ||
+                      (rc = dalloc(c->stream, &t.g1, cap * 16))))
             return rc;
         c->train_ws = train;
         t.cap = cap;
@@ -247,7 +254,7 @@ int build_rows(nmr_corrector* c, nmr_ctx* ctx)
             if ((*sets[l])[i]) ids.push_back((int)i);
         if (c->rows[l]) HIP_TRY(hipFree(c->rows[l]));
         c->rows[l] = nullptr;
-        int rc = dalloc(&c->rows[l], ids.size());
+        int rc = dalloc(c->stream, &c->rows[l], ids.size());
         if (rc) return rc;
         if (!ids.empty()) HIP_TRY(hipMemcpy(c->rows[l], ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
         c->nrows[l] = (int)ids.size();
@@ -311,12 +318,12 @@ int adam_step(nmr_corrector* c, double lr)
     return repack(c);
 }
 
-int ensure_dev_doubles(double** p, size_t* cap, size_t n)
+int ensure_dev_doubles(nmr_corrector* c, double** p, size_t* cap, size_t n)
 {
     if (n <= *cap) return NMR_OK;
     if (*p) HIP_TRY(hipFree(*p));
     *p = nullptr;
-    int rc = dalloc(p, n);
+    int rc = dalloc(c->stream, p, n);
     if (rc) return rc;
     *cap = n;
     return NMR_OK;
@@ -387,10 +394,11 @@ int nmr_hip_corrector_create(uint64_t seed, nmr_corrector** out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(NMR_ERR_DEVICE, "stream creation failed"));
     int rc;
-    if ((rc = dalloc(&c->w, kParams)) || (rc = dalloc(&c->m, kParams)) || (rc = dalloc(&c->v, kParams)) ||
-        (rc = dalloc(&c->grad, kParams)) || (rc = dalloc(&c->f1, kFragF1)) || (rc = dalloc(&c->f2, kFragF)) ||
-        (rc = dalloc(&c->f3, kFragF)) || (rc = dalloc(&c->d2, kFragF)) || (rc = dalloc(&c->d3, kFragF)) ||
-        (rc = dalloc(&c->wpart, (size_t)kWgradGrid * 4 * nmr::kWgradPart)) || (rc = dalloc(&c->dstat, 8)))
+    hipStream_t s = c->stream;
+    if ((rc = dalloc(s, &c->w, kParams)) || (rc = dalloc(s, &c->m, kParams)) || (rc = dalloc(s, &c->v, kParams)) ||
+        (rc = dalloc(s, &c->grad, kParams)) || (rc = dalloc(s, &c->f1, kFragF1)) || (rc = dalloc(s, &c->f2, kFragF)) ||
+        (rc = dalloc(s, &c->f3, kFragF)) || (rc = dalloc(s, &c->d2, kFragF)) || (rc = dalloc(s, &c->d3, kFragF)) ||
+        (rc = dalloc(s, &c->wpart, (size_t)kWgradGrid * 4 * nmr::kWgradPart)) || (rc = dalloc(s, &c->dstat, 8)))
         return bail(rc);
     // He-normal weights (last layer scaled by 0.1), zero biases
     std::mt19937_64 gen(seed);
@@ -562,14 +570,14 @@ int nmr_hip_corrector_train(nmr_corrector* c, nmr_batch* b, const nmr_train_opts
         if (spart) (void)hipFree(spart);
     };
     for (int q = 0; q < 2 && !rc; q++)
-        if ((rc = dalloc(&coarse[q], nmax)) || (rc = dalloc(&tgt[q], nmax))) break;
+        if ((rc = dalloc(c->stream, &coarse[q], nmax)) || (rc = dalloc(c->stream, &tgt[q], nmax))) break;
     for (int i = 0; i < b->ninst && !rc; i++)
-        for (int q = 0; q < 2 && !rc; q++) rc = dalloc(&kept[2 * i + q], b->inst[i].n_m);
+        for (int q = 0; q < 2 && !rc; q++) rc = dalloc(c->stream, &kept[2 * i + q], b->inst[i].n_m);
     const int nlb = nmr::loss_blocks(mx, my, mz);
-    if (!rc) rc = ensure_dev_doubles(&c->lpart, &c->lpart_cap, (size_t)nlb);
+    if (!rc) rc = ensure_dev_doubles(c, &c->lpart, &c->lpart_cap, (size_t)nlb);
     // partials of the coarse sweep (tile 32 x 32, z-chunks of 8)
     const size_t nswp = (size_t)((mx + 31) / 32) * ((my + 31) / 32) * ((mz + 7) / 8);
-    if (!rc) rc = dalloc(&spart, nswp * 2);
+    if (!rc) rc = dalloc(c->stream, &spart, nswp * 2);
     if (rc) {
         cleanup();
         return rc;
@@ -715,7 +723,7 @@ int nmr_hip_run_learned(nmr_ctx* ctx, nmr_corrector* c, int64_t ncoarse, int32_t
     int rc = ensure_tensors(c, p.nx, p.ny, p.nz, false);
     if (rc) return rc;
     const int nab = nmr::apply_blocks(p.nx, p.ny, p.nz);
-    if ((rc = ensure_dev_doubles(&c->apart, &c->apart_cap, (size_t)nab * 6))) return rc;
+    if ((rc = ensure_dev_doubles(c, &c->apart, &c->apart_cap, (size_t)nab * 6))) return rc;
     if ((rc = build_rows(c, ctx))) return rc;
     // The loop works on the row buffers (the coarse step, the net, the
     // statistics and the fallback runs); it leaves the state there and the

@@ -200,6 +200,12 @@ def batch_instance(i: int) -> dict:
                 mask=mask, vel=vel)
 
 
+def batch_edges(n: int = 256) -> list:
+    """Cube edges of the first n batched-workload instances (the first draw
+    of batch_instance's generator), without generating the instances."""
+    return [int(np.random.default_rng(0x4E4D52 + 4000 + i).integers(32, 129)) for i in range(n)]
+
+
 def batch_instances(n: int = 256, workers: int = 16) -> list:
     """The first n batched-workload instances, generated in a pool of worker
     processes started fresh (spawn: safe after the GPU is in use)."""
