@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel-steps/s & %HBM roofline @256³; learned vs plain time-to-signal @1e-3"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + Vx,Vy,Vz (12)
+BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + velocity (12)
 # MacCormack (--scheme maccormack): predictor pass 29 B (as above, writing the
 # predictor) + corrector pass 37 B (reads the predictor and the state, 8 each)
 BYTES_PER_VOXEL_STEP_MAK = 66.0
@@ -36,11 +36,11 @@ BYTES_PER_VOXEL_STEP_MAK = 66.0
 # automatic tile, brick layout): rocprofv3 --pmc FETCH_SIZE (x2.0) +
 # WRITE_SIZE (x1.0) per dispatch, scaled by what the same counters read for a
 # plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
-DEFAULT_TRAFFIC = 241.2e6 + 50.4e6
+DEFAULT_TRAFFIC = 223.0e6 + 50.4e6
 # The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
 # predictor and the corrector launch of one step together (summary under
 # profiles/, gsweep_maccormack_256cube_flow_pgse.md).
-MAK_TRAFFIC = (241.5e6 + 50.4e6) + (308.8e6 + 50.4e6)
+MAK_TRAFFIC = (222.8e6 + 50.4e6) + (290.9e6 + 50.4e6)
 
 
 def _cpu_model() -> str:
