@@ -253,6 +253,19 @@ def test_flow_pgse_5000_steps(phi):
     assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (e_sig, e_vox)
 
 
+@pytest.mark.parametrize("g", [0.0, 0.075, 0.15, 0.225])
+def test_flow_pgse_gradient_sweep(g):
+    """The rest of the 256^3 workload's gradient sweep (G = 0 ... 0.3 T/m in
+    steps of 0.075; 0.3 above), same physics and 5000 steps on the same
+    36 x 40 x 44 grid, against the FP64 oracle at the same bars."""
+    case = generators.case_flow(seed=91 + 20, shape=(36, 40, 44), phi=0.20, g=g, nsteps=5000)
+    m0 = [case.mask.astype(np.float64), np.zeros(case.shape), np.zeros(case.shape)]
+    sig_ref, f_ref = oracle.run_case(case, m0=m0)
+    sig, f = _gpu_run(case, case.nsteps, m0)
+    e_sig, e_vox = _report(f"36x40x44 G {g} flow+PGSE 5000 steps", sig, sig_ref[:, :2], f, f_ref[:2])
+    assert e_sig <= SIG_TOL and e_vox <= VOX_TOL, (g, e_sig, e_vox)
+
+
 def _mirror(mask, vx, vy, vz, axis):
     """Reflect a compact problem about its upper face along axis (0 = z,
     1 = y, 2 = x): cells are mirrored, the face velocity normal to the
