@@ -374,6 +374,13 @@ def main() -> int:
                     help="plain workload: grid (default n^3)")
     ap.add_argument("--single-steps", type=int, default=200,
                     help="batch workload: steps of the one-instance-at-a-time comparison (0: skip)")
+    ap.add_argument("--bc", choices=("neumann", "periodic", "dirichlet"), default="neumann",
+                    help="plain workload: ghost rule on every axis (other than neumann: the general sweep)")
+    ap.add_argument("--field", action="store_true",
+                    help="plain workload: a seeded per-voxel field (+-20 uT) along z in place of the gradient "
+                         "field's z term (the general sweep)")
+    ap.add_argument("--coef", action="store_true",
+                    help="plain workload: seeded per-voxel D and 1/T2 (the general sweep's coefficient variant)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="time a plain device-to-device copy of MIB MiB instead (HBM ceiling / counter scale)")
     args = ap.parse_args()
@@ -395,6 +402,11 @@ def main() -> int:
 
     mak = args.scheme == "maccormack"
     bpv = BYTES_PER_VOXEL_STEP_MAK if mak else BYTES_PER_VOXEL_STEP
+    # per-voxel inputs each pass reads: the field 4 B, D and 1/T2 8 B
+    extra = (4.0 if args.field else 0.0) + (8.0 if args.coef else 0.0)
+    bpv += extra * (2 if mak else 1)
+    bc = {"neumann": _lib.NMR_BC_NEUMANN, "periodic": _lib.NMR_BC_PERIODIC,
+          "dirichlet": _lib.NMR_BC_DIRICHLET}[args.bc]
     total = args.warmup + args.steps
     shape = tuple(args.shape) if args.shape else (args.n,) * 3
     case = generators.case_flow(shape=shape, phi=args.phi, nsteps=total, rho=args.rho,
@@ -402,8 +414,14 @@ def main() -> int:
     # the streaming ceiling first: it also brings the GPU's clocks up before
     # the warm-up steps
     ceiling = copy_ceiling()
-    s = HipSolver.from_case(case, scheme=_lib.NMR_SCHEME_MACCORMACK if mak else _lib.NMR_SCHEME_UPWIND)
+    s = HipSolver.from_case(case, scheme=_lib.NMR_SCHEME_MACCORMACK if mak else _lib.NMR_SCHEME_UPWIND,
+                            bc=(bc, bc, bc))
     s.set_tile(args.variant, args.lz)
+    rng = np.random.default_rng(0x4E4D52 + 7000)
+    if args.field:
+        s.upload_field(rng.uniform(-2e-5, 2e-5, case.shape))
+    if args.coef:
+        s.upload_coefficients(np.where(case.mask != 0, case.diffusivity, 0.0), rng.uniform(0.5, 5.0, case.shape))
     # the upload converts the state to the sweep's brick layout and sizes its
     # buffers; the state then stays on the device, in bricks, from run to run
     s.upload_magnetization(case.mask.astype("float64"), np.zeros(case.shape))
@@ -420,7 +438,9 @@ def main() -> int:
     value = nvox * args.steps / wall
     kernel_s = gpu_ms * 1e-3 / args.steps
     achieved = bpv * nvox / kernel_s / 1e9
-    default = (shape, args.phi, args.variant, args.lz, args.rho) == ((256, 256, 256), 0.20, -1, 0, 10e-6)
+    plain_physics = args.bc == "neumann" and not args.field and not args.coef
+    default = (shape, args.phi, args.variant, args.lz, args.rho) == ((256, 256, 256), 0.20, -1, 0, 10e-6) \
+        and plain_physics
     traffic = (MAK_TRAFFIC if mak else DEFAULT_TRAFFIC) if default else None
     out = {
         "metric": METRIC,
@@ -438,7 +458,10 @@ def main() -> int:
         "config": {
             "workload": f"plain FP32 {'MacCormack' if mak else 'upwind'} Bloch-Torrey step, "
                         f"{'x'.join(str(v) for v in shape[::-1])}, phi={args.phi}, rho={args.rho:g} m/s, "
-                        "flow + PGSE G=0.3 T/m, complex (Mx,My)",
+                        "flow + PGSE G=0.3 T/m, complex (Mx,My)"
+                        + ("" if args.bc == "neumann" else f", {args.bc} ghosts")
+                        + (", per-voxel field along z" if args.field else "")
+                        + (", per-voxel D and 1/T2" if args.coef else ""),
             "grid": list(shape[::-1]),
             "bytes_per_voxel_step": bpv,
         },
@@ -463,7 +486,7 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
+    if not args.no_cpu and plain_physics:
         # the same grid and physics, with a schedule long enough for the sample
         cb = dataclasses.replace(case, nsteps=404, grad_sched=generators.pgse_schedule(404, case.dt, 0.3, 2e-3,
                                                                                        20e-3))

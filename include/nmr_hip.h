@@ -205,8 +205,9 @@ int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms);
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);
 
 /* Device layout of the sweeps (no reference counterpart): 1 (the default)
- * keeps the state, the velocity and the per-voxel coefficients, field and
- * phase weights in bricks of 128 bytes (8 x-voxels by 4 rows of one plane)
+ * keeps the state, the velocity (as the per-voxel face-velocity sums the
+ * sweeps read) and the per-voxel coefficients, field and phase weights in
+ * bricks of 128 bytes (8 x-voxels by 4 rows of one plane)
  * -- for the one-step sweep with any one-step tile variant and for the
  * general sweep -- so that all-solid bricks are never fetched.  The uploads
  * convert their arrays once; the state stays in bricks from run to run
@@ -257,8 +258,8 @@ int nmr_hip_batch_upload(nmr_batch* b, int32_t i, const uint8_t* mask, const dou
                          const double* vz, const double* mx, const double* my, const double* mz);
 int nmr_hip_batch_download(nmr_batch* b, int32_t i, double* mx, double* my, double* mz);
 /* Device layout of the batch (no reference counterpart): 1 (the default)
- * keeps every instance's state and velocity in bricks of 8 x-voxels by 4
- * rows from run to run, as nmr_hip_set_layout does for one domain; 0 sweeps
+ * keeps every instance's state and per-voxel face-velocity sums in bricks
+ * of 8 x-voxels by 4 rows from run to run, as nmr_hip_set_layout does for one domain; 0 sweeps
  * the row layout.  The fields are the same bits either way. */
 int nmr_hip_batch_set_layout(nmr_batch* b, int32_t brick);
 /* signal (nullable): [ninst][nsteps+1][ncomp]; step counting continues

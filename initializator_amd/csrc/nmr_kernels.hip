@@ -1668,99 +1668,14 @@ __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
 // ---------------------------------------------------------------------------
 // General z-sweep (MacCormack stages, any ghost rule, per-voxel coefficients).
 
-// One voxel of stage ST with the per-voxel kernel's arithmetic
-// (general_kernel): kn (pore flags of the six neighbours xm xp ym yp zm zp),
-// w (their diffusion weights), the face velocities (upwind stage) and their
-// sums su, sv, sw, the centre m0 and the neighbours f per component, mav
-// (the state at the start of the step, corrector stage), the precession
-// angles (value selects between the gradient field's and, along axis bnax,
-// the per-voxel field's tb), COEF: the per-voxel rates r2in / r1in.
-template <int NC, bool VEL, int ST, bool COEF>
-__device__ __forceinline__ void gupdate_voxel(const Coef& cf, const float (&kn)[6], float (&w)[6], float um,
-                                              float up, float vm, float vp, float wm, float wp, float su, float sv,
-                                              float sw, bool amask, float r2in, bool r1p, float r1in, float tb,
-                                              int bnax, int x, float thy, float thz, const float (&m0)[NC],
-                                              const float (&f)[NC][6], const float (&mav)[NC], bool pore,
-                                              float (&out)[NC])
-{
-    const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
-    if constexpr (VEL) {
-        if constexpr (ST == GEN_UPWIND) {
-            bool open[6];
-#pragma unroll
-            for (int s = 0; s < 6; s++) open[s] = !amask || kn[s] != 0.f;
-            w[0] = (um != 0.f && open[0]) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
-            w[1] = (up != 0.f && open[1]) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
-            w[2] = (vm != 0.f && open[2]) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
-            w[3] = (vp != 0.f && open[3]) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
-            w[4] = (wm != 0.f && open[4]) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
-            w[5] = (wp != 0.f && open[5]) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
-        } else if constexpr (ST == GEN_PREDICT) {
-            w[0] = fmaf(kn[0] * hxh, su, w[0]);
-            w[2] = fmaf(kn[2] * hyh, sv, w[2]);
-            w[4] = fmaf(kn[4] * hzh, sw, w[4]);
-        } else {
-            w[1] = fmaf(-kn[1] * hxh, su, w[1]);
-            w[3] = fmaf(-kn[3] * hyh, sv, w[3]);
-            w[5] = fmaf(-kn[5] * hzh, sw, w[5]);
-        }
-    }
-    const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
-    float rate2;
-    if constexpr (COEF) rate2 = r2in;
-    else rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
-    const float ty = bnax == 1 ? tb : thy, tz = bnax == 2 ? tb : thz;
-    float d[NC];
-#pragma unroll
-    for (int q = 0; q < NC; q++) {
-        const float m = m0[q];
-        float t = w[0] * (f[q][0] - m);
-        t = fmaf(w[1], f[q][1] - m, t);
-        t = fmaf(w[2], f[q][2] - m, t);
-        t = fmaf(w[3], f[q][3] - m, t);
-        t = fmaf(w[4], f[q][4] - m, t);
-        d[q] = fmaf(w[5], f[q][5] - m, t);
-    }
-    if constexpr (NC == 3) {
-        float rate1;
-        if (COEF && r1p) rate1 = r1in;
-        else rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
-        const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
-        const float nn[3] = {cf.n0, cf.n1, cf.n2};
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const float mpar = nn[q] * par;
-            d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
-            d[q] = fmaf(rate2, mpar - m0[q], d[q]);
-        }
-        const float tx = bnax == 0 ? tb : cf.bgx * ((float)x + 0.5f);
-        d[0] = fmaf(tz, m0[1], fmaf(-ty, m0[2], d[0]));
-        d[1] = fmaf(-tz, m0[0], fmaf(tx, m0[2], d[1]));
-        d[2] = fmaf(ty, m0[0], fmaf(-tx, m0[1], d[2]));
-    } else {
-#pragma unroll
-        for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
-        if constexpr (NC == 2) {
-            d[0] = fmaf(tz, m0[1], d[0]);
-            d[1] = fmaf(-tz, m0[0], d[1]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < NC; q++) {
-        if constexpr (ST == GEN_CORRECT) out[q] = pore ? 0.5f * (mav[q] + (m0[q] + d[q])) : 0.f;
-        else if constexpr (ST == GEN_PREDICT) out[q] = pore ? m0[q] + d[q] : 0.f; // the predictor holds 0 at solids
-        else out[q] = pore ? m0[q] + d[q] : m0[q];
-    }
-}
-
 // Update of one 4-voxel chunk with the per-voxel kernel's arithmetic
 // (general_kernel): stage ST of a step, face weights from the mask and, with
 // COEF, harmonic means of per-voxel diffusivities; ghost neighbours already
 // hold their rule's value and the pore mask.  dxl / dxr: diffusivity of the x
 // neighbours of elements 0 and 3; mav: the state at the start of the step
 // (corrector stage).  SUMS (MacCormack stages only): vx4, vy4, vz0 carry the
-// voxels' face-velocity sums um + up, vm + vp, wm + wp (csum_kernel) and the
-// other velocity arguments are not read.
+// voxels' face-velocity sums um + up, vm + vp, wm + wp (adv_kernel, centred)
+// and the other velocity arguments are not read.
 template <int NC, bool VEL, int ST, bool COEF, bool SUMS = false>
 __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC], const f4 (&fzm)[NC],
                                               const f4 (&fzp)[NC], const f4 (&fym)[NC], const f4 (&fyp)[NC],
@@ -1774,6 +1689,7 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
                                               const f4& bnc, int bnax, float dtg, f4 (&out)[NC])
 {
     static_assert(!SUMS || ST != GEN_UPWIND, "velocity sums serve the MacCormack stages");
+    const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
     const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
 #pragma unroll
     for (int e = 0; e < 4; e++) {
@@ -1791,42 +1707,94 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
 #pragma unroll
             for (int s = 0; s < 6; s++) w[s] = cd[s / 2] * kn[s];
         }
-        float um = 0.f, up = 0.f, vm = 0.f, vp = 0.f, wm = 0.f, wp = 0.f, su = 0.f, sv = 0.f, sw = 0.f;
-        if constexpr (VEL) {
-            if constexpr (SUMS) {
-                su = vx4[e];
-                sv = vy4[e];
-                sw = vz0[e];
+        if constexpr (VEL && SUMS) {
+            if constexpr (ST == GEN_PREDICT) {
+                w[0] = fmaf(kn[0] * hxh, vx4[e], w[0]);
+                w[2] = fmaf(kn[2] * hyh, vy4[e], w[2]);
+                w[4] = fmaf(kn[4] * hzh, vz0[e], w[4]);
             } else {
-                um = vx4[e];
-                up = (e < 3) ? vx4[e + 1] : vxe;
-                vm = vy4[e];
-                vp = vy4n[e];
-                wm = vz0[e];
-                wp = vz1[e];
-                su = um + up;
-                sv = vm + vp;
-                sw = wm + wp;
+                w[1] = fmaf(-kn[1] * hxh, vx4[e], w[1]);
+                w[3] = fmaf(-kn[3] * hyh, vy4[e], w[3]);
+                w[5] = fmaf(-kn[5] * hzh, vz0[e], w[5]);
+            }
+        } else if constexpr (VEL) {
+            const float um = vx4[e], up = (e < 3) ? vx4[e + 1] : vxe;
+            const float vm = vy4[e], vp = vy4n[e];
+            const float wm = vz0[e], wp = vz1[e];
+            const float su = um + up, sv = vm + vp, sw = wm + wp;
+            if constexpr (ST == GEN_UPWIND) {
+                bool open[6];
+#pragma unroll
+                for (int s = 0; s < 6; s++) open[s] = !amask || kn[s] != 0.f;
+                w[0] = (um != 0.f && open[0]) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
+                w[1] = (up != 0.f && open[1]) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
+                w[2] = (vm != 0.f && open[2]) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
+                w[3] = (vp != 0.f && open[3]) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
+                w[4] = (wm != 0.f && open[4]) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
+                w[5] = (wp != 0.f && open[5]) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
+            } else if constexpr (ST == GEN_PREDICT) {
+                w[0] = fmaf(kn[0] * hxh, su, w[0]);
+                w[2] = fmaf(kn[2] * hyh, sv, w[2]);
+                w[4] = fmaf(kn[4] * hzh, sw, w[4]);
+            } else {
+                w[1] = fmaf(-kn[1] * hxh, su, w[1]);
+                w[3] = fmaf(-kn[3] * hyh, sv, w[3]);
+                w[5] = fmaf(-kn[5] * hzh, sw, w[5]);
             }
         }
+        const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
+        float rate2;
+        if constexpr (COEF) rate2 = r2c[e];
+        else rate2 = fmaf(cf.s2z, nsz, fmaf(cf.s2y, nsy, fmaf(cf.s2x, nsx, cf.r2)));
+        // precession angles: the gradient field's, or along axis bnax the
+        // per-voxel field's (value selects, as in general_kernel)
         const float tb = bnax >= 0 ? dtg * bnc[e] : 0.f;
-        float m0[NC], f[NC][6], mv[NC], o[NC];
+        const float ty = bnax == 1 ? tb : thy, tz = bnax == 2 ? tb : thz;
+        float m0[NC], d[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++) {
-            m0[q] = mc[q][e];
-            f[q][0] = (e > 0) ? mc[q][e - 1] : xl[q];
-            f[q][1] = (e < 3) ? mc[q][e + 1] : xr[q];
-            f[q][2] = fym[q][e];
-            f[q][3] = fyp[q][e];
-            f[q][4] = fzm[q][e];
-            f[q][5] = fzp[q][e];
-            mv[q] = mav[q][e];
+            const float m = mc[q][e];
+            m0[q] = m;
+            const float f[6] = {(e > 0) ? mc[q][e - 1] : xl[q], (e < 3) ? mc[q][e + 1] : xr[q], fym[q][e], fyp[q][e],
+                                fzm[q][e], fzp[q][e]};
+            float t = w[0] * (f[0] - m);
+            t = fmaf(w[1], f[1] - m, t);
+            t = fmaf(w[2], f[2] - m, t);
+            t = fmaf(w[3], f[3] - m, t);
+            t = fmaf(w[4], f[4] - m, t);
+            d[q] = fmaf(w[5], f[5] - m, t);
+        }
+        if constexpr (NC == 3) {
+            float rate1;
+            if (COEF && r1p) rate1 = r1c[e];
+            else rate1 = fmaf(cf.s1z, nsz, fmaf(cf.s1y, nsy, fmaf(cf.s1x, nsx, cf.r1)));
+            const float par = fmaf(cf.n2, m0[2], fmaf(cf.n1, m0[1], cf.n0 * m0[0]));
+            const float nn[3] = {cf.n0, cf.n1, cf.n2};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float mpar = nn[q] * par;
+                d[q] = fmaf(rate1, fmaf(cf.peq, nn[q], -mpar), d[q]);
+                d[q] = fmaf(rate2, mpar - m0[q], d[q]);
+            }
+            const float tx = bnax == 0 ? tb : cf.bgx * ((float)(ix + e) + 0.5f);
+            d[0] = fmaf(tz, m0[1], fmaf(-ty, m0[2], d[0]));
+            d[1] = fmaf(-tz, m0[0], fmaf(tx, m0[2], d[1]));
+            d[2] = fmaf(ty, m0[0], fmaf(-tx, m0[1], d[2]));
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; q++) d[q] = fmaf(-rate2, m0[q], d[q]);
+            if constexpr (NC == 2) {
+                d[0] = fmaf(tz, m0[1], d[0]);
+                d[1] = fmaf(-tz, m0[0], d[1]);
+            }
         }
         const bool pore = ((kc >> (8 * e)) & 0xffu) != 0;
-        gupdate_voxel<NC, VEL, ST, COEF>(cf, kn, w, um, up, vm, vp, wm, wp, su, sv, sw, amask, r2c[e], r1p, r1c[e],
-                                         tb, bnax, ix + e, thy, thz, m0, f, mv, pore, o);
 #pragma unroll
-        for (int q = 0; q < NC; q++) out[q][e] = o[q];
+        for (int q = 0; q < NC; q++) {
+            if constexpr (ST == GEN_CORRECT) out[q][e] = pore ? 0.5f * (mav[q][e] + (m0[q] + d[q])) : 0.f;
+            else if constexpr (ST == GEN_PREDICT) out[q][e] = pore ? m0[q] + d[q] : 0.f; // the predictor holds 0 at solids
+            else out[q][e] = pore ? m0[q] + d[q] : m0[q];
+        }
     }
 }
 
