@@ -352,7 +352,7 @@ GGeom general_geometry(const nmr_ctx* c, bool gbrick)
     int lz = c->lz > 0 ? c->lz
                        : round_lz(g.ntx * g.nty, p.nz,
                                   nmr::gsweep_resident_blocks(c->nc, c->vel, c->hetero, g.lanes, gbrick,
-                                                              c->weighted));
+                                                              c->weighted, p.scheme != NMR_SCHEME_MACCORMACK));
     g.lz = std::min(std::max(lz, 1), p.nz);
     g.ntz = (p.nz + g.lz - 1) / g.lz;
     g.nwg = g.ntx * g.nty * g.ntz;

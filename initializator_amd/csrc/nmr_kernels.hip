@@ -2218,8 +2218,14 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
 
 // W: phase-weighted sums, 2*NC per workgroup (the NC sums weighted by w[0],
 // then the NC weighted by w[1], as field_sum_kernel lays them out)
+// The coefficient variant of the corrector (and of the upwind stage) holds
+// just over 168 VGPRs, i.e. 2 workgroups per CU; for one or two components
+// the corrector is asked to fit 3 (168 VGPRs, at most one register spilled:
+// 188.5 against 194.1 us per MacCormack step at 256^3).  Three components
+// would spill dozens and keep 2.
 template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false>
-__global__ void __launch_bounds__(256) gsweep_kernel(GSweepArgs a)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((COEF && !W && ST == GEN_CORRECT && NC <= 2) ? 3 : 1))) gsweep_kernel(GSweepArgs a)
 {
     constexpr int NS = W ? 2 * NC : NC;
     __shared__ double sred[4][NS];
@@ -2605,20 +2611,22 @@ int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
     return resident_blocks(f);
 }
 
-int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, bool weighted)
+int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, bool weighted, bool upwind)
 {
-    // the upwind / corrector kernel (the predictor holds fewer registers);
-    // on bricks with fused phase-weighted sums its instantiation with 2*NC
-    // accumulators
+    // the kernel a step launches with the most registers: the upwind stage,
+    // or the MacCormack corrector (the predictor holds fewer); on bricks with
+    // fused phase-weighted sums its instantiation with 2*NC accumulators
     const void* f = nullptr;
-#define NMR_PICK(NC_, V_, C_)                                                                   \
-    if (brick && weighted) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true, true>; \
-    else if (brick) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 16, true>;          \
-    else if (lanes == 64) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 64>;          \
-    else if (lanes == 32) f = (const void*)gsweep_kernel<NC_, V_, GEN_CORRECT, C_, 32>;          \
+#define NMR_PICK_ST(NC_, V_, ST_, C_)                                                                \
+    if (brick && weighted) f = (const void*)gsweep_kernel<NC_, V_, ST_, C_, 16, true, true>;          \
+    else if (brick) f = (const void*)gsweep_kernel<NC_, V_, ST_, C_, 16, true>;                       \
+    else if (lanes == 64) f = (const void*)gsweep_kernel<NC_, V_, ST_, C_, 64>;                       \
+    else if (lanes == 32) f = (const void*)gsweep_kernel<NC_, V_, ST_, C_, 32>;                       \
     else return -1;
-#define NMR_PICKC(NC_)                                                                          \
-    if (coef) { if (vel) { NMR_PICK(NC_, true, true) } else { NMR_PICK(NC_, false, true) } }    \
+#define NMR_PICK(NC_, V_, C_)                                                                         \
+    if (upwind) { NMR_PICK_ST(NC_, V_, GEN_UPWIND, C_) } else { NMR_PICK_ST(NC_, V_, GEN_CORRECT, C_) }
+#define NMR_PICKC(NC_)                                                                                \
+    if (coef) { if (vel) { NMR_PICK(NC_, true, true) } else { NMR_PICK(NC_, false, true) } }          \
     else { if (vel) { NMR_PICK(NC_, true, false) } else { NMR_PICK(NC_, false, false) } }
     if (nc == 1) { NMR_PICKC(1) }
     else if (nc == 2) { NMR_PICKC(2) }
@@ -2626,6 +2634,7 @@ int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, b
     else return -1;
 #undef NMR_PICKC
 #undef NMR_PICK
+#undef NMR_PICK_ST
     return resident_blocks(f);
 }
 
