@@ -60,3 +60,12 @@ def test_b_value_round_trip():
 
 def test_stable_dt_config_value():
     assert generators.stable_dt(1e-6, 2.3e-9) == pytest.approx(2.8986e-5, rel=1e-4)
+
+
+def test_batch_edges_match_the_instances():
+    """The cheap edge scan (used to pick the smallest batched instances
+    without generating all 256) draws what batch_instance draws first."""
+    edges = generators.batch_edges(256)
+    assert len(edges) == 256 and all(32 <= e <= 128 for e in edges)
+    for i in (0, 7, 92, 255):
+        assert generators.batch_instance(i)["shape"] == (edges[i],) * 3
