@@ -40,7 +40,7 @@ DEFAULT_TRAFFIC = 223.0e6 + 50.4e6
 # The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
 # predictor and the corrector launch of one step together (summary under
 # profiles/, gsweep_maccormack_256cube_flow_pgse.md).
-MAK_TRAFFIC = (222.8e6 + 50.4e6) + (290.9e6 + 50.4e6)
+MAK_TRAFFIC = (222.8e6 + 50.4e6) + (290.5e6 + 50.4e6)
 
 
 def _cpu_model() -> str:
