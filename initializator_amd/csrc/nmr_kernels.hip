@@ -1134,10 +1134,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
     block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
-// Persistent batched sweep: one launch advances every instance of a batch by
-// one step.  The grid is sized to the resident workgroups; workgroup w takes
-// tiles w, w + gridDim.x, ... of a list sorted by cost, each tile naming its
-// instance.  Instance descriptors and the step counter live in device memory
+// Batched sweep: one launch advances every instance of a batch by one step.
+// The grid holds one workgroup per tile, or is sized to the resident
+// workgroups (persistent); workgroup w takes tiles w, w + gridDim.x, ... of
+// a list in descending cost (permuted in runs so that each XCD takes
+// neighbouring tiles), each tile naming its instance.  Instance descriptors and the step counter live in device memory
 // so that a chunk of launches can be captured in a graph and replayed.
 template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
 __global__ void __launch_bounds__(256) batch_kernel(BatchArgs b)
