@@ -36,6 +36,7 @@ using nmri::g_last_error;
 using nmri::round_up4;
 
 constexpr int kChunkSteps = 512; // steps whose partials are held before a reduction
+constexpr size_t kPinnedSignal = (size_t)1 << 17; // doubles of pinned host staging for signal rows (1 MiB)
 
 } // namespace
 
@@ -113,6 +114,7 @@ void free_all(nmr_ctx* c)
     if (c->vz) (void)hipFree(c->vz);
     if (c->partials) (void)hipFree(c->partials);
     if (c->dsignal) (void)hipFree(c->dsignal);
+    if (c->hsignal) (void)hipHostFree(c->hsignal);
     if (c->dflag) (void)hipFree(c->dflag);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -378,13 +380,42 @@ void plain_geometry(const nmr_ctx* c, Launch* L1, Launch* L2, bool* brick)
 int finish_signal(nmr_ctx* c, int64_t nsteps, bool have0, const double* sig0, double* signal)
 {
     const size_t ns = (size_t)c->nsig(), first = have0 ? 1 : 0;
-    if (signal && (size_t)nsteps + 1 > first)
-        HIP_TRY(hipMemcpyAsync(signal + first * ns, c->dsignal + first * ns,
-                               sizeof(double) * ns * ((size_t)nsteps + 1 - first), hipMemcpyDeviceToHost, c->stream));
-    if (!signal && (nsteps > 0 || !have0))
-        HIP_TRY(hipMemcpyAsync(c->sig_now, c->dsignal + (size_t)nsteps * ns, sizeof(double) * ns,
-                               hipMemcpyDeviceToHost, c->stream));
+    // rows [r0, r0 + n) of dsignal go to dst: the caller's rows, or only the
+    // last row (the signal of the new state) when the caller takes none
+    size_t r0 = 0, n = 0;
+    double* dst = nullptr;
+    if (signal && (size_t)nsteps + 1 > first) {
+        r0 = first;
+        n = (size_t)nsteps + 1 - first;
+        dst = signal + first * ns;
+    } else if (!signal && (nsteps > 0 || !have0)) {
+        r0 = (size_t)nsteps;
+        n = 1;
+        dst = c->sig_now;
+    }
+    // Through a pinned staging buffer the copy is one DMA that the stream
+    // synchronisation below waits for; into pageable memory the runtime
+    // blocks for the stream, stages the copy and blocks again (about 20 us
+    // more per run).  Not under the no-allocation entry point, and not for
+    // long signals (over 1 MiB), where that cost is noise.
+    const size_t cnt = n * ns;
+    double* stage = nullptr;
+    if (cnt && !c->no_alloc && cnt <= kPinnedSignal) {
+        if (c->hsignal_cap < cnt) {
+            if (c->hsignal) HIP_TRY(hipHostFree(c->hsignal));
+            c->hsignal = nullptr;
+            c->hsignal_cap = 0;
+            const size_t cap = std::min(kPinnedSignal, std::max(cnt, (size_t)1024));
+            HIP_TRY(hipHostMalloc((void**)&c->hsignal, cap * sizeof(double), hipHostMallocDefault));
+            c->hsignal_cap = cap;
+        }
+        stage = c->hsignal;
+    }
+    if (cnt)
+        HIP_TRY(hipMemcpyAsync(stage ? stage : dst, c->dsignal + r0 * ns, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                               c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (stage) std::copy(stage, stage + cnt, dst);
     if (signal) {
         if (have0) std::copy(sig0, sig0 + ns, signal);
         std::copy(signal + (size_t)nsteps * ns, signal + ((size_t)nsteps + 1) * ns, c->sig_now);

@@ -104,6 +104,8 @@ struct nmr_ctx {
     size_t partials_cap = 0; // doubles
     double* dsignal = nullptr;
     size_t dsignal_cap = 0;  // doubles
+    double* hsignal = nullptr; // pinned host staging of a run's signal rows (at most kPinnedSignal doubles)
+    size_t hsignal_cap = 0;    // doubles
     int cur = 0;
     bool has_mask = false, has_mag = false;
     int64_t mask_version = 0; // bumped by every mask upload
