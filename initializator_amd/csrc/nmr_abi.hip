@@ -377,6 +377,24 @@ void plain_geometry(const nmr_ctx* c, Launch* L1, Launch* L2, bool* brick)
 // Copies a run's signal rows to the host -- row 0 from the kept signal of
 // the current state when there was one (sig0) -- and keeps the last row as
 // the signal of the new current state.  Waits for the stream.
+// Pinned host staging for at least `doubles` signal values (grown by
+// doubling, at most kPinnedSignal); sized by the uploads so that runs of up
+// to kChunkSteps steps never allocate it.
+int ensure_host_signal(nmr_ctx* c, size_t doubles)
+{
+    if (doubles <= c->hsignal_cap) return NMR_OK;
+    doubles = std::min(kPinnedSignal, std::max(doubles, 2 * c->hsignal_cap));
+    if (c->hsignal) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipHostFree(c->hsignal));
+        c->hsignal = nullptr;
+        c->hsignal_cap = 0;
+    }
+    HIP_TRY(hipHostMalloc((void**)&c->hsignal, doubles * sizeof(double), hipHostMallocDefault));
+    c->hsignal_cap = doubles;
+    return NMR_OK;
+}
+
 int finish_signal(nmr_ctx* c, int64_t nsteps, bool have0, const double* sig0, double* signal)
 {
     const size_t ns = (size_t)c->nsig(), first = have0 ? 1 : 0;
@@ -401,14 +419,8 @@ int finish_signal(nmr_ctx* c, int64_t nsteps, bool have0, const double* sig0, do
     const size_t cnt = n * ns;
     double* stage = nullptr;
     if (cnt && !c->no_alloc && cnt <= kPinnedSignal) {
-        if (c->hsignal_cap < cnt) {
-            if (c->hsignal) HIP_TRY(hipHostFree(c->hsignal));
-            c->hsignal = nullptr;
-            c->hsignal_cap = 0;
-            const size_t cap = std::min(kPinnedSignal, std::max(cnt, (size_t)1024));
-            HIP_TRY(hipHostMalloc((void**)&c->hsignal, cap * sizeof(double), hipHostMallocDefault));
-            c->hsignal_cap = cap;
-        }
+        int rc = ensure_host_signal(c, cnt);
+        if (rc) return rc;
         stage = c->hsignal;
     }
     if (cnt)
@@ -744,6 +756,7 @@ int presize_buffers(nmr_ctx* c)
     need = std::max(need, (size_t)nmr::field_sum_blocks(f) * ns);
     int rc = ensure_partials(c, need);
     if (!rc) rc = ensure_signal(c, (size_t)(kChunkSteps + 1) * ns);
+    if (!rc) rc = ensure_host_signal(c, (size_t)(kChunkSteps + 1) * ns);
     return rc;
 }
 
