@@ -37,6 +37,7 @@ using nmri::round_up4;
 
 constexpr int kChunkSteps = 512; // steps whose partials are held before a reduction
 constexpr size_t kPinnedSignal = (size_t)1 << 17; // doubles of pinned host staging for signal rows (1 MiB)
+constexpr size_t kBatchStage = (size_t)1 << 20;   // doubles of pinned staging for a batch run's signal (8 MiB)
 
 } // namespace
 
@@ -1598,6 +1599,7 @@ void batch_free(nmr_batch* b)
     for (void* p : {(void*)b->d_inst, (void*)b->d_tiles, (void*)b->d_off, (void*)b->d_ids, (void*)b->d_step,
                     (void*)b->d_part, (void*)b->d_signal, (void*)b->d_fpart, (void*)b->d_flag})
         if (p) (void)hipFree(p);
+    if (b->h_signal) (void)hipHostFree(b->h_signal);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -2246,10 +2248,14 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
     if ((rc = batch_partials(b, (size_t)std::max(S, 2)))) return rc;
     const size_t need = (size_t)b->ninst * rows * nc;
     if (need > b->signal_cap) {
+        // room for runs of up to kChunkSteps steps, so that a short warm-up
+        // run leaves nothing to allocate for the next one
+        const size_t cap = std::max({need, 2 * b->signal_cap, (size_t)b->ninst * (kChunkSteps + 1) * nc});
         if (b->d_signal) HIP_TRY(hipFree(b->d_signal));
         b->d_signal = nullptr;
-        if ((rc = balloc(b->stream, &b->d_signal, need))) return rc;
-        b->signal_cap = need;
+        b->signal_cap = 0;
+        if ((rc = balloc(b->stream, &b->d_signal, cap))) return rc;
+        b->signal_cap = cap;
     }
     // row 0: signal of every instance before the run (kept from the last
     // run, else summed over the state where it lives)
@@ -2323,20 +2329,42 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal)
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     b->home = brick ? nmri::kHomeBricks : nmri::kHomeRows;
-    if (signal)
-        HIP_TRY(hipMemcpyAsync(signal, b->d_signal, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
-    // the last row of every instance: the signal of its new state (row 0 of
-    // an instance that kept its signal is not written on the device)
-    std::vector<double> last((size_t)b->ninst * nc);
-    HIP_TRY(hipMemcpy2DAsync(last.data(), sizeof(double) * nc, b->d_signal + (size_t)nsteps * nc,
-                             sizeof(double) * rows * nc, sizeof(double) * nc, b->ninst, hipMemcpyDeviceToHost,
-                             b->stream));
+    // The signal rows go to the host in one copy of the whole
+    // [ninst][rows][nc] block: into the caller's buffer, else into pinned
+    // staging (up to kBatchStage doubles), and the last row of every
+    // instance -- the signal of its new state -- is read from there.  Only
+    // past that size are the last rows copied alone: a strided copy, which
+    // the runtime splits into one small transfer per instance (256 of them
+    // took 2-11 ms).  Row 0 of an instance that kept its signal is not
+    // written on the device.
+    double* host = signal;
+    if (!host && need <= kBatchStage) {
+        if (need > b->h_signal_cap) {
+            const size_t cap = std::min(kBatchStage, std::max(need, 2 * b->h_signal_cap));
+            if (b->h_signal) HIP_TRY(hipHostFree(b->h_signal));
+            b->h_signal = nullptr;
+            b->h_signal_cap = 0;
+            HIP_TRY(hipHostMalloc((void**)&b->h_signal, cap * sizeof(double), hipHostMallocDefault));
+            b->h_signal_cap = cap;
+        }
+        host = b->h_signal;
+    }
+    std::vector<double> last;
+    if (host) {
+        HIP_TRY(hipMemcpyAsync(host, b->d_signal, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
+    } else {
+        last.resize((size_t)b->ninst * nc);
+        HIP_TRY(hipMemcpy2DAsync(last.data(), sizeof(double) * nc, b->d_signal + (size_t)nsteps * nc,
+                                 sizeof(double) * rows * nc, sizeof(double) * nc, b->ninst, hipMemcpyDeviceToHost,
+                                 b->stream));
+    }
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (int i = 0; i < b->ninst; i++) {
         const size_t o = (size_t)i * nc;
-        if (nsteps > 0 || !have0[i]) std::copy(last.begin() + o, last.begin() + o + nc, b->sig_now.begin() + o);
+        const double* lr = host ? host + ((size_t)i * rows + (size_t)nsteps) * nc : last.data() + o;
+        if (nsteps > 0 || !have0[i]) std::copy(lr, lr + nc, b->sig_now.begin() + o);
         if (signal && have0[i]) std::copy(sig0.begin() + o, sig0.begin() + o + nc, signal + (size_t)i * rows * nc);
     }
     std::fill(b->sig_ok.begin(), b->sig_ok.end(), 1);

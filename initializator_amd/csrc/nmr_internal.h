@@ -175,6 +175,8 @@ struct nmr_batch {
     double* d_part = nullptr;
     size_t part_cap = 0;
     double* d_signal = nullptr;
+    double* h_signal = nullptr; // pinned host staging of a run's signal block
+    size_t h_signal_cap = 0;    // doubles
     size_t signal_cap = 0;
     double* d_fpart = nullptr; // field-sum partials (max nz)
     int* d_flag = nullptr;
