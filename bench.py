@@ -114,6 +114,30 @@ def copy_ceiling(mib: int = 1024, reps: int = 10) -> float:
     return 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+def mfma_ceiling(iters: int = 16384, reps: int = 5) -> float:
+    """TFLOP/s of the library's matrix-core rate kernel: dense BF16
+    v_mfma_f32_16x16x32_bf16 chains on register operands, four waves per
+    SIMD, no memory traffic -- the ceiling the convolutions are priced at.
+    16384 rounds (about 3.6 ms per launch) let the clocks settle: shorter
+    launches read lower (256 rounds: about 2.1 PFLOP/s)."""
+    import ctypes
+
+    from initializator_amd import _lib
+
+    lib = _lib.load()
+    tf = ctypes.c_double()
+    rc = lib.nmr_hip_mfma_rate(iters, reps, ctypes.byref(tf))
+    if rc != 0:
+        raise RuntimeError(f"nmr_hip_mfma_rate: {rc}")
+    return tf.value
+
+
+# MFMA work of the corrector's three layers per voxel: 35 instructions of
+# 16 384 FLOP per 16-voxel row (7 for 6(->8)->16, 14 for 16->16, 14 for
+# 16->2), padding included
+CONV_MFMA_FLOP_PER_VOXEL = 35 * 16384 / 16
+
+
 def copy_probe(mib: int) -> int:
     """The library's copy kernel on mib MiB: its GB/s is the streaming ceiling
     on this GPU, and its FETCH_SIZE / WRITE_SIZE counters (rocprofv3 --pmc)
@@ -305,6 +329,11 @@ def learned_main(args) -> int:
     ref = plain[::factor]
     err = float(np.linalg.norm(sig - ref) / np.linalg.norm(ref))
     net.close()
+    # the floor the three layers' MFMA work puts under one coarse step, at
+    # the measured matrix-core rate, against the 3x budget per coarse step
+    mfma_tf = mfma_ceiling()
+    conv_tflop = CONV_MFMA_FLOP_PER_VOXEL * case.nvox / 1e12
+    budget_us = plain_s / ncoarse / 3 * 1e6
     out = {
         "metric": "learned vs plain time-to-signal @1e-3",
         "value": plain_s / learned_s,
@@ -333,6 +362,13 @@ def learned_main(args) -> int:
             "train_samples": int(losses.size),
             "train_s": train_s,
             "train_loss_first_last": [float(losses[:2].mean()), float(losses[-2:].mean())],
+        },
+        "mfma_floor": {
+            "ceiling_tflops": mfma_tf,
+            "peak_tflops": 2500.0,
+            "conv_tflop_per_coarse_step": conv_tflop,
+            "conv_floor_us": conv_tflop / mfma_tf * 1e6,
+            "budget_us_per_coarse_step_at_3x": budget_us,
         },
     }
     print(json.dumps(out), flush=True)

@@ -200,6 +200,15 @@ int nmr_hip_signal(nmr_ctx* ctx, double* out);
  * compared with.  Allocates and frees its two buffers. */
 int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms);
 
+/* Measurement, no reference counterpart: dense BF16 matrix-core rate
+ * (TFLOP/s, the fastest of `reps` timed launches) of the instruction the
+ * learned corrector's convolutions issue, v_mfma_f32_16x16x32_bf16, with
+ * every wave running `iters` rounds of 8 independent accumulation chains
+ * on register operands (four waves per SIMD) -- the ceiling the
+ * convolutions' MFMA work is compared with.  Allocates and frees a small
+ * output buffer.  iters in [1, 2^24]. */
+int nmr_hip_mfma_rate(int64_t iters, int32_t reps, double* tflops);
+
 /* GPU time (ms) between HIP events recorded on the context's stream around
  * the kernels of the last nmr_hip_run, and the number of sweep launches. */
 int nmr_hip_last_run_ms(nmr_ctx* ctx, double* ms, int64_t* sweeps);

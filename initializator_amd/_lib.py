@@ -59,6 +59,7 @@ EXPORTS = (
     "nmr_hip_run",
     "nmr_hip_signal",
     "nmr_hip_copy_bandwidth",
+    "nmr_hip_mfma_rate",
     "nmr_hip_batch_set_layout",
     "nmr_hip_run_static",
     "nmr_hip_pulse",
@@ -200,6 +201,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_run": (c_int, [vp, c_int64, dp, dp]),
         "nmr_hip_signal": (c_int, [vp, dp]),
         "nmr_hip_copy_bandwidth": (c_int, [c_int64, c_int32, dp]),
+        "nmr_hip_mfma_rate": (c_int, [c_int64, c_int32, dp]),
         "nmr_hip_batch_set_layout": (c_int, [vp, c_int32]),
         "nmr_hip_run_static": (c_int, [vp, c_int64, dp, dp]),
         "nmr_hip_pulse": (c_int, [vp, c_int32, c_double, c_double, c_double, c_double]),

@@ -1352,6 +1352,51 @@ int nmr_hip_copy_bandwidth(int64_t bytes, int32_t reps, double* ms)
     return rc;
 }
 
+int nmr_hip_mfma_rate(int64_t iters, int32_t reps, double* tflops)
+{
+    if (!tflops) return fail(NMR_ERR_NULL, "nmr_hip_mfma_rate: null argument");
+    if (iters < 1 || iters > (1 << 24) || reps < 1)
+        return fail(NMR_ERR_INVALID, "iters %lld, reps %d", (long long)iters, (int)reps);
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cus = std::max(cus, 1);
+    const int grid = 4 * cus; // four workgroups of four waves per CU: four waves per SIMD
+    float* out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = NMR_OK;
+    auto check = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == NMR_OK)
+            rc = fail(e == hipErrorOutOfMemory ? NMR_ERR_NOMEM : NMR_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+        return rc == NMR_OK;
+    };
+    double best = 0.0;
+    if (check(hipMalloc((void**)&out, (size_t)grid * 256 * sizeof(float)), "hipMalloc") &&
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
+        check(hipEventCreate(&e0), "hipEventCreate") && check(hipEventCreate(&e1), "hipEventCreate") &&
+        check(nmr::launch_mfma_rate((int)iters, grid, out, st), "mfma")) {
+        for (int r = 0; r < reps && rc == NMR_OK; r++) {
+            float t = 0.f;
+            if (check(hipEventRecord(e0, st), "hipEventRecord") &&
+                check(nmr::launch_mfma_rate((int)iters, grid, out, st), "mfma") &&
+                check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "hipEventSynchronize") &&
+                check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))
+                if (best == 0.0 || t < best) best = t;
+        }
+    }
+    if (rc == NMR_OK) {
+        const double flop = (double)grid * 4.0 * (double)iters * 8.0 * 16384.0;
+        *tflops = best > 0.0 ? flop / (best * 1e-3) / 1e12 : 0.0;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (out) (void)hipFree(out);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
 int nmr_hip_device_count(int32_t* count)
 {
     if (!count) return fail(NMR_ERR_NULL, "nmr_hip_device_count: null argument");

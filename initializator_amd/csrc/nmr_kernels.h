@@ -263,6 +263,8 @@ int apply_blocks(int nx, int ny, int nz);
 hipError_t launch_apply(const ApplyArgs& a, hipStream_t s);
 hipError_t launch_stats_reduce(const double* part, int nblocks, double* out, hipStream_t s);
 hipError_t launch_row_flags(const uint8_t* mask, int nx, int ny, int nz, int pitch, uint8_t* flag, hipStream_t s);
+// grid workgroups of 256 threads, each wave `iters` x 8 MFMA 16x16x32 BF16; out: grid*256 floats
+hipError_t launch_mfma_rate(int iters, int grid, float* out, hipStream_t s);
 
 int num_variants();
 // the compacted brick sweep's variant index (the last one)

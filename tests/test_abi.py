@@ -181,3 +181,13 @@ def test_batch_null_arguments(lib):
     assert lib.nmr_hip_batch_upload(None, 0, None, None, None, None, None, None, None) == _lib.NMR_ERR_NULL
     assert lib.nmr_hip_batch_autotune(None, 1, None, None, None) == _lib.NMR_ERR_NULL
     assert lib.nmr_hip_batch_set_tile(None, 0, 0, 0, 1) == _lib.NMR_ERR_NULL
+
+
+def test_measurement_entry_points_reject_bad_arguments(lib):
+    out = ctypes.c_double()
+    assert lib.nmr_hip_mfma_rate(16, 1, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_mfma_rate(0, 1, ctypes.byref(out)) == _lib.NMR_ERR_INVALID
+    assert lib.nmr_hip_mfma_rate(1 << 25, 1, ctypes.byref(out)) == _lib.NMR_ERR_INVALID
+    assert lib.nmr_hip_mfma_rate(16, 0, ctypes.byref(out)) == _lib.NMR_ERR_INVALID
+    assert lib.nmr_hip_copy_bandwidth(1 << 20, 1, None) == _lib.NMR_ERR_NULL
+    assert lib.nmr_hip_copy_bandwidth(1024, 1, ctypes.byref(out)) == _lib.NMR_ERR_INVALID

@@ -261,3 +261,18 @@ def test_trained_corrector_gains_over_the_coarse_step_alone():
     assert e_coarse > 3 * e_learn
     net.close()
     zero.close()
+
+
+def test_matrix_core_rate_is_measured():
+    """nmr_hip_mfma_rate times the convolutions' MFMA instruction on
+    register operands: a positive rate no higher than the dense BF16 peak
+    (2.5 PFLOP/s, with 10 % for clock boost)."""
+    import ctypes
+
+    from initializator_amd import _lib
+
+    lib = _lib.load()
+    tf = ctypes.c_double()
+    assert lib.nmr_hip_mfma_rate(256, 2, ctypes.byref(tf)) == 0, lib.nmr_hip_last_error().decode()
+    print(f"[mfma] {tf.value:.0f} TFLOP/s dense BF16 (v_mfma_f32_16x16x32_bf16)", flush=True)
+    assert 0.0 < tf.value <= 2750.0
