@@ -281,9 +281,10 @@ int nmr_hip_batch_run(nmr_batch* b, int64_t nsteps, double* signal);
  * each looping over tiles; 0: one workgroup per tile). */
 int nmr_hip_batch_set_tile(nmr_batch* b, int32_t variant, int32_t lz, int32_t graph_steps, int32_t persistent);
 /* Times every one-step tile variant (and -3), z-chunk and grid kind on this batch
- * (repeated single steps from the current state, which stays unchanged) and
- * keeps the fastest; the choice is cached per set of instance shapes.  Any
- * output pointer may be NULL. */
+ * (repeated single steps from the current state, which stays unchanged),
+ * times the four fastest again over three times the steps on the warm
+ * device, and keeps the fastest of those; the choice is cached per set of
+ * instance shapes.  Any output pointer may be NULL. */
 int nmr_hip_batch_autotune(nmr_batch* b, int32_t trial_steps, int32_t* variant, int32_t* lz,
                            int32_t* persistent);
 /* GPU ms of the last run's steps and the voxel-steps they advanced. */
