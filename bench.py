@@ -32,15 +32,17 @@ BYTES_PER_VOXEL_STEP = 29.0  # read Mx,My (8) + write Mx,My (8) + mask (1) + vel
 # MacCormack (--scheme maccormack): predictor pass 29 B (as above, writing the
 # predictor) + corrector pass 37 B (reads the predictor and the state, 8 each)
 BYTES_PER_VOXEL_STEP_MAK = 66.0
-# DRAM bytes per sweep launch of the default workload (256^3, phi 0.20,
-# automatic tile, brick layout): rocprofv3 --pmc FETCH_SIZE (x2.0) +
+# Bytes past the L2 per sweep launch of the default workload (256^3, phi
+# 0.20, automatic tile, brick layout): rocprofv3 --pmc FETCH_SIZE (x2.0) +
 # WRITE_SIZE (x1.0) per dispatch, scaled by what the same counters read for a
 # plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
-DEFAULT_TRAFFIC = 223.0e6 + 50.4e6
+# The device's memory-side cache serves part of these requests; the counters
+# at the L2 do not tell which.
+DEFAULT_TRAFFIC = 219.2e6 + 50.4e6
 # The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
 # predictor and the corrector launch of one step together (summary under
 # profiles/, gsweep_maccormack_256cube_flow_pgse.md).
-MAK_TRAFFIC = (222.8e6 + 50.4e6) + (290.5e6 + 50.4e6)
+MAK_TRAFFIC = (219.2e6 + 50.4e6) + (286.7e6 + 50.4e6)
 
 
 def _cpu_model() -> str:

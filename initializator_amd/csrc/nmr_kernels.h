@@ -271,8 +271,10 @@ int num_variants();
 int compact_variant();
 // steps: time steps per pass (1, or 2 for the temporally blocked sweep)
 bool variant_shape(int v, int* lx, int* rpt, int* tx, int* ty, int* steps);
+// keep: the state's loads and stores go without the non-temporal hint (the
+// brick 64 x 16 tile, variant 4; other tiles ignore it)
 hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s,
-                        bool brick = false);
+                        bool brick = false, bool keep = false);
 // whether tile variant v has a brick-layout sweep, and the automatic one
 bool brick_variant(int v);
 int brick_auto_variant(int nx);
@@ -302,8 +304,9 @@ hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
 // lanes per tile row of the general sweep for a grid of width nx (tile 4*lanes wide)
 int gsweep_lanes(int nx);
+// keep: as for launch_sweep (brick sweeps without coefficients or phase weights)
 hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
-                         bool brick = false);
+                         bool brick = false, bool keep = false);
 // m[q][i] = 0 where mask[i] == 0, i < n, for q < nc
 hipError_t launch_zero_solids(float* const* m, int nc, const uint8_t* mask, size_t n, hipStream_t s);
 // the same for fields in bricks (nz planes of ny rows of pitch voxels)

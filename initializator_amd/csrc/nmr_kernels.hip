@@ -67,6 +67,22 @@ __device__ __forceinline__ uint32_t ldw_nt(const uint8_t* base, uint32_t boff)
 {
     return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(base + boff));
 }
+// Loads and stores of the state: with KEEP, without the non-temporal hint,
+// so that the device's memory-side cache (256 MB) keeps what one pass
+// writes for the next -- worth it while the state's two buffers fit about
+// twice that (the host picks KEEP); past that the hint pays.
+template <bool KEEP>
+__device__ __forceinline__ f4 ld4_state(const float* base, uint32_t boff)
+{
+    if constexpr (KEEP) return ld4(base, boff);
+    else return ld4_nt(base, boff);
+}
+template <bool KEEP>
+__device__ __forceinline__ void st4_state(float* base, uint32_t boff, f4 v)
+{
+    if constexpr (KEEP) *reinterpret_cast<f4*>(reinterpret_cast<char*>(base) + boff) = v;
+    else st4_nt(base, boff, v);
+}
 
 // Harmonic mean 2ab/(a + b) of two (scaled) diffusivities for the face
 // between them, with the hardware reciprocal (v_rcp_f32, about 1 ulp) in
@@ -446,7 +462,7 @@ __device__ __forceinline__ void block_reduce(double (&acc)[NC], double (*sred)[N
 // of 4 rows, so that a DRAM line holds a compact block of voxels (all-solid
 // lines are then never fetched) and a wavefront of 16 lanes x 4 rows reads
 // whole lines.
-template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
+template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false, bool KEEP = false>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, double (&acc)[NC])
 {
     constexpr int NT = 256;
@@ -559,7 +575,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
         for (int r = 0; r < RPT; r++) {
             if (rin[r] && (!skip_m || kw[r] != 0)) {
 #pragma unroll
-                for (int q = 0; q < NC; q++) m[r][q] = ld4_nt(a.min[q], pf + rowf[r]);
+                for (int q = 0; q < NC; q++) m[r][q] = ld4_state<KEEP>(a.min[q], pf + rowf[r]);
             } else {
 #pragma unroll
                 for (int q = 0; q < NC; q++) m[r][q] = f4{0.f, 0.f, 0.f, 0.f};
@@ -776,7 +792,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int t, doub
             if (trim_z && trim_y) accumulate<NC>(acc, out, kc[r], trim_x);
             const uint32_t of = zo(k) + rowf[r];
 #pragma unroll
-            for (int q = 0; q < NC; q++) st4_nt(a.mout[q], of, out[q]);
+            for (int q = 0; q < NC; q++) st4_state<KEEP>(a.mout[q], of, out[q]);
         }
 
         // rotate the register planes and the LDS slots
@@ -1123,14 +1139,14 @@ __global__ void __launch_bounds__(256) csweep_kernel(SweepArgs a)
     block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
-template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false>
+template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false, bool KEEP = false>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
     __shared__ double sred[4][NC];
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) acc[q] = 0.0;
-    sweep_tile<NC, VEL, EXACT, LX, RPT, BRK>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    sweep_tile<NC, VEL, EXACT, LX, RPT, BRK, KEEP>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
     block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
@@ -1816,7 +1832,7 @@ __device__ __forceinline__ f4 fill_tail(f4 v, int last, float g)
 // every voxel that is updated.  Ghost rows and columns are staged in LDS by
 // the threads that own the tile's halo; ghost planes are loaded in place of
 // the out-of-range planes.
-template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false>
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false, bool KEEP = false>
 __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, double (&acc)[W ? 2 * NC : NC])
 {
     constexpr int NT = 256;
@@ -1950,7 +1966,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         if (xin && zs >= 0 && (!skip_m || kw != 0)) {
             const uint32_t of = zo(zs) + rowf;
 #pragma unroll
-            for (int q = 0; q < NC; q++) m[q] = ld4_nt(a.f[q], of);
+            for (int q = 0; q < NC; q++) m[q] = ld4_state<KEEP>(a.f[q], of);
         } else {
 #pragma unroll
             for (int q = 0; q < NC; q++) m[q] = f4{0.f, 0.f, 0.f, 0.f};
@@ -2076,7 +2092,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             }
             if constexpr (ST == GEN_CORRECT) {
 #pragma unroll
-                for (int q = 0; q < NC; q++) mav[q] = ld4_nt(a.mavg[q], pz + rowf);
+                for (int q = 0; q < NC; q++) mav[q] = ld4_state<KEEP>(a.mavg[q], pz + rowf);
             }
         }
     };
@@ -2189,7 +2205,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
             }
             const uint32_t of = zo(k) + rowf;
 #pragma unroll
-            for (int q = 0; q < NC; q++) st4_nt(a.out[q], of, out[q]);
+            for (int q = 0; q < NC; q++) st4_state<KEEP>(a.out[q], of, out[q]);
         }
 
         // rotate the register planes; ghost fixes of the planes loaded above
@@ -2224,7 +2240,7 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
 // the corrector is asked to fit 3 (168 VGPRs, at most one register spilled:
 // 188.5 against 194.1 us per MacCormack step at 256^3).  Three components
 // would spill dozens and keep 2.
-template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false>
+template <int NC, bool VEL, int ST, bool COEF, int LX, bool BRK = false, bool W = false, bool KEEP = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu((COEF && !W && ST == GEN_CORRECT && NC <= 2) ? 3 : 1))) gsweep_kernel(GSweepArgs a)
 {
@@ -2233,7 +2249,7 @@ __attribute__((amdgpu_waves_per_eu((COEF && !W && ST == GEN_CORRECT && NC <= 2) 
     double acc[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) acc[q] = 0.0;
-    gsweep_tile<NC, VEL, ST, COEF, LX, BRK, W>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
+    gsweep_tile<NC, VEL, ST, COEF, LX, BRK, W, KEEP>(a, xcd_tile(blockIdx.x, a.nblocks), acc);
     if (ST != GEN_PREDICT && a.partials) block_reduce<NS, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NS);
 }
 
@@ -2417,9 +2433,13 @@ bool brick_variant(int v) { return (v >= 0 && v <= 6) || v == kCompactVariant; }
 int brick_auto_variant(int nx) { return nx > 32 ? 4 : 6; }
 
 template <int NC, bool VEL, bool EXACT>
-static hipError_t launch_nc_brick(int v, const SweepArgs& a, hipStream_t s)
+static hipError_t launch_nc_brick(int v, const SweepArgs& a, hipStream_t s, bool keep)
 {
     const dim3 grid(a.nblocks), block(256);
+    if (keep && v == 4) { // the brick tile the round model picks; other shapes keep the hint
+        hipLaunchKernelGGL((sweep_kernel<NC, VEL, EXACT, 16, 1, true, true>), grid, block, 0, s, a);
+        return hipGetLastError();
+    }
     if (v == kCompactVariant) {
         hipLaunchKernelGGL((csweep_kernel<NC, VEL, EXACT>), grid, block, 0, s, a);
         return hipGetLastError();
@@ -2454,19 +2474,24 @@ static hipError_t launch_nc(int v, const SweepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s, bool brick)
+hipError_t launch_sweep(int nc, bool vel, bool exact, int variant, const SweepArgs& a, hipStream_t s, bool brick,
+                        bool keep)
 {
     if (!brick && variant == kCompactVariant) variant = 4; // the same tile in rows
     if (brick) {
         if (nc == 1)
-            return vel ? launch_nc_brick<1, true, false>(variant, a, s) : launch_nc_brick<1, false, false>(variant, a, s);
+            return vel ? launch_nc_brick<1, true, false>(variant, a, s, keep)
+                       : launch_nc_brick<1, false, false>(variant, a, s, keep);
         if (nc == 2) {
             if (exact)
-                return vel ? launch_nc_brick<2, true, true>(variant, a, s) : launch_nc_brick<2, false, true>(variant, a, s);
-            return vel ? launch_nc_brick<2, true, false>(variant, a, s) : launch_nc_brick<2, false, false>(variant, a, s);
+                return vel ? launch_nc_brick<2, true, true>(variant, a, s, keep)
+                           : launch_nc_brick<2, false, true>(variant, a, s, keep);
+            return vel ? launch_nc_brick<2, true, false>(variant, a, s, keep)
+                       : launch_nc_brick<2, false, false>(variant, a, s, keep);
         }
         if (nc == 3)
-            return vel ? launch_nc_brick<3, true, false>(variant, a, s) : launch_nc_brick<3, false, false>(variant, a, s);
+            return vel ? launch_nc_brick<3, true, false>(variant, a, s, keep)
+                       : launch_nc_brick<3, false, false>(variant, a, s, keep);
         return hipErrorInvalidValue;
     }
     if (nc == 1) return vel ? launch_nc<1, true, false>(variant, a, s) : launch_nc<1, false, false>(variant, a, s);
@@ -2725,11 +2750,28 @@ static hipError_t launch_gs(int stage, int lanes, const GSweepArgs& a, hipStream
 }
 
 template <int NC, bool VEL, bool COEF>
-static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
+static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s, bool keep)
 {
     const dim3 grid(a.nblocks), block(256);
     const bool w = a.w[0] != nullptr; // fused phase-weighted sums
     if (w && !a.w[1]) return hipErrorInvalidValue;
+    if constexpr (!COEF) { // the per-voxel coefficient arrays would not fit beside the state
+        if (keep && !w) {
+            switch (stage) {
+            case GEN_UPWIND:
+                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, false, true>), grid, block, 0, s, a);
+                break;
+            case GEN_PREDICT:
+                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true, false, true>), grid, block, 0, s, a);
+                break;
+            case GEN_CORRECT:
+                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, false, true>), grid, block, 0, s, a);
+                break;
+            default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+    }
     switch (stage) {
     case GEN_UPWIND:
         if (w) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, true>), grid, block, 0, s, a);
@@ -2746,14 +2788,15 @@ static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s)
 }
 
 hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
-                         bool brick)
+                         bool brick, bool keep)
 {
     if (brick) { // lanes 16; the per-voxel arrays are read from their brick copies
         if (lanes != 16) return hipErrorInvalidValue;
 #define NMR_GSB(NC_)                                                                         \
-    if (coef) return vel ? launch_gs_brick<NC_, true, true>(stage, a, s)                      \
-                         : launch_gs_brick<NC_, false, true>(stage, a, s);                    \
-    return vel ? launch_gs_brick<NC_, true, false>(stage, a, s) : launch_gs_brick<NC_, false, false>(stage, a, s);
+    if (coef) return vel ? launch_gs_brick<NC_, true, true>(stage, a, s, keep)                \
+                         : launch_gs_brick<NC_, false, true>(stage, a, s, keep);              \
+    return vel ? launch_gs_brick<NC_, true, false>(stage, a, s, keep)                         \
+               : launch_gs_brick<NC_, false, false>(stage, a, s, keep);
         switch (nc) {
         case 1: NMR_GSB(1)
         case 2: NMR_GSB(2)
