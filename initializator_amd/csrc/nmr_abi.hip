@@ -107,7 +107,7 @@ void free_all(nmr_ctx* c)
             if (c->m[b][q]) (void)hipFree(c->m[b][q]);
     for (int q = 0; q < 3; q++)
         if (c->pred[q]) (void)hipFree(c->pred[q]);
-    for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1], c->vxb, c->vyb, c->vzb})
+    for (float* q : {c->dn, c->r1v, c->r2v, c->bn, c->w[0], c->w[1]})
         if (q) (void)hipFree(q);
     for (int q = 0; q < 3; q++)
         for (float* f : {c->adv[q], c->advb[q], c->csum[q], c->csumb[q]})
@@ -487,7 +487,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
     if (fused && (rc = ensure_partials(ctx, (size_t)std::min<int64_t>(nsteps, kChunkSteps) * g.nwg * ns))) return rc;
     if ((rc = gbrick ? state_bricks(ctx) : nmri::state_rows(ctx))) return rc;
     if (gbrick && (rc = ensure_pv_bricks(ctx))) return rc;
-    if (mak && !pervoxel && (rc = velocity_sums(ctx, gbrick, true))) return rc;
+    if (!pervoxel && (rc = velocity_sums(ctx, gbrick, mak))) return rc;
     const bool have0 = ctx->sig_valid;
     double sig0[6];
     std::copy(ctx->sig_now, ctx->sig_now + 6, sig0);
@@ -545,9 +545,6 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         signal_window(p, w);
         nmr::GSweepArgs a{};
         a.mask = ctx->mask;
-        a.vx = gbrick ? ctx->vxb : ctx->vx;
-        a.vy = gbrick ? ctx->vyb : ctx->vy;
-        a.vz = gbrick ? ctx->vzb : ctx->vz;
         // per-voxel arrays: their brick copies on bricks
         a.dn = ctx->dn ? (gbrick ? ctx->pvb[0] : ctx->dn) : nullptr;
         a.r1v = ctx->r1v ? (gbrick ? ctx->pvb[1] : ctx->r1v) : nullptr;
@@ -573,8 +570,8 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
         a.skip_m = ctx->inert;
         a.amask = (p.flags & NMR_F_ADVECT_MASK) ? 1 : 0;
         a.nby = (p.ny + 3) / 4;
-        a.nbyv = (p.ny + 4) / 4;
-        for (int q = 0; q < 3; q++) a.cs[q] = gbrick ? ctx->csumb[q] : ctx->csum[q];
+        for (int q = 0; q < 3; q++)
+            a.cs[q] = mak ? (gbrick ? ctx->csumb[q] : ctx->csum[q]) : (gbrick ? ctx->advb[q] : ctx->adv[q]);
         const bool coef = ctx->hetero;
         const bool keep = gbrick && keep_state(ctx);
         int64_t done = 0;
@@ -642,7 +639,7 @@ bool bricks_fit(const nmr_ctx* c)
 int ensure_bricks(nmr_ctx* c)
 {
     const nmr_problem& p = c->prob;
-    const int nbx = c->pitch / 4, nbx1 = c->pitch1 / 4;
+    const int nbx = c->pitch / 4;
     for (int b = 0; b < 2; b++)
         for (int q = 0; q < c->nc; q++)
             if (!c->mb[b][q]) {
@@ -655,20 +652,6 @@ int ensure_bricks(nmr_ctx* c)
                 int rc = dev_alloc(c, &c->pb[q], nmr::brick_floats(nbx, p.ny, p.nz));
                 if (rc) return rc;
             }
-    // the face velocities in bricks serve the general sweep's upwind stage
-    if (!c->general || p.scheme == NMR_SCHEME_MACCORMACK) return NMR_OK;
-    if (c->vel && !c->vxb) {
-        int rc = dev_alloc(c, &c->vxb, nmr::brick_floats(nbx1, p.ny, p.nz));
-        if (!rc) rc = dev_alloc(c, &c->vyb, nmr::brick_floats(nbx, p.ny + 1, p.nz));
-        if (!rc) rc = dev_alloc(c, &c->vzb, nmr::brick_floats(nbx, p.ny, p.nz + 1));
-        if (rc) return rc;
-    }
-    if (c->vel && !c->brick_vel_valid) {
-        HIP_TRY(nmr::launch_brick(c->vx, c->vxb, nbx1, p.ny, p.nz, true, c->stream));
-        HIP_TRY(nmr::launch_brick(c->vy, c->vyb, nbx, p.ny + 1, p.nz, true, c->stream));
-        HIP_TRY(nmr::launch_brick(c->vz, c->vzb, nbx, p.ny, p.nz + 1, true, c->stream));
-        c->brick_vel_valid = true;
-    }
     return NMR_OK;
 }
 
@@ -791,10 +774,9 @@ int prepare(nmr_ctx* c)
         if ((rc = ensure_bricks(c))) return rc;
         if ((rc = ensure_pv_bricks(c))) return rc;
     }
-    if (!c->general && (rc = ensure_adv(c, c->brick_on && bricks_fit(c)))) return rc;
-    if (c->general && c->prob.scheme == NMR_SCHEME_MACCORMACK &&
-        (rc = velocity_sums(c, c->brick_on && bricks_fit(c), true)))
-        return rc;
+    // the per-voxel velocity sums the sweeps read: upwind (the plain sweep,
+    // the general sweep's upwind stage) or centred (MacCormack)
+    if ((rc = velocity_sums(c, c->brick_on && bricks_fit(c), c->prob.scheme == NMR_SCHEME_MACCORMACK))) return rc;
     if ((rc = presize_buffers(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return NMR_OK;
@@ -1057,7 +1039,6 @@ int nmr_hip_upload_velocity(nmr_ctx* ctx, const double* vx, const double* vy, co
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     ctx->inert_dirty = true;
-    ctx->brick_vel_valid = false;
     ctx->adv_valid = ctx->advb_valid = false;
     ctx->csum_valid = ctx->csumb_valid = false;
     return prepare(ctx);

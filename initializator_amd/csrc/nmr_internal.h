@@ -67,9 +67,7 @@ struct nmr_ctx {
     // the sweeps in 128-byte bricks (8 x-voxels by 4 rows): state copies and velocity
     bool brick_on = true;       // nmr_hip_set_layout
     float* mb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    float *vxb = nullptr, *vyb = nullptr, *vzb = nullptr; // face velocities (general sweep)
     float* pb[3] = {nullptr, nullptr, nullptr}; // MacCormack predictor in bricks
-    bool brick_vel_valid = false;
     // upwind velocity sums of the plain sweep (nmr::launch_adv), in rows
     // and, for brick sweeps, in bricks; computed once per velocity upload
     float* adv[3] = {nullptr, nullptr, nullptr};

@@ -134,9 +134,6 @@ struct GSweepArgs {
     const float* mavg[3];   // corrector stage: the state at the start of the step
     float* out[3];          // result of the stage
     const uint8_t* mask;
-    const float* vx;
-    const float* vy;
-    const float* vz;
     const float* dn;        // per-voxel diffusivity / reference (coefficient variant)
     const float* r1v;       // per-voxel dt/T1
     const float* r2v;       // per-voxel dt/T2
@@ -145,15 +142,15 @@ struct GSweepArgs {
     double* partials;       // [nblocks][nc] pore sums of the result, or null
     const float* w[2];      // brick sweep: phase weights (row layout) of fused weighted sums,
                             // partials then [nblocks][2*nc]; or null
-    const float* cs[3];     // MacCormack stages: face-velocity sums per voxel (AdvArgs centred),
-                            // laid out as the state; the upwind stage reads vx, vy, vz
+    const float* cs[3];     // face-velocity sums per voxel, laid out as the state: centred
+                            // (MacCormack stages) or upwind (the upwind stage), AdvArgs
     int nx, ny, nz, pitch, pitch1;
     int ntx, nty, ntz, lz, nblocks;
     int bx0, bx1, by0, by1, bz0, bz1; // signal window [b0, b1) per axis
     int bc[3];              // BC_* per axis
     int skip_m;             // all-solid chunks are never read by a pore: not loaded
     int amask;              // upwind advection only across faces to pores (NMR_F_ADVECT_MASK)
-    int nby, nbyv;          // brick layout (lanes 16): brick rows per plane of M / Vx / Vz and of Vy
+    int nby;                // brick layout (lanes 16): brick rows per plane
     Coef c;
 };
 

@@ -1690,22 +1690,23 @@ __global__ void __launch_bounds__(256) field_sum_kernel(FieldSumArgs a)
 // COEF, harmonic means of per-voxel diffusivities; ghost neighbours already
 // hold their rule's value and the pore mask.  dxl / dxr: diffusivity of the x
 // neighbours of elements 0 and 3; mav: the state at the start of the step
-// (corrector stage).  SUMS (MacCormack stages only): vx4, vy4, vz0 carry the
-// voxels' face-velocity sums um + up, vm + vp, wm + wp (adv_kernel, centred)
-// and the other velocity arguments are not read.
-template <int NC, bool VEL, int ST, bool COEF, bool SUMS = false>
+// (corrector stage).  vx4, vy4, vz0: the voxels' face-velocity sums per
+// axis (adv_kernel): for the MacCormack stages the plain sums um + up (the
+// centred velocity, doubled), for the upwind stage the upwind sums (um + up
+// when the inflow face carries velocity, else 0), which give the face
+// tests' bits.
+template <int NC, bool VEL, int ST, bool COEF>
 __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC], const f4 (&fzm)[NC],
                                               const f4 (&fzp)[NC], const f4 (&fym)[NC], const f4 (&fyp)[NC],
                                               const float (&xl)[NC], const float (&xr)[NC], uint32_t kc,
                                               uint32_t kxm, uint32_t kxp, uint32_t kym, uint32_t kyp,
-                                              uint32_t kzm, uint32_t kzp, const f4& vx4, float vxe, const f4& vy4,
-                                              const f4& vy4n, const f4& vz0, const f4& vz1, const f4& dc,
+                                              uint32_t kzm, uint32_t kzp, const f4& vx4, const f4& vy4,
+                                              const f4& vz0, const f4& dc,
                                               float dxl, float dxr, const f4& dym, const f4& dyp, const f4& dzm,
                                               const f4& dzp, bool r1p, const f4& r1c, const f4& r2c,
                                               const f4 (&mav)[NC], bool amask, int ix, float thy, float thz,
                                               const f4& bnc, int bnax, float dtg, f4 (&out)[NC])
 {
-    static_assert(!SUMS || ST != GEN_UPWIND, "velocity sums serve the MacCormack stages");
     const float hxh = 0.5f * cf.hx, hyh = 0.5f * cf.hy, hzh = 0.5f * cf.hz;
     const float cd[3] = {cf.cdx, cf.cdy, cf.cdz};
 #pragma unroll
@@ -1724,8 +1725,19 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
 #pragma unroll
             for (int s = 0; s < 6; s++) w[s] = cd[s / 2] * kn[s];
         }
-        if constexpr (VEL && SUMS) {
-            if constexpr (ST == GEN_PREDICT) {
+        if constexpr (VEL) {
+            if constexpr (ST == GEN_UPWIND) {
+                // with NMR_F_ADVECT_MASK only across faces to a pore
+                bool open[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) open[q] = !amask || kn[q] != 0.f;
+                w[0] = open[0] ? fmaf(hxh, fmaxf(vx4[e], 0.f), w[0]) : w[0];
+                w[1] = open[1] ? fmaf(-hxh, fminf(vx4[e], 0.f), w[1]) : w[1];
+                w[2] = open[2] ? fmaf(hyh, fmaxf(vy4[e], 0.f), w[2]) : w[2];
+                w[3] = open[3] ? fmaf(-hyh, fminf(vy4[e], 0.f), w[3]) : w[3];
+                w[4] = open[4] ? fmaf(hzh, fmaxf(vz0[e], 0.f), w[4]) : w[4];
+                w[5] = open[5] ? fmaf(-hzh, fminf(vz0[e], 0.f), w[5]) : w[5];
+            } else if constexpr (ST == GEN_PREDICT) {
                 w[0] = fmaf(kn[0] * hxh, vx4[e], w[0]);
                 w[2] = fmaf(kn[2] * hyh, vy4[e], w[2]);
                 w[4] = fmaf(kn[4] * hzh, vz0[e], w[4]);
@@ -1733,30 +1745,6 @@ __device__ __forceinline__ void gupdate_chunk(const Coef& cf, const f4 (&mc)[NC]
                 w[1] = fmaf(-kn[1] * hxh, vx4[e], w[1]);
                 w[3] = fmaf(-kn[3] * hyh, vy4[e], w[3]);
                 w[5] = fmaf(-kn[5] * hzh, vz0[e], w[5]);
-            }
-        } else if constexpr (VEL) {
-            const float um = vx4[e], up = (e < 3) ? vx4[e + 1] : vxe;
-            const float vm = vy4[e], vp = vy4n[e];
-            const float wm = vz0[e], wp = vz1[e];
-            const float su = um + up, sv = vm + vp, sw = wm + wp;
-            if constexpr (ST == GEN_UPWIND) {
-                bool open[6];
-#pragma unroll
-                for (int s = 0; s < 6; s++) open[s] = !amask || kn[s] != 0.f;
-                w[0] = (um != 0.f && open[0]) ? fmaf(hxh, fmaxf(su, 0.f), w[0]) : w[0];
-                w[1] = (up != 0.f && open[1]) ? fmaf(-hxh, fminf(su, 0.f), w[1]) : w[1];
-                w[2] = (vm != 0.f && open[2]) ? fmaf(hyh, fmaxf(sv, 0.f), w[2]) : w[2];
-                w[3] = (vp != 0.f && open[3]) ? fmaf(-hyh, fminf(sv, 0.f), w[3]) : w[3];
-                w[4] = (wm != 0.f && open[4]) ? fmaf(hzh, fmaxf(sw, 0.f), w[4]) : w[4];
-                w[5] = (wp != 0.f && open[5]) ? fmaf(-hzh, fminf(sw, 0.f), w[5]) : w[5];
-            } else if constexpr (ST == GEN_PREDICT) {
-                w[0] = fmaf(kn[0] * hxh, su, w[0]);
-                w[2] = fmaf(kn[2] * hyh, sv, w[2]);
-                w[4] = fmaf(kn[4] * hzh, sw, w[4]);
-            } else {
-                w[1] = fmaf(-kn[1] * hxh, su, w[1]);
-                w[3] = fmaf(-kn[3] * hyh, sv, w[3]);
-                w[5] = fmaf(-kn[5] * hzh, sw, w[5]);
             }
         }
         const float nsx = 2.f - kn[0] - kn[1], nsy = 2.f - kn[2] - kn[3], nsz = 2.f - kn[4] - kn[5];
@@ -1872,32 +1860,23 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     const uint32_t Pb = (uint32_t)a.pitch * 4u;
     const uint32_t plane_b = (uint32_t)ny * Pb;
     const uint32_t plane_m = (uint32_t)ny * (uint32_t)a.pitch;
-    const uint32_t P1b = (uint32_t)a.pitch1 * 4u;
-    const uint32_t planex_b = (uint32_t)ny * P1b;
-    const uint32_t planey_b = (uint32_t)(ny + 1) * Pb;
     // byte offsets along x, y, z of the float arrays (row layout, or with
     // BRK the 128-byte bricks of 8 x-voxels by 4 rows of the plain sweep);
     // the per-voxel arrays (coefficients, field, phase weights) are laid out
     // as the state
-    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3, nbx1 = ((uint32_t)a.pitch1 + 7u) >> 3;
+    const uint32_t nbx = ((uint32_t)a.pitch + 7u) >> 3;
     auto xo = [&](int x) -> uint32_t {
         return BRK ? ((uint32_t)(x >> 3) << 7) + ((uint32_t)(x & 7) << 2) : (uint32_t)x * 4u;
     };
     auto yo = [&](int y) -> uint32_t {
         return BRK ? (uint32_t)(y >> 2) * nbx * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * Pb;
     };
-    auto yo1 = [&](int y) -> uint32_t {
-        return BRK ? (uint32_t)(y >> 2) * nbx1 * 128u + ((uint32_t)(y & 3) << 5) : (uint32_t)y * P1b;
-    };
     auto zo = [&](int z) -> uint32_t {
         return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx * 128u : (uint32_t)z * plane_b;
     };
-    auto zo1 = [&](int z) -> uint32_t { return BRK ? (uint32_t)z * (uint32_t)a.nby * nbx1 * 128u : (uint32_t)z * planex_b; };
-    auto zov = [&](int z) -> uint32_t { return BRK ? (uint32_t)z * (uint32_t)a.nbyv * nbx * 128u : (uint32_t)z * planey_b; };
     const int yc = iy < ny ? iy : ny - 1; // rows past ny never load
     const uint32_t rowf = yo(yc) + xo(xin ? ix : 0);
     const uint32_t rowm = (uint32_t)yc * (uint32_t)a.pitch + (uint32_t)(xin ? ix : 0);
-    const uint32_t rowx = yo1(yc) + xo(xin ? ix : 0);
 
     // halo / ghost row: the top thread stages row y0-1, the bottom thread
     // row y0+TY, or the ghost row ny at LDS row 1 + (ny - y0) when the tile
@@ -2054,9 +2033,8 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
         }
     };
 
-    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, vy4n = vx4, wc = vx4, wn = vx4, r1c = vx4, r2c = vx4, bnc = vx4;
+    f4 vx4 = f4{0.f, 0.f, 0.f, 0.f}, vy4 = vx4, wc = vx4, r1c = vx4, r2c = vx4, bnc = vx4;
     f4 pw1 = vx4, pw2 = vx4; // W: phase weights of the own chunk
-    float vxe = 0.f;
     const int bnax = a.bn ? a.bnax : -1;
     f4 mav[NC];
 #pragma unroll
@@ -2065,18 +2043,10 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
     auto load_own = [&](int kk, uint32_t kw) {
         if (rin && kw != 0) {
             const uint32_t pz = zo(kk);
-            if constexpr (VEL && ST != GEN_UPWIND) { // the voxels' face-velocity sums
+            if constexpr (VEL) { // the voxels' face-velocity sums
                 vx4 = ld4_nt(a.cs[0], pz + rowf);
                 vy4 = ld4_nt(a.cs[1], pz + rowf);
                 wc = ld4_nt(a.cs[2], pz + rowf);
-            } else if constexpr (VEL) {
-                const uint32_t px = zo1(kk), py = zov(kk);
-                vx4 = ld4_nt(a.vx, px + rowx);
-                vxe = (xlast >= 3) ? ld1(a.vx, px + yo1(yc) + xo(ix + 4)) : 0.f;
-                vy4 = ld4_nt(a.vy, py + rowf);
-                vy4n = ld4(a.vy, py + yo(yc + 1) + xo(ix));
-                wc = ld4(a.vz, pz + rowf);
-                wn = ld4(a.vz, zo(kk + 1) + rowf);
             }
             if constexpr (COEF) {
                 const uint32_t pc = zo(kk) + rowf;
@@ -2194,8 +2164,8 @@ __device__ __forceinline__ void gsweep_tile(const GSweepArgs& a, const int t, do
                 dxr = sd[buf][lr][c + 4];
             }
             f4 out[NC];
-            gupdate_chunk<NC, VEL, ST, COEF, ST != GEN_UPWIND>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, vx4,
-                                             vxe, vy4, vy4n, wc, wn, dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
+            gupdate_chunk<NC, VEL, ST, COEF>(cf, mc, mp, mn, ym, yp, xl, xr, kc, kxm, kxp, kym, kyp, kp, kn, vx4, vy4, wc,
+                                             dcv, dxl, dxr, dym, dyp, dp, dnv, a.r1v != nullptr, r1c,
                                              r2c, mav, a.amask != 0,
                                              ix, thy, thz, bnc, bnax, cf.dtg, out);
             if constexpr (W) {
