@@ -2725,22 +2725,20 @@ static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s,
     const dim3 grid(a.nblocks), block(256);
     const bool w = a.w[0] != nullptr; // fused phase-weighted sums
     if (w && !a.w[1]) return hipErrorInvalidValue;
-    if constexpr (!COEF) { // the per-voxel coefficient arrays would not fit beside the state
-        if (keep && !w) {
-            switch (stage) {
-            case GEN_UPWIND:
-                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, false, true>), grid, block, 0, s, a);
-                break;
-            case GEN_PREDICT:
-                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true, false, true>), grid, block, 0, s, a);
-                break;
-            case GEN_CORRECT:
-                hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, false, true>), grid, block, 0, s, a);
-                break;
-            default: return hipErrorInvalidValue;
-            }
-            return hipGetLastError();
+    if (keep && !w) { // the state without the non-temporal hint
+        switch (stage) {
+        case GEN_UPWIND:
+            hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, false, true>), grid, block, 0, s, a);
+            break;
+        case GEN_PREDICT:
+            hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true, false, true>), grid, block, 0, s, a);
+            break;
+        case GEN_CORRECT:
+            hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, false, true>), grid, block, 0, s, a);
+            break;
+        default: return hipErrorInvalidValue;
         }
+        return hipGetLastError();
     }
     switch (stage) {
     case GEN_UPWIND:
