@@ -198,7 +198,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "nmr_hip_upload_coefficients": (c_int, [vp, dp, dp, dp, c_int32]),
         "nmr_hip_upload_field": (c_int, [vp, dp, c_int32]),
         "nmr_hip_upload_phase_weights": (c_int, [vp, dp, dp, c_int32]),
-        "nmr_hip_run": (c_int, [vp, c_int64, dp, dp]),
+        # the hot call takes plain addresses (HipSolver.run passes integers)
+        "nmr_hip_run": (c_int, [vp, c_int64, vp, vp]),
         "nmr_hip_signal": (c_int, [vp, dp]),
         "nmr_hip_copy_bandwidth": (c_int, [c_int64, c_int32, dp]),
         "nmr_hip_mfma_rate": (c_int, [c_int64, c_int32, dp]),
