@@ -473,7 +473,7 @@ int run_general(nmr_ctx* ctx, int64_t nsteps, const double* grad_sched, double* 
             for (int q = 0; q < 3; q++) g[q] = grad_sched[3 * step + q];
         return make_coef(ctx, g);
     };
-    // the general sweep runs on the bricks (state, velocity, per-voxel
+    // the general sweep runs on the bricks (state, velocity sums, per-voxel
     // arrays and the MacCormack predictor) whenever they are addressable
     const bool gbrick = !pervoxel && ctx->brick_on && bricks_fit(ctx);
     GGeom g{};
@@ -633,9 +633,9 @@ bool bricks_fit(const nmr_ctx* c)
 }
 
 // Brick buffers of the state (both ping-pong buffers and, for MacCormack,
-// the predictor) and, for the general sweep, the brick copy of the face
-// velocities, allocated and converted by the uploads (or the first brick run
-// after the layout switch).
+// the predictor), allocated by the uploads (or the first brick run after
+// the layout switch); the velocity enters the brick sweeps as per-voxel
+// sums (velocity_sums).
 int ensure_bricks(nmr_ctx* c)
 {
     const nmr_problem& p = c->prob;
