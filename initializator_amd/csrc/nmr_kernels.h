@@ -301,7 +301,7 @@ hipError_t launch_pulse(const PulseArgs& a, hipStream_t s);
 hipError_t launch_general(int nc, int stage, const GenArgs& a, hipStream_t s);
 // lanes per tile row of the general sweep for a grid of width nx (tile 4*lanes wide)
 int gsweep_lanes(int nx);
-// keep: as for launch_sweep (brick sweeps without phase weights)
+// keep: as for launch_sweep (the brick sweeps)
 hipError_t launch_gsweep(int nc, bool vel, int stage, bool coef, int lanes, const GSweepArgs& a, hipStream_t s,
                          bool brick = false, bool keep = false);
 // m[q][i] = 0 where mask[i] == 0, i < n, for q < nc

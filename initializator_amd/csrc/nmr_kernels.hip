@@ -2725,16 +2725,18 @@ static hipError_t launch_gs_brick(int stage, const GSweepArgs& a, hipStream_t s,
     const dim3 grid(a.nblocks), block(256);
     const bool w = a.w[0] != nullptr; // fused phase-weighted sums
     if (w && !a.w[1]) return hipErrorInvalidValue;
-    if (keep && !w) { // the state without the non-temporal hint
+    if (keep) { // the state without the non-temporal hint
         switch (stage) {
         case GEN_UPWIND:
-            hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, false, true>), grid, block, 0, s, a);
+            if (w) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, true, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_UPWIND, COEF, 16, true, false, true>), grid, block, 0, s, a);
             break;
         case GEN_PREDICT:
             hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_PREDICT, COEF, 16, true, false, true>), grid, block, 0, s, a);
             break;
         case GEN_CORRECT:
-            hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, false, true>), grid, block, 0, s, a);
+            if (w) hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, true, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((gsweep_kernel<NC, VEL, GEN_CORRECT, COEF, 16, true, false, true>), grid, block, 0, s, a);
             break;
         default: return hipErrorInvalidValue;
         }
