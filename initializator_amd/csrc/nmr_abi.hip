@@ -184,6 +184,8 @@ int round_lz(long tiles_xy, int nz, long resident)
     return std::min(best, nz);
 }
 
+bool keep_state(const nmr_ctx* c);
+
 void auto_launch(const nmr_ctx* c, int variant, int lz, Launch* L, bool brick = false)
 {
     const nmr_problem& p = c->prob;
@@ -199,15 +201,17 @@ void auto_launch(const nmr_ctx* c, int variant, int lz, Launch* L, bool brick = 
     L->lz = 1;
     fill_launch(p, L);
     if (lz <= 0) {
-        // one-step: 8 planes per z-chunk (measured best from 256^3 to
-        // 512x512x1024; longer chunks leave fewer, longer workgroups);
+        // one-step: the z-chunk that best fills whole rounds of the
+        // workgroups the device holds at once (of the kernel the run
+        // launches: with the state kept cached, 4 per CU instead of 3);
         // two-step (one 8-wave workgroup per CU): >= 512 workgroups
         if (L->steps == 2) {
             const long tiles_xy = (long)L->ntx * L->nty;
             const long ntz = std::max<long>(1, (512 + tiles_xy - 1) / tiles_xy);
             lz = (int)std::max<long>(4, (p.nz + ntz - 1) / ntz);
         } else {
-            const int res = nmr::sweep_resident_blocks(c->nc, c->vel, c->exact, variant, brick);
+            const int res = nmr::sweep_resident_blocks(c->nc, c->vel, c->exact, variant, brick,
+                                                       brick && keep_state(c));
             lz = round_lz((long)L->ntx * L->nty, p.nz, res);
         }
     }

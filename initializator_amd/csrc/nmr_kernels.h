@@ -288,7 +288,7 @@ int batch_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick 
 // the general sweep's upwind kernel (upwind) or MacCormack corrector (on
 // bricks with fused phase-weighted sums: its weighted instantiation)
 // resident on the device at once (CUs x occupancy); -1 if the query fails.
-int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick);
+int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick, bool keep = false);
 int gsweep_resident_blocks(int nc, bool vel, bool coef, int lanes, bool brick, bool weighted, bool upwind);
 hipError_t launch_batch_reduce(const double* part, int ntiles, int nc, const int* off, const int* ids,
                                const int* step_base, int row0, int64_t rows, double* signal, int nsteps, int ninst,

@@ -1139,8 +1139,14 @@ __global__ void __launch_bounds__(256) csweep_kernel(SweepArgs a)
     block_reduce<NC, 256>(acc, sred, a.partials + (size_t)blockIdx.x * NC);
 }
 
+// KEEP (one or two components, linear phase): capped at 128 VGPRs, so that
+// 4 workgroups fit per CU.  With the state served by the memory-side cache
+// the extra wavefront per SIMD pays (256^3: 58.7 against 61.0 us per step,
+// the z-chunk filling the 1 024 slots; phi 0.35: 67.1 against 68.2 us);
+// with the hint it did not.
 template <int NC, bool VEL, bool EXACT, int LX, int RPT, bool BRK = false, bool KEEP = false>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((KEEP && NC <= 2 && !EXACT) ? 4 : 1)))
+sweep_kernel(SweepArgs a)
 {
     __shared__ double sred[4][NC];
     double acc[NC];
@@ -2573,9 +2579,20 @@ int batch_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
     return resident_blocks(f);
 }
 
-int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick)
+int sweep_resident_blocks(int nc, bool vel, bool exact, int variant, bool brick, bool keep)
 {
     const void* f = nullptr;
+    if (keep && brick && variant == 4) { // the kernel launch_sweep runs with keep
+        if (nc == 1) f = vel ? (const void*)sweep_kernel<1, true, false, 16, 1, true, true>
+                             : (const void*)sweep_kernel<1, false, false, 16, 1, true, true>;
+        else if (nc == 2 && exact) f = vel ? (const void*)sweep_kernel<2, true, true, 16, 1, true, true>
+                                           : (const void*)sweep_kernel<2, false, true, 16, 1, true, true>;
+        else if (nc == 2) f = vel ? (const void*)sweep_kernel<2, true, false, 16, 1, true, true>
+                                  : (const void*)sweep_kernel<2, false, false, 16, 1, true, true>;
+        else if (nc == 3) f = vel ? (const void*)sweep_kernel<3, true, false, 16, 1, true, true>
+                                  : (const void*)sweep_kernel<3, false, false, 16, 1, true, true>;
+        return resident_blocks(f);
+    }
     if (variant == kCompactVariant && brick) {
         if (nc == 1) f = vel ? (const void*)csweep_kernel<1, true, false> : (const void*)csweep_kernel<1, false, false>;
         else if (nc == 2 && exact)
