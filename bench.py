@@ -38,7 +38,7 @@ BYTES_PER_VOXEL_STEP_MAK = 66.0
 # plain 1 GiB copy (summary under profiles/, sweep_kernel_256cube_flow_pgse.md).
 # The device's memory-side cache serves part of these requests; the counters
 # at the L2 do not tell which.
-DEFAULT_TRAFFIC = 219.2e6 + 50.4e6
+DEFAULT_TRAFFIC = 226.1e6 + 52.5e6
 # The same for --scheme maccormack at 256^3, phi 0.20 (brick layout): the
 # predictor and the corrector launch of one step together (summary under
 # profiles/, gsweep_maccormack_256cube_flow_pgse.md).
